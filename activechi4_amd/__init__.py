@@ -1,0 +1,388 @@
+"""activechi4_amd: MI355X-native active Kob-Andersen MD + dynamic-heterogeneity
+analysis behind a C ABI (include/ac4.h, libac4.so).
+
+This module is a thin ctypes binding used by the tests and bench.py; the
+product path is libac4.so (hand-written HIP kernels for gfx950) and the C
+drop-in executables built from activechi4_amd/host/.  There is no CPU
+fallback: every compute call goes through libac4.so and fails loudly when the
+library or a GPU is missing.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+__all__ = ["lib", "load", "MD", "Analysis", "ReplicaParams", "AC4Error", "FP64", "FP32",
+           "OBS_MSD", "OBS_OVERLAP", "OBS_S4", "OBS_FS", "NQ", "LIB_PATH"]
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libac4.so")
+FP64, FP32 = 0, 1
+OBS_MSD, OBS_OVERLAP, OBS_S4, OBS_FS = 1, 2, 4, 8
+NQ = 162
+
+_D = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.POINTER(ctypes.c_int)
+_U64 = ctypes.POINTER(ctypes.c_uint64)
+_U32 = ctypes.POINTER(ctypes.c_uint32)
+_VP = ctypes.c_void_p
+
+
+class ReplicaParams(ctypes.Structure):
+    """ac4_replica_params of include/ac4.h."""
+    _fields_ = [("n", ctypes.c_int), ("density", ctypes.c_double), ("T", ctypes.c_double),
+                ("f0", ctypes.c_double), ("n_act", ctypes.c_int), ("tau_per", ctypes.c_int),
+                ("seed", ctypes.c_uint64)]
+
+
+class AC4Error(RuntimeError):
+    pass
+
+
+_SIGS = {
+    "ac4_last_error": (ctypes.c_char_p, []),
+    "ac4_device_count": (ctypes.c_int, []),
+    "ac4_md_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ReplicaParams), ctypes.c_int,
+                                     ctypes.POINTER(_VP)]),
+    "ac4_md_destroy": (ctypes.c_int, [_VP]),
+    "ac4_md_total_particles": (ctypes.c_int, [_VP]),
+    "ac4_md_set_state": (ctypes.c_int, [_VP, ctypes.c_int] + [_D] * 9 + [_I] * 5),
+    "ac4_md_get_state": (ctypes.c_int, [_VP, ctypes.c_int] + [_D] * 9 + [_I] * 3),
+    "ac4_md_get_forces": (ctypes.c_int, [_VP, ctypes.c_int, _D, _D, _D]),
+    "ac4_md_set_chain": (ctypes.c_int, [_VP, ctypes.c_int, _D]),
+    "ac4_md_get_chain": (ctypes.c_int, [_VP, ctypes.c_int, _D]),
+    "ac4_md_update_cell_list": (ctypes.c_int, [_VP, ctypes.c_int, _I, _I]),
+    "ac4_md_calculate_forces": (ctypes.c_int, [_VP, _D]),
+    "ac4_md_step": (ctypes.c_int, [_VP, ctypes.c_int]),
+    "ac4_md_step_graph": (ctypes.c_int, [_VP, ctypes.c_int]),
+    "ac4_md_set_rebuild_criterion": (ctypes.c_int, [_VP, ctypes.c_int]),
+    "ac4_md_relax": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_double, ctypes.c_double]),
+    "ac4_md_shuffle": (ctypes.c_int, [_VP]),
+    "ac4_md_shuffle_uniforms": (ctypes.c_int, [_VP, ctypes.c_int, _D]),
+    "ac4_md_stats": (ctypes.c_int, [_VP, ctypes.c_int, _D]),
+    "ac4_md_rebuilds": (ctypes.c_int, [_VP, ctypes.c_int]),
+    "ac4_md_sync": (ctypes.c_int, [_VP]),
+    "ac4_md_kernel_times": (ctypes.c_int, [_VP, ctypes.c_int, _D]),
+    "ac4_philox4x32": (ctypes.c_int, [_U32, _U32, _U32]),
+    "ac4_ana_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_int, _I, ctypes.c_int,
+                                      ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                      ctypes.POINTER(_VP)]),
+    "ac4_ana_destroy": (ctypes.c_int, [_VP]),
+    "ac4_ana_gap": (ctypes.c_int, [_VP]),
+    "ac4_ana_push_frame": (ctypes.c_int, [_VP, ctypes.c_int, _D, _D, _D, _I]),
+    "ac4_ana_push_frame_device": (ctypes.c_int, [_VP, ctypes.c_int, _VP, _VP, _VP, _VP]),
+    "ac4_ana_push_md": (ctypes.c_int, [_VP, ctypes.c_int, _VP, ctypes.c_int]),
+    "ac4_ana_finalize": (ctypes.c_int, [_VP]),
+    "ac4_ana_nslots": (ctypes.c_int, [_VP, ctypes.c_int]),
+    "ac4_ana_get_msd": (ctypes.c_int, [_VP, _D, _D, _D, _I, _D, _I]),
+    "ac4_ana_get_overlap": (ctypes.c_int, [_VP, _D, _D, _I, _D, _I]),
+    "ac4_ana_get_phase": (ctypes.c_int, [_VP, ctypes.c_int, _D, _D, _D, _D, _I, _D, _D]),
+    "ac4_ana_npairs": (ctypes.c_int, [_VP, ctypes.c_int]),
+    "ac4_ana_get_pairs": (ctypes.c_int, [_VP, ctypes.c_int, _D, _I]),
+    "ac4_ana_write": (ctypes.c_int, [_VP, ctypes.c_char_p]),
+    "ac4_schedule": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int, ctypes.c_double, _U64, ctypes.c_int]),
+    "ac4_origin_gap": (ctypes.c_int, [_I, ctypes.c_int]),
+    "ac4_chi4_ensemble": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _D, _I, _D]),
+    "ac4_init_lattice": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_uint64]
+                         + [_D] * 9 + [_I] * 5),
+}
+
+lib = None
+
+
+def load(path=LIB_PATH):
+    """Load libac4.so (raises when it is missing: there is no fallback)."""
+    global lib
+    if lib is not None:
+        return lib
+    if not os.path.exists(path):
+        raise AC4Error("libac4.so not built at %s (run __graft_entry__.build())" % path)
+    L = ctypes.CDLL(path)
+    for name, (res, args) in _SIGS.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    lib = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        raise AC4Error("libac4 error %d: %s" % (rc, lib.ac4_last_error().decode(errors="replace")))
+    return rc
+
+
+def _dp(a):
+    return None if a is None else a.ctypes.data_as(_D)
+
+
+def _ip(a):
+    return None if a is None else a.ctypes.data_as(_I)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def require_gpu():
+    load()
+    if lib.ac4_device_count() < 1:
+        raise AC4Error("no HIP device visible: libac4 has no CPU path")
+
+
+class MD:
+    """R replicas of the active KA system on one GPU (ac4_md_* of include/ac4.h)."""
+
+    def __init__(self, params, precision=FP64):
+        load()
+        params = list(params)
+        arr = (ReplicaParams * len(params))(*params)
+        h = _VP()
+        _check(lib.ac4_md_create(len(params), arr, precision, ctypes.byref(h)))
+        self.h = h
+        self.params = params
+        self.R = len(params)
+
+    def close(self):
+        if self.h:
+            lib.ac4_md_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def n(self, r=0):
+        return self.params[r].n
+
+    def set_state(self, r, x, y, z, xu, yu, zu, px, py, pz, typ, act_tag=None, kx=None, ky=None, kz=None):
+        a = [_f64(v) for v in (x, y, z, xu, yu, zu, px, py, pz)]
+        t = _i32(typ)
+        at = None if act_tag is None else _i32(act_tag)
+        ks = [None if k is None else _i32(k) for k in (kx, ky, kz)]
+        _check(lib.ac4_md_set_state(self.h, r, *[_dp(v) for v in a], _ip(t), _ip(at), *[_ip(k) for k in ks]))
+
+    def get_state(self, r=0):
+        n = self.n(r)
+        out = [np.empty(n) for _ in range(9)]
+        nact = self.nact(r)
+        ks = [np.empty(max(nact, 1), dtype=np.int32) for _ in range(3)]
+        _check(lib.ac4_md_get_state(self.h, r, *[_dp(v) for v in out], *[_ip(k) for k in ks]))
+        keys = ["x", "y", "z", "xu", "yu", "zu", "px", "py", "pz"]
+        d = dict(zip(keys, out))
+        d["kx"], d["ky"], d["kz"] = [k[:nact] for k in ks]
+        return d
+
+    def nact(self, r=0):
+        p = self.params[r]
+        return int(p.n * 0.1) if p.n_act < 0 else p.n_act
+
+    def forces(self, r=0):
+        n = self.n(r)
+        f = [np.empty(n) for _ in range(3)]
+        _check(lib.ac4_md_get_forces(self.h, r, *[_dp(v) for v in f]))
+        return f
+
+    def set_chain(self, r, chain9):
+        c = _f64(chain9)
+        _check(lib.ac4_md_set_chain(self.h, r, _dp(c)))
+
+    def get_chain(self, r=0):
+        c = np.empty(9)
+        _check(lib.ac4_md_get_chain(self.h, r, _dp(c)))
+        return c
+
+    def update_cell_list(self, r=0):
+        n = self.n(r)
+        cell = np.empty(n, dtype=np.int32)
+        cnt = np.empty(n, dtype=np.int32)
+        _check(lib.ac4_md_update_cell_list(self.h, r, _ip(cell), _ip(cnt)))
+        return cell, cnt
+
+    def calculate_forces(self):
+        out = np.empty(2 * self.R)
+        _check(lib.ac4_md_calculate_forces(self.h, _dp(out)))
+        return out.reshape(self.R, 2)
+
+    def step(self, nsteps, graph=False):
+        _check((lib.ac4_md_step_graph if graph else lib.ac4_md_step)(self.h, nsteps))
+
+    def relax(self, nsteps, f_start=1.5, f_end=1.0):
+        """Velocity-rescale ramp from f_start*T to f_end*T (ac4_md_relax)."""
+        _check(lib.ac4_md_relax(self.h, nsteps, f_start, f_end))
+
+    def init_chain(self, r, seed=None):
+        """InitialiseNoseHoover: eta = 0, p_eta ~ N(0, T*Q) (MDcode/time_evolve.c:476-493).
+        The draw depends on `seed` only (default: the replica index)."""
+        p = self.params[r]
+        q = np.array([(3.0 * p.n - 3.0) * p.T * 0.04, p.T * 0.04, p.T * 0.04])
+        rng = np.random.default_rng(7919 * r if seed is None else seed)
+        self.set_chain(r, np.concatenate([np.zeros(3), np.sqrt(p.T * q) * rng.normal(size=3), q]))
+
+    def set_rebuild_criterion(self, crit):
+        _check(lib.ac4_md_set_rebuild_criterion(self.h, crit))
+
+    def shuffle(self):
+        _check(lib.ac4_md_shuffle(self.h))
+
+    def shuffle_uniforms(self, r=0):
+        u = np.empty(3 * max(self.nact(r), 1))
+        _check(lib.ac4_md_shuffle_uniforms(self.h, r, _dp(u)))
+        return u[: 3 * self.nact(r)].reshape(3, -1)
+
+    def stats(self, r=0):
+        o = np.empty(8)
+        _check(lib.ac4_md_stats(self.h, r, _dp(o)))
+        return dict(zip(["pe", "virial", "ke", "T", "P", "drift", "Heff", "steps"], o))
+
+    def rebuilds(self, r=0):
+        v = lib.ac4_md_rebuilds(self.h, r)
+        if v < 0:
+            _check(v)
+        return v
+
+    def sync(self):
+        _check(lib.ac4_md_sync(self.h))
+
+    def kernel_times(self, nsteps):
+        """Average ms per step of {first_half, rebuild, force, shuffle} (HIP events)."""
+        o = np.empty(4)
+        _check(lib.ac4_md_kernel_times(self.h, nsteps, _dp(o)))
+        return dict(zip(["first_half", "rebuild", "force", "shuffle"], o))
+
+
+class Analysis:
+    """Streaming analysis handle (ac4_ana_* of include/ac4.h)."""
+
+    def __init__(self, n, L, fidx, gap=0, factor=1.4, a=0.3, obs=15, type_flag=0):
+        load()
+        self.fidx = _i32(fidx)
+        h = _VP()
+        _check(lib.ac4_ana_create(n, L, len(self.fidx), _ip(self.fidx), gap, factor, a, obs, type_flag,
+                                  ctypes.byref(h)))
+        self.h = h
+        self.n = n
+        self.obs = obs
+
+    def close(self):
+        if self.h:
+            lib.ac4_ana_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def gap(self):
+        return lib.ac4_ana_gap(self.h)
+
+    def push(self, pos, x, y, z, t):
+        x, y, z, t = _f64(x), _f64(y), _f64(z), _i32(t)
+        _check(lib.ac4_ana_push_frame(self.h, pos, _dp(x), _dp(y), _dp(z), _ip(t)))
+
+    def push_md(self, pos, md, r=0):
+        _check(lib.ac4_ana_push_md(self.h, pos, md.h, r))
+
+    def finalize(self):
+        _check(lib.ac4_ana_finalize(self.h))
+
+    def nslots(self, obs):
+        return lib.ac4_ana_nslots(self.h, obs)
+
+    def msd(self):
+        ns = self.nslots(OBS_MSD)
+        m0, m1, m2, dt = (np.empty(ns) for _ in range(4))
+        cn = np.empty(ns, dtype=np.int32)
+        noi = ctypes.c_int()
+        _check(lib.ac4_ana_get_msd(self.h, _dp(m0), _dp(m1), _dp(m2), _ip(cn), _dp(dt), ctypes.byref(noi)))
+        return dict(m0=m0, m1=m1, m2=m2, counters=cn, delta_t=dt, noi=noi.value)
+
+    def overlap(self):
+        ns = self.nslots(OBS_OVERLAP)
+        c, x4, dt = (np.empty(ns) for _ in range(3))
+        cn = np.empty(ns, dtype=np.int32)
+        noi = ctypes.c_int()
+        _check(lib.ac4_ana_get_overlap(self.h, _dp(c), _dp(x4), _ip(cn), _dp(dt), ctypes.byref(noi)))
+        return dict(c=c, x4=x4, counters=cn, delta_t=dt, noi=noi.value)
+
+    def phase(self, obs):
+        ns = self.nslots(obs)
+        cs, ss, cs2, ss2 = (np.empty((ns, NQ)) for _ in range(4))
+        cn = np.empty(ns, dtype=np.int32)
+        dt = np.empty(ns)
+        qv = np.empty(NQ)
+        _check(lib.ac4_ana_get_phase(self.h, obs, _dp(cs), _dp(ss), _dp(cs2), _dp(ss2), _ip(cn), _dp(dt), _dp(qv)))
+        return dict(cs=cs, ss=ss, cs2=cs2, ss2=ss2, counters=cn, delta_t=dt, qval=qv)
+
+    def pairs(self, obs):
+        np_ = lib.ac4_ana_npairs(self.h, obs)
+        if obs == OBS_MSD:
+            v = np.empty((max(np_, 1), 3))
+            _check(lib.ac4_ana_get_pairs(self.h, obs, _dp(v), None))
+            return v[:np_]
+        if obs == OBS_OVERLAP:
+            iv = np.empty(max(np_, 1), dtype=np.int32)
+            _check(lib.ac4_ana_get_pairs(self.h, obs, None, _ip(iv)))
+            return iv[:np_]
+        v = np.empty((max(np_, 1), 2 * NQ))
+        iv = np.empty(max(np_, 1), dtype=np.int32)
+        _check(lib.ac4_ana_get_pairs(self.h, obs, _dp(v), _ip(iv)))
+        return (v[:np_], iv[:np_]) if obs == OBS_S4 else v[:np_]
+
+    def write(self, sub_dir):
+        _check(lib.ac4_ana_write(self.h, sub_dir.encode()))
+
+
+def schedule(run_length, n_origins=200, factor=1.4):
+    """prepareSavingArray (MDcode/main.c:62-119) through libac4 (host logic)."""
+    load()
+    cap = n_origins * 64 + 64
+    out = np.zeros(cap, dtype=np.uint64)
+    n = lib.ac4_schedule(run_length, n_origins, factor, out.ctypes.data_as(_U64), cap)
+    if n < 0:
+        _check(n)
+    return out[:n].copy(), out
+
+
+def init_lattice(n, T, seed, n_act=-1):
+    """Seeded lattice state (ac4_init_lattice): dict of SPHERE fields + k's."""
+    load()
+    nact = int(n * 0.1) if n_act < 0 else n_act
+    d = {k: np.empty(n) for k in ("x", "y", "z", "xu", "yu", "zu", "px", "py", "pz")}
+    d["type"] = np.empty(n, dtype=np.int32)
+    d["act_tag"] = np.empty(n, dtype=np.int32)
+    for k in ("kx", "ky", "kz"):
+        d[k] = np.empty(max(nact, 1), dtype=np.int32)
+    _check(lib.ac4_init_lattice(n, T, n_act, seed, *[_dp(d[k]) for k in ("x", "y", "z", "xu", "yu", "zu",
+                                                                         "px", "py", "pz")],
+                                _ip(d["type"]), _ip(d["act_tag"]), _ip(d["kx"]), _ip(d["ky"]), _ip(d["kz"])))
+    for k in ("kx", "ky", "kz"):
+        d[k] = d[k][:nact]
+    return d
+
+
+def origin_gap(fidx):
+    load()
+    f = _i32(fidx)
+    g = lib.ac4_origin_gap(_ip(f), len(f))
+    if g < 0:
+        _check(g)
+    return g
+
+
+def philox(ctr4, key2):
+    load()
+    c = np.ascontiguousarray(ctr4, dtype=np.uint32)
+    k = np.ascontiguousarray(key2, dtype=np.uint32)
+    o = np.empty(4, dtype=np.uint32)
+    _check(lib.ac4_philox4x32(c.ctypes.data_as(_U32), k.ctypes.data_as(_U32), o.ctypes.data_as(_U32)))
+    return o

@@ -1,0 +1,378 @@
+/*
+ * ac4_host.c -- host-side scalar bookkeeping of libac4 (C99): error buffer,
+ * frame schedule, origin detection, (origin, lag) walk, ensemble chi4 and the
+ * reference's output formats.  No per-particle work happens here.
+ */
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include "ac4.h"
+#include "ac4_host.h"
+
+static __thread char g_err[1024];
+
+void ac4_set_error(const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+}
+
+const char *ac4_last_error(void) { return g_err; }
+
+/* MDcode/main.c:17-41: shell sort of a[1..n] (Numerical Recipes indexing) */
+static void shell_sort_1based(int n, uint64_t *a) {
+    int i, j, inc = 1;
+    uint64_t v;
+    do {
+        inc *= 3;
+        inc++;
+    } while (inc <= n);
+    do {
+        inc /= 3;
+        for (i = inc + 1; i <= n; i++) {
+            v = a[i];
+            j = i;
+            while (a[j - inc] > v) {
+                a[j] = a[j - inc];
+                j -= inc;
+                if (j <= inc) break;
+            }
+            a[j] = v;
+        }
+    } while (inc > 1);
+}
+
+int ac4_schedule(uint64_t run_length, int n_origins, double factor, uint64_t *out, int cap) {
+    if (!out || n_origins <= 0 || cap < 4 || !(factor > 1.0)) {
+        ac4_set_error("ac4_schedule: bad arguments");
+        return AC4_EARG;
+    }
+    /* MDcode/main.c:71-95: origins k*linearInterval, offsets x factor */
+    uint64_t cur = 0, lin = run_length / (uint64_t)n_origins, maxint = run_length;
+    for (uint64_t k = 0; k < (uint64_t)n_origins; k++) {
+        if (cur + 2 >= (uint64_t)cap) { ac4_set_error("ac4_schedule: cap %d too small", cap); return AC4_ECAP; }
+        out[cur++] = k * lin;
+        uint64_t off = 1;
+        while (off < maxint) {
+            uint64_t idx = k * lin + off;
+            if (idx < run_length) {
+                if (cur + 2 >= (uint64_t)cap) { ac4_set_error("ac4_schedule: cap %d too small", cap); return AC4_ECAP; }
+                out[cur++] = idx;
+            }
+            if ((uint64_t)(off * factor) == off)
+                off++;
+            else
+                off = (uint64_t)((double)off * factor);
+        }
+    }
+    /* shellSort(current, a) reads a[current], one word past the filled range;
+     * a freshly allocated heap block holds zero there (MDcode/main.c:483). */
+    for (uint64_t t = cur; t < (uint64_t)cap; t++) out[t] = 0;
+    shell_sort_1based((int)cur, out);
+    /* duplicate removal, MDcode/main.c:97-104 */
+    uint64_t i = 0, j = 0;
+    while (j < cur) {
+        while (j + 1 < (uint64_t)cap && out[j] == out[i]) j++;
+        i++;
+        out[i] = out[j];
+    }
+    return (int)i;
+}
+
+int ac4_origin_gap(const int *fidx, int nframes) {
+    /* Analysis/read_all_data.c:36-48 */
+    if (!fidx || nframes < 2) { ac4_set_error("ac4_origin_gap: need >= 2 frames"); return AC4_EARG; }
+    int index = 0;
+    while (index + 1 < nframes && fidx[index] + 1 == fidx[index + 1]) index++;
+    int run = index, consecutive = 0;
+    while (consecutive < run) {
+        if (index + 1 >= nframes) { ac4_set_error("ac4_origin_gap: no second origin run"); return AC4_EARG; }
+        if (fidx[index] + 1 == fidx[index + 1])
+            consecutive++;
+        else
+            consecutive = 0;
+        index++;
+    }
+    return fidx[index - run];
+}
+
+int ac4_walk(const int *fidx, int nframes, int gap, double factor, int rule, ac4_walk_t *w) {
+    /* origins counted on frames k*gap (+1 for rule 1): msd_pos.c:17-23, overlap.c:16-22 */
+    int cur = 0, on = 0;
+    do {
+        while (cur < nframes && fidx[cur] != on * gap + rule) cur++;
+        if (cur < nframes) on++;
+    } while (cur < nframes);
+    w->maximal_interval = (int)(on * gap / 2.0); /* DIVIDE_TO, msd_pos.c:28 */
+    w->n_origins_counted = on;
+    int cap = 64, np = 0;
+    w->cur = malloc(sizeof(int) * cap);
+    w->oth = malloc(sizeof(int) * cap);
+    w->slot = malloc(sizeof(int) * cap);
+    int current = 0, origin_number = 0, max_slot = 0;
+    do { /* msd_pos.c:44-107 */
+        int current_frame = fidx[current];
+        int other = current, offset = 1, time_index = 1;
+        while (other < nframes && offset < w->maximal_interval) {
+            int other_frame = current_frame + offset;
+            while (other < nframes && fidx[other] != other_frame) other++;
+            if (other < nframes) {
+                if (np == cap) {
+                    cap *= 2;
+                    w->cur = realloc(w->cur, sizeof(int) * cap);
+                    w->oth = realloc(w->oth, sizeof(int) * cap);
+                    w->slot = realloc(w->slot, sizeof(int) * cap);
+                }
+                w->cur[np] = current;
+                w->oth[np] = other;
+                w->slot[np] = time_index;
+                if (time_index > max_slot) max_slot = time_index;
+                np++;
+                if ((int)(offset * factor) == offset)
+                    offset++;
+                else
+                    offset = (int)(((double)offset) * factor);
+                time_index++;
+            }
+        }
+        origin_number++;
+        while (current < nframes && fidx[current] != origin_number * gap) current++;
+    } while (current < nframes);
+    w->np = np;
+    w->max_slot = max_slot;
+    return AC4_OK;
+}
+
+void ac4_walk_free(ac4_walk_t *w) {
+    free(w->cur);
+    free(w->oth);
+    free(w->slot);
+    memset(w, 0, sizeof *w);
+}
+
+int ac4_chi4_ensemble(int nrep, int nor, const double *q, const int *n_r, double *out2) {
+    if (nrep < 1 || nor < 1 || !q || !n_r || !out2) { ac4_set_error("ac4_chi4_ensemble: bad arguments"); return AC4_EARG; }
+    double s = 0.0, s2 = 0.0, nbar = 0.0, mean_r = 0.0;
+    for (int r = 0; r < nrep; r++) {
+        double a = 0.0, a2 = 0.0;
+        for (int o = 0; o < nor; o++) {
+            double v = q[(size_t)r * nor + o];
+            a += v;
+            a2 += v * v;
+        }
+        s += a;
+        s2 += a2;
+        nbar += (double)n_r[r];
+        mean_r += (double)n_r[r] * (a2 / nor - (a / nor) * (a / nor));
+    }
+    double m = (double)nrep * nor;
+    nbar /= (double)nrep;
+    out2[0] = nbar * (s2 / m - (s / m) * (s / m));
+    out2[1] = mean_r / (double)nrep;
+    return AC4_OK;
+}
+
+/* Degeneracy of a q vector, Analysis/S4Qt.c:72-76 */
+int ac4_degen(int x, int y, int z) {
+    if (y != z && y != x && z != x) return 6;
+    if (y == z && z == x) return 1;
+    return 3;
+}
+
+int ac4_write_msd(const char *path, int noi, const double *m0, const double *m1, const double *m2,
+                  const int *counters, const double *delta_t) {
+    FILE *f = fopen(path, "wb"); /* msd_pos.c:112-120 */
+    if (!f) { ac4_set_error("cannot write %s", path); return AC4_EIO; }
+    fprintf(f, "0.0\t0.0\t0.0\t0.0\n");
+    for (int k = 1; k < noi; k++)
+        if (counters[k])
+            fprintf(f, "%.14lf\t%.14lf\t%.14lf\t%.14lf\n", delta_t[k], m0[k] / (double)counters[k],
+                    m1[k] / (double)counters[k], m2[k] / (double)counters[k]);
+    fclose(f);
+    return AC4_OK;
+}
+
+int ac4_write_overlap(const char *path, int noi, const double *c, const double *x4, const int *counters,
+                      const double *delta_t) {
+    FILE *f = fopen(path, "wb"); /* overlap.c:108-121 */
+    if (!f) { ac4_set_error("cannot write %s", path); return AC4_EIO; }
+    fprintf(f, "0.0\t1.0\t0.0\n");
+    for (int k = 1; k < noi; k++)
+        if (counters[k])
+            fprintf(f, "%.14lf\t%.14lf\t%.14lf\n", delta_t[k], c[k] / (double)counters[k],
+                    x4[k] / (double)counters[k] - c[k] * c[k] / ((double)(counters[k] * counters[k])));
+    fclose(f);
+    return AC4_OK;
+}
+
+int ac4_write_phase(const char *path, int is_s4, int N, double totpart, int nslots, const int *vx, const int *vy,
+                    const int *vz, const double *cs, const double *ss, const double *cs2, const double *ss2,
+                    const int *counters, const double *delta_t, const double *qval) {
+    FILE *f = fopen(path, "wb");
+    if (!f) { ac4_set_error("cannot write %s", path); return AC4_EIO; }
+    const int nq = AC4_NQ;
+    if (is_s4) { /* S4Qt.c:251-271: slot-major, Degen-weighted, / N */
+        for (int k = 0; k < nslots; k++) {
+            if (counters[k] == 0) continue;
+            for (int n = 0; n < nq; n++) {
+                double dg = (double)ac4_degen(vx[n], vy[n], vz[n]);
+                double C = cs[k * nq + n] / (double)(counters[k]);
+                double C2 = cs2[k * nq + n] / (double)(counters[k]);
+                C2 = dg * (C2 - C * C) / (double)(N);
+                C = dg * C / (double)(N);
+                double S = ss[k * nq + n] / (double)(counters[k]);
+                double S2 = ss2[k * nq + n] / (double)(counters[k]);
+                S2 = dg * (S2 - S * S) / (double)(N);
+                S = dg * S / (double)(N);
+                fprintf(f, "%0.10g\t%0.10g\t%.10g\t%.10g\t%.10g\n", delta_t[k], qval[n], C, S, (S2 + C2));
+            }
+            fprintf(f, "\n\n");
+        }
+    } else { /* Fsqt.c:257-278: q-major, / TotPart */
+        for (int n = 0; n < nq; n++) {
+            for (int k = 0; k < nslots; k++) {
+                if (counters[k] == 0) continue;
+                double C = cs[k * nq + n] / (double)(counters[k]);
+                double C2 = cs2[k * nq + n] / (double)(counters[k]);
+                C2 = (double)(C2 - C * C) / totpart;
+                C = (double)C / totpart;
+                double S = ss[k * nq + n] / (double)(counters[k]);
+                double S2 = ss2[k * nq + n] / (double)(counters[k]);
+                S2 = (double)(S2 - S * S) / totpart;
+                S = (double)S / totpart;
+                fprintf(f, "%0.10g\t%0.10g\t%.10g\t%.10g\t%.10g\n", delta_t[k], qval[n], C, S, (S2 + C2));
+            }
+            fprintf(f, "\n\n");
+        }
+    }
+    fclose(f);
+    return AC4_OK;
+}
+
+/* Analysis/S4Qt.c:297-356 (nq = 15): the 162 lattice vectors sorted on |q|^2
+ * by the same shell sort (indices 1..161, so its tie order is reproduced). */
+int ac4_getq(int *vx, int *vy, int *vz, int *q) {
+    int count = 1;
+    q[0] = 1; vx[0] = 1; vy[0] = 0; vz[0] = 0;
+    for (int k = 1; k < 3; k++)
+        for (int i = k; i < 22; i++) { q[count] = k * k + i * i; vx[count] = 0; vy[count] = k; vz[count] = i; count++; }
+    for (int i = 1; i <= 15; i++)
+        for (int j = 1; j <= i; j++) { q[count] = i * i + j * j + j * j; vx[count] = i; vy[count] = j; vz[count] = j; count++; }
+    count--;
+    int inc = 1;
+    do { inc *= 3; inc++; } while (inc <= count);
+    do {
+        inc /= 3;
+        for (int i = inc + 1; i <= count; i++) {
+            int v = q[i], a = vx[i], b = vy[i], c = vz[i], j = i;
+            while (q[j - inc] > v) {
+                q[j] = q[j - inc]; vx[j] = vx[j - inc]; vy[j] = vy[j - inc]; vz[j] = vz[j - inc];
+                j -= inc;
+                if (j <= inc) break;
+            }
+            q[j] = v; vx[j] = a; vy[j] = b; vz[j] = c;
+        }
+    } while (inc > 1);
+    return count + 1;
+}
+
+/* ---- seeded initial state (replaces MDcode/initialise.c:34-101 and
+ * active.c:4-43, whose streams come from time() and glibc rand()) ---- */
+typedef struct { uint32_t k0, k1; uint64_t ctr; uint32_t buf[4]; int left; } ac4_rng;
+
+static void ac4_philox_host(const uint32_t c_in[4], const uint32_t k_in[2], uint32_t out[4]);
+
+static uint32_t rng_u32(ac4_rng *g) {
+    if (g->left == 0) {
+        uint32_t c[4] = {(uint32_t)g->ctr, (uint32_t)(g->ctr >> 32), 0x696e6974u, 0u}, k[2] = {g->k0, g->k1};
+        ac4_philox_host(c, k, g->buf);
+        g->ctr++;
+        g->left = 4;
+    }
+    return g->buf[4 - g->left--];
+}
+static double rng_u01(ac4_rng *g) {
+    uint32_t a = rng_u32(g), b = rng_u32(g);
+    return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6)) * (1.0 / 9007199254740992.0);
+}
+static void rng_perm(ac4_rng *g, int *p, int n) { /* uniform Fisher-Yates */
+    for (int i = 0; i < n; i++) p[i] = i;
+    for (int i = n - 1; i > 0; i--) {
+        int j = (int)(rng_u01(g) * (double)(i + 1));
+        int t = p[i]; p[i] = p[j]; p[j] = t;
+    }
+}
+
+static void ac4_philox_host(const uint32_t c_in[4], const uint32_t k_in[2], uint32_t out[4]) {
+    /* Philox-4x32-10, Salmon et al. SC 2011 (same as the device version) */
+    uint32_t c0 = c_in[0], c1 = c_in[1], c2 = c_in[2], c3 = c_in[3], k0 = k_in[0], k1 = k_in[1];
+    for (int r = 0; r < 10; ++r) {
+        if (r) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0, hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        c0 = hi1 ^ c1 ^ k0; c1 = lo1; c2 = hi0 ^ c3 ^ k1; c3 = lo0;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+int ac4_init_lattice(int n, double T, int n_act, uint64_t seed, double *x, double *y, double *z, double *xu,
+                     double *yu, double *zu, double *px, double *py, double *pz, int *type, int *act_tag, int *kx,
+                     int *ky, int *kz) {
+    if (n < 2 || !(T > 0) || !x || !y || !z || !px || !py || !pz || !type) {
+        ac4_set_error("ac4_init_lattice: bad arguments");
+        return AC4_EARG;
+    }
+    ac4_rng g = {(uint32_t)seed, (uint32_t)(seed >> 32), 0, {0, 0, 0, 0}, 0};
+    int t = 1;
+    while (t * t * t < n) t++; /* initialise.c:35-37 */
+    int nsite = t * t * t;
+    int *perm = malloc(sizeof(int) * (size_t)(nsite > n ? nsite : n));
+    if (!perm) { ac4_set_error("ac4_init_lattice: out of memory"); return AC4_EARG; }
+    rng_perm(&g, perm, nsite);
+    double dr = 1.0 / (double)t;
+    for (int i = 0; i < n; i++) { /* simple-cubic sites, chosen at random */
+        int s = perm[i];
+        x[i] = dr * (double)(s % t);
+        y[i] = dr * (double)((s / t) % t);
+        z[i] = dr * (double)(s / (t * t));
+        if (xu) { xu[i] = x[i]; yu[i] = y[i]; zu[i] = z[i]; }
+    }
+    int na = (int)(.80 * (double)n); /* global.h:16 */
+    rng_perm(&g, perm, n);
+    for (int i = 0; i < n; i++) type[perm[i]] = i < na ? 0 : 1;
+    /* Maxwell-Boltzmann momenta (Box-Muller), zero drift, T = sum p^2 / DOF */
+    double sx = 0, sy = 0, sz = 0;
+    for (int i = 0; i < n; i++) {
+        double v[3];
+        for (int k = 0; k < 3; k++) {
+            double u1 = rng_u01(&g), u2 = rng_u01(&g);
+            v[k] = sqrt(-2.0 * log(1.0 - u1)) * cos(2.0 * 3.14159265358979323846 * u2);
+        }
+        px[i] = v[0]; py[i] = v[1]; pz[i] = v[2];
+        sx += v[0]; sy += v[1]; sz += v[2];
+    }
+    double p2 = 0.0;
+    for (int i = 0; i < n; i++) {
+        px[i] -= sx / n; py[i] -= sy / n; pz[i] -= sz / n;
+        p2 += px[i] * px[i] + py[i] * py[i] + pz[i] * pz[i];
+    }
+    double scale = sqrt(T * (3.0 * (double)n - 3.0) / p2);
+    for (int i = 0; i < n; i++) { px[i] *= scale; py[i] *= scale; pz[i] *= scale; }
+    /* active particles and initial directions, MDcode/active.c:24-41 */
+    if (n_act < 0) n_act = (int)((double)n * 0.1000);
+    if (act_tag) {
+        rng_perm(&g, perm, n);
+        for (int i = 0; i < n; i++) act_tag[i] = -1;
+        for (int a = 0; a < n_act; a++) act_tag[perm[a]] = a;
+    }
+    if (kx && ky && kz)
+        for (int a = 0; a < n_act; a++) {
+            int v = a < n_act / 2 ? 1 : -1;
+            kx[a] = v; ky[a] = v; kz[a] = v;
+        }
+    free(perm);
+    return AC4_OK;
+}

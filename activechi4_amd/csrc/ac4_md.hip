@@ -1,0 +1,1204 @@
+// ac4_md.hip -- active Kob-Andersen MD step for MI355X (gfx950).
+//
+// Replaces MDcode/cell_list+forces.c, MDcode/time_evolve.c:476-609 and
+// MDcode/active.c:45-85 (paths relative to the reference checkout).  The
+// whole box of every replica is resident in HBM (no domain decomposition,
+// MDcode/Parallel.c); replicas are packed into every launch through a block
+// table (block -> replica, first particle), so one launch sequence per step
+// advances all of them.
+//
+// Per step (every launch covers all replicas, kernels early-exit per replica):
+//   k_shuffle      ShuffleActDirec when step % tau_per == 0 (sequential swap
+//                  chain of active.c:58-75 per component, Philox uniforms)
+//   k_first_half   U4 U3 U4 (chain start, computed redundantly per block),
+//                  U2 kick, U1 drift + wrap + max displacement (atomicMax)
+//   k_cell_zero .. k_nlist   counting-sort cell build + neighbour lists,
+//                  executed only for replicas whose displacement criterion
+//                  fired (flag read on device, so the sequence is graphable)
+//   k_force        pair forces + active force + second U2 kick + block
+//                  partial sums; the last block of each replica reduces the
+//                  partials in block order and runs U4 U3 U4 (chain end).
+// The momentum factor of the last U3 is kept pending (p_scale) and applied at
+// the next first half, so a step needs one sum of p^2 instead of four.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+#include <stdlib.h>
+#include <vector>
+#include <algorithm>
+#include "ac4_internal.h"
+
+#define BLK 256
+
+// MDcode/global.h:24-45 (Sigma = 1.00)
+#define K_LCUTSQR (6.25 * 1.00 * 1.00)
+#define K_SKIN (.35 * 1.00)
+#define K_CELL_CUT ((2.5 * 1.00) + (.35 * 1.00))
+#define K_CELL_CUT_SQ (K_CELL_CUT * K_CELL_CUT)
+#define K_DT 0.005
+#define K_HALF_DT (K_DT * 0.5)
+#define K_C0 0.0162665594880
+#define K_C2 -0.001949973872640
+
+struct RepC {
+    int n, off, nact, off_act;
+    int nc, ncell, off_cell, tau;      // list grid: nc = floor(L/2.85) cells per axis
+    int nblk, blk_off, nc_ref, pad1;   // nc_ref: the reference's grid (Parallel.c:72-80)
+    double L, invL, inv_cell_ref, T, f0, dof, density;
+    uint32_t key0, key1;
+};
+
+struct ChainS {
+    double eta[3], peta[3], q[3];
+    double S_end, p_scale;
+    double pe, vir, ke, inst_T, heff, drift;
+    unsigned long long maxbits;
+    long long step, shuffles;
+    int rebuilds, force_rebuild, err, crit;
+    unsigned int done_blocks;
+    int mode;                 // 0: Nose-Hoover chain, 1: velocity-rescale ramp
+    int ramp_n, ramp_k;
+    double ramp0, ramp1;      // target T = T*(ramp0 + (ramp1-ramp0)*k/(n-1))
+};
+
+// MDcode/cell_list+forces.c:19-20 restricted to the two KA species.
+__device__ __forceinline__ double sig_ab(int a, int b) { return (a + b) == 0 ? 1.00 : ((a + b) == 1 ? 0.80 : 0.88); }
+__device__ __forceinline__ double eps_ab(int a, int b) { return (a + b) == 0 ? 1.0 : ((a + b) == 1 ? 1.5 : 0.50); }
+
+__device__ __forceinline__ double minimg_md(double d) {  // cell_list+forces.c:92-97
+    if (d < -0.5) d += 1.0;
+    if (d > 0.5) d -= 1.0;
+    return d;
+}
+
+// U4 propagator (MDcode/time_evolve.c:578-609) with the p^2 sum supplied.
+__host__ __device__ inline void nhc_u4(double t, int jStart, int jEnd, double S, double T, double dof,
+                                       const double *q, double *peta) {
+    int dj = 1;
+    double gj = 0.0;
+    if (jStart > jEnd) dj = -1;
+    int j = jStart;
+    do {
+        if (j == 0) {
+            gj = S;
+            gj -= dof * T;
+        } else {
+            gj = (peta[j - 1] * peta[j - 1]) / q[j - 1] - T;
+        }
+        if (j == 2) {
+            peta[j] = peta[j] + t * gj;
+        } else {
+            double x = t * peta[j + 1] / q[j + 1];
+            peta[j] = peta[j] * exp(-x) + t * gj * (1.0 - exp(-x)) / x;
+        }
+        j = j + dj;
+    } while (j - dj != jEnd);
+}
+
+// U4(dt/4,2->0) U3(dt/2) U4(dt/4,0->2) on a copy of the chain (time_evolve.c:499-501
+// and :517-519); S = sum p^2 entering; returns the U3 momentum factor.
+__host__ __device__ inline double nhc_half(double S, double T, double dof, const double *q, double *eta,
+                                           double *peta, double *S_out) {
+    nhc_u4(K_DT * 0.25, 2, 0, S, T, dof, q, peta);
+    double t = K_DT * 0.50;
+    double s = exp(-t * peta[0] / q[0]);                        // time_evolve.c:571
+    for (int i = 0; i < 3; i++) eta[i] = eta[i] + t * peta[i] / q[i];  // :575
+    double S2 = S * s * s;
+    nhc_u4(K_DT * 0.25, 0, 2, S2, T, dof, q, peta);
+    *S_out = S2;
+    return s;
+}
+
+__device__ __forceinline__ bool need_rebuild(const RepC &R, const ChainS &c) {
+    if (c.force_rebuild) return true;
+    double maxD = __longlong_as_double((long long)c.maxbits);
+    maxD *= R.L * R.L;  // time_evolve.c:553
+    return (c.crit == 1 ? 3.0 : 4.0) * maxD > K_SKIN * K_SKIN;  // :508 (crit 1)
+}
+
+// ---------------------------------------------------------------- kernels
+__global__ void k_shuffle(const RepC *__restrict__ reps, ChainS *__restrict__ chain, int8_t *kx,
+                          int8_t *ky, int8_t *kz, int force) {
+    const int r = blockIdx.x;
+    const RepC R = reps[r];
+    ChainS &c = chain[r];
+    if (!force && (c.mode != 0 || R.tau <= 0 || (c.step % R.tau) != 0)) return;
+    const int comp = threadIdx.x;
+    if (comp < 3 && R.nact > 0) {
+        int8_t *k = (comp == 0 ? kx : (comp == 1 ? ky : kz)) + R.off_act;
+        const uint64_t s = (uint64_t)c.shuffles;
+        // MDcode/active.c:58-63: for i: j = (int)(Nact*U[0,1)); swap(k[i], k[j])
+        for (int i = 0; i < R.nact; i++) {
+            double u = ac4_shuffle_uniform(R.key0, R.key1, s, comp, i);
+            int j = (int)((double)R.nact * u);
+            int8_t tmp = k[i];
+            k[i] = k[j];
+            k[j] = tmp;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) c.shuffles += 1;
+}
+
+__global__ void __launch_bounds__(BLK) k_first_half(const RepC *__restrict__ reps, const ChainS *chain,
+                                                    const int *__restrict__ blk_rep, const int *__restrict__ blk_first,
+                                                    double *x, double *y, double *z, double *xu, double *yu, double *zu,
+                                                    double *px, double *py, double *pz, const double *__restrict__ fx,
+                                                    const double *__restrict__ fy, const double *__restrict__ fz,
+                                                    const double *__restrict__ xo, const double *__restrict__ yo,
+                                                    const double *__restrict__ zo, const int *__restrict__ type,
+                                                    double4 *pos4, ChainS *chain_w) {
+    __shared__ double s_scale[2];
+    __shared__ double s_max[BLK / 64];
+    const int b = blockIdx.x, r = blk_rep[b];
+    const RepC R = reps[r];
+    if (threadIdx.x == 0) {
+        const ChainS &c = chain[r];
+        double eta[3] = {c.eta[0], c.eta[1], c.eta[2]}, peta[3] = {c.peta[0], c.peta[1], c.peta[2]}, S2;
+        s_scale[0] = c.p_scale;
+        s_scale[1] = c.mode == 0 ? nhc_half(c.S_end, R.T, R.dof, c.q, eta, peta, &S2) : 1.0;
+    }
+    __syncthreads();
+    const double ps = s_scale[0], s1 = s_scale[1];
+    const int i = blk_first[b] + threadIdx.x;
+    double d2 = 0.0;
+    if (i < R.n) {
+        const int g = R.off + i;
+        double vx = px[g], vy = py[g], vz = pz[g];
+        vx = vx * ps; vy = vy * ps; vz = vz * ps;   // end-of-step U3 of the previous step
+        vx = vx * s1; vy = vy * s1; vz = vz * s1;   // U3, time_evolve.c:571-573
+        vx += fx[g] * K_HALF_DT;                    // U2, :562-564
+        vy += fy[g] * K_HALF_DT;
+        vz += fz[g] * K_HALF_DT;
+        px[g] = vx; py[g] = vy; pz[g] = vz;
+        double dx = vx * K_DT, dy = vy * K_DT, dz = vz * K_DT;   // U1, :532-540
+        double sx = x[g] + dx * R.invL, sy = y[g] + dy * R.invL, sz = z[g] + dz * R.invL;
+        double ux = xu[g] + dx * R.invL, uy = yu[g] + dy * R.invL, uz = zu[g] + dz * R.invL;
+        // applypbc1 (:15-26) loops until the coordinate is in [0,1); a
+        // coordinate that is not finite or many boxes away means the run blew
+        // up: flag it instead of looping
+        if (!(fabs(sx) < 8.0 && fabs(sy) < 8.0 && fabs(sz) < 8.0)) {
+            atomicOr(&chain_w[r].err, 2);
+            sx = 0.0; sy = 0.0; sz = 0.0;
+        }
+        while (sx >= 1.0) sx -= 1.0;
+        while (sx < 0.0) sx += 1.0;
+        while (sy >= 1.0) sy -= 1.0;
+        while (sy < 0.0) sy += 1.0;
+        while (sz >= 1.0) sz -= 1.0;
+        while (sz < 0.0) sz += 1.0;
+        x[g] = sx; y[g] = sy; z[g] = sz;
+        xu[g] = ux; yu[g] = uy; zu[g] = uz;
+        pos4[g] = make_double4(sx, sy, sz, (double)type[g]);
+        double ex = ux - xo[g], ey = uy - yo[g], ez = uz - zo[g];  // :547-551
+        d2 = ex * ex + ey * ey + ez * ez;
+    }
+    d2 = ac4_wave_max(d2);
+    if ((threadIdx.x & 63) == 0) s_max[threadIdx.x >> 6] = d2;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = s_max[0];
+        for (int w = 1; w < BLK / 64; w++) m = fmax(m, s_max[w]);
+        atomicMax(&chain_w[r].maxbits, (unsigned long long)__double_as_longlong(m));
+    }
+}
+
+// ---- rebuild: counting sort into cells, canonical order, permutation of the
+// particle arrays into cell order, neighbour lists from LDS-staged cells.
+// Binning uses the finest grid with cells >= 2.85 (list cutoff); the
+// reference's own grid (MDcode/Parallel.c:72-80) is evaluated beside it for
+// the cell ids update_cell_list reports.
+__global__ void __launch_bounds__(BLK) k_cell_zero(const RepC *__restrict__ reps, const ChainS *__restrict__ chain,
+                                                   int *cnt) {
+    const RepC R = reps[blockIdx.x];
+    const int c = blockIdx.y * BLK + threadIdx.x;
+    if (c >= R.ncell || !need_rebuild(R, chain[blockIdx.x])) return;
+    cnt[R.off_cell + c] = 0;
+}
+
+__device__ __forceinline__ int clampc(int c, int n) { return min(max(c, 0), n - 1); }
+
+__global__ void __launch_bounds__(BLK) k_bin(const RepC *__restrict__ reps, const ChainS *__restrict__ chain,
+                                             const int *__restrict__ blk_rep, const int *__restrict__ blk_first,
+                                             const double *__restrict__ x, const double *__restrict__ y,
+                                             const double *__restrict__ z, int *cnt, int *cell_of, int *rank,
+                                             int *cell_ref) {
+    const int b = blockIdx.x, r = blk_rep[b];
+    const RepC R = reps[r];
+    if (!need_rebuild(R, chain[r])) return;
+    const int i = blk_first[b] + threadIdx.x;
+    if (i >= R.n) return;
+    const int g = R.off + i;
+    const double sx = x[g], sy = y[g], sz = z[g];
+    // reference cell id, cell_list+forces.c:73-74 (floor in double, then sum)
+    const int rx = clampc((int)floor(sx * R.inv_cell_ref), R.nc_ref), ry = clampc((int)floor(sy * R.inv_cell_ref), R.nc_ref),
+              rz = clampc((int)floor(sz * R.inv_cell_ref), R.nc_ref);
+    cell_ref[g] = rx + ry * R.nc_ref + rz * R.nc_ref * R.nc_ref;
+    const int cx = clampc((int)floor(sx * R.nc), R.nc), cy = clampc((int)floor(sy * R.nc), R.nc),
+              cz = clampc((int)floor(sz * R.nc), R.nc);
+    const int c = cx + cy * R.nc + cz * R.nc * R.nc;
+    cell_of[g] = c;
+    rank[g] = atomicAdd(&cnt[R.off_cell + c], 1);
+}
+
+__global__ void __launch_bounds__(BLK) k_cell_scan(const RepC *__restrict__ reps, const ChainS *__restrict__ chain,
+                                                   const int *__restrict__ cnt, int *start) {
+    const RepC R = reps[blockIdx.x];
+    if (!need_rebuild(R, chain[blockIdx.x])) return;
+    __shared__ int s_part[BLK];
+    const int per = (R.ncell + BLK - 1) / BLK;
+    const int c0 = threadIdx.x * per, c1 = min(c0 + per, R.ncell);
+    int s = 0;
+    for (int c = c0; c < c1; c++) s += cnt[R.off_cell + c];
+    s_part[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int t = 0; t < BLK; t++) { int v = s_part[t]; s_part[t] = acc; acc += v; }
+    }
+    __syncthreads();
+    int acc = s_part[threadIdx.x];
+    for (int c = c0; c < c1; c++) { start[R.off_cell + c] = acc; acc += cnt[R.off_cell + c]; }
+}
+
+__global__ void __launch_bounds__(BLK) k_scatter(const RepC *__restrict__ reps, const ChainS *__restrict__ chain,
+                                                 const int *__restrict__ blk_rep, const int *__restrict__ blk_first,
+                                                 const int *__restrict__ cell_of, const int *__restrict__ rank,
+                                                 const int *__restrict__ start, int *members) {
+    const int b = blockIdx.x, r = blk_rep[b];
+    const RepC R = reps[r];
+    if (!need_rebuild(R, chain[r])) return;
+    const int i = blk_first[b] + threadIdx.x;
+    if (i >= R.n) return;
+    const int g = R.off + i;
+    members[R.off + start[R.off_cell + cell_of[g]] + rank[g]] = i;
+}
+
+// Members of a cell in increasing original index (the binning order of
+// cell_list+forces.c:71-80): a canonical order, so lists and force sums do not
+// depend on launch timing or on which replicas share the launch.  One wave per
+// cell: keys staged in LDS, each lane ranks its entries by counting.
+#define SORT_CAP 256
+__global__ void __launch_bounds__(BLK) k_cell_sort(const RepC *__restrict__ reps, const ChainS *__restrict__ chain,
+                                                   const int *__restrict__ cnt, const int *__restrict__ start,
+                                                   const int *__restrict__ id, int *members) {
+    __shared__ int s_m[BLK / 64][SORT_CAP];
+    __shared__ int s_k[BLK / 64][SORT_CAP];
+    const RepC R = reps[blockIdx.x];
+    if (!need_rebuild(R, chain[blockIdx.x])) return;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int c = (blockIdx.y * (BLK / 64)) + w;
+    if (c >= R.ncell) return;
+    int *m = members + R.off + start[R.off_cell + c];
+    const int n = cnt[R.off_cell + c];
+    if (n <= SORT_CAP) {
+        for (int k = lane; k < n; k += 64) { s_m[w][k] = m[k]; s_k[w][k] = id[R.off + m[k]]; }
+        __syncwarp();
+        for (int k = lane; k < n; k += 64) {
+            const int v = s_k[w][k];
+            int rank = 0;
+            for (int q = 0; q < n; q++) rank += s_k[w][q] < v;
+            m[rank] = s_m[w][k];
+        }
+    } else if (lane == 0) {  // crowded cell: serial insertion sort on the key
+        for (int a = 1; a < n; a++) {
+            int v = m[a], kv = id[R.off + v], k = a - 1;
+            while (k >= 0 && id[R.off + m[k]] > kv) { m[k + 1] = m[k]; k--; }
+            m[k + 1] = v;
+        }
+    }
+}
+
+struct PartArrays {  // particle arrays that move with a particle
+    double *x, *y, *z, *xu, *yu, *zu, *px, *py, *pz;
+    int *type, *act, *id, *cell_ref, *cell_of;
+};
+
+// new slot s <- old slot members[s]
+__global__ void __launch_bounds__(BLK) k_gather(const RepC *__restrict__ reps, const ChainS *__restrict__ chain,
+                                                const int *__restrict__ blk_rep, const int *__restrict__ blk_first,
+                                                const int *__restrict__ members, PartArrays src, PartArrays dst) {
+    const int b = blockIdx.x, r = blk_rep[b];
+    const RepC R = reps[r];
+    if (!need_rebuild(R, chain[r])) return;
+    const int i = blk_first[b] + threadIdx.x;
+    if (i >= R.n) return;
+    const int g = R.off + i, o = R.off + members[g];
+    dst.x[g] = src.x[o]; dst.y[g] = src.y[o]; dst.z[g] = src.z[o];
+    dst.xu[g] = src.xu[o]; dst.yu[g] = src.yu[o]; dst.zu[g] = src.zu[o];
+    dst.px[g] = src.px[o]; dst.py[g] = src.py[o]; dst.pz[g] = src.pz[o];
+    dst.type[g] = src.type[o]; dst.act[g] = src.act[o]; dst.id[g] = src.id[o];
+    dst.cell_ref[g] = src.cell_ref[o]; dst.cell_of[g] = src.cell_of[o];
+}
+
+__global__ void __launch_bounds__(BLK) k_commit(const RepC *__restrict__ reps, const ChainS *__restrict__ chain,
+                                                const int *__restrict__ blk_rep, const int *__restrict__ blk_first,
+                                                PartArrays src, PartArrays dst, double4 *pos4, double *xo, double *yo,
+                                                double *zo) {
+    const int b = blockIdx.x, r = blk_rep[b];
+    const RepC R = reps[r];
+    if (!need_rebuild(R, chain[r])) return;
+    const int i = blk_first[b] + threadIdx.x;
+    if (i >= R.n) return;
+    const int g = R.off + i;
+    const double sx = src.x[g], sy = src.y[g], sz = src.z[g], ux = src.xu[g], uy = src.yu[g], uz = src.zu[g];
+    const int t = src.type[g];
+    dst.x[g] = sx; dst.y[g] = sy; dst.z[g] = sz;
+    dst.xu[g] = ux; dst.yu[g] = uy; dst.zu[g] = uz;
+    dst.px[g] = src.px[g]; dst.py[g] = src.py[g]; dst.pz[g] = src.pz[g];
+    dst.type[g] = t; dst.act[g] = src.act[g]; dst.id[g] = src.id[g];
+    dst.cell_ref[g] = src.cell_ref[g]; dst.cell_of[g] = src.cell_of[g];
+    pos4[g] = make_double4(sx, sy, sz, (double)t);
+    xo[g] = ux;  // update_cell_list, cell_list+forces.c:141-146
+    yo[g] = uy;
+    zo[g] = uz;
+}
+
+// Distinct neighbour-cell coordinates along one axis (fewer than 3 cells per
+// axis would otherwise alias the 27-cell stencil, cf. cell_list+forces.c:37-49).
+__device__ __forceinline__ int stencil_axis(int c, int nc, int *out) {
+    if (nc >= 3) { out[0] = (c + nc - 1) % nc; out[1] = c; out[2] = (c + 1) % nc; return 3; }
+    if (nc == 2) { out[0] = 0; out[1] = 1; return 2; }
+    out[0] = 0;
+    return 1;
+}
+
+// One wave per cell: the particles of each neighbour cell (contiguous slots
+// after the permutation) are staged in LDS 64 at a time and every lane tests
+// them against its home particle.  Lists are stored neighbour-major
+// (nl[k*stride + slot]) so the force kernel reads them coalesced.
+#define NL_WAVE 64
+__global__ void __launch_bounds__(NL_WAVE) k_nlist(const RepC *__restrict__ reps, ChainS *chain,
+                                                   const int *__restrict__ cblk_rep, const int *__restrict__ cblk_cell,
+                                                   const double4 *__restrict__ pos4, const int *__restrict__ cnt,
+                                                   const int *__restrict__ start, int *nn, int *nl, int stride,
+                                                   int cap) {
+    __shared__ double4 s_p[NL_WAVE];
+    const int r = cblk_rep[blockIdx.x], c = cblk_cell[blockIdx.x];
+    const RepC R = reps[r];
+    if (!need_rebuild(R, chain[r])) return;
+    const int lane = threadIdx.x, nc = R.nc;
+    const int hs = start[R.off_cell + c], hn = cnt[R.off_cell + c];
+    const int cx = c % nc, cy = (c / nc) % nc, cz = c / (nc * nc);
+    int ax[3], ay[3], az[3];
+    const int nx = stencil_axis(cx, nc, ax), ny = stencil_axis(cy, nc, ay), nz = stencil_axis(cz, nc, az);
+    const double L = R.L;
+    for (int h0 = 0; h0 < hn; h0 += NL_WAVE) {
+        const int hl = hs + h0 + lane;  // local slot of the home particle
+        const bool act = h0 + lane < hn;
+        const int g = R.off + hl;
+        double4 pi = make_double4(0, 0, 0, 0);
+        if (act) pi = pos4[g];
+        int count = 0;
+        for (int iz = 0; iz < nz; iz++)
+            for (int iy = 0; iy < ny; iy++)
+                for (int ix = 0; ix < nx; ix++) {
+                    const int c2 = R.off_cell + ax[ix] + ay[iy] * nc + az[iz] * nc * nc;
+                    const int s2 = start[c2], n2 = cnt[c2];
+                    for (int k0 = 0; k0 < n2; k0 += NL_WAVE) {
+                        __syncthreads();
+                        if (k0 + lane < n2) s_p[lane] = pos4[R.off + s2 + k0 + lane];
+                        __syncthreads();
+                        const int m = min(NL_WAVE, n2 - k0);
+                        if (act)
+                            for (int k = 0; k < m; k++) {
+                                const int jl = s2 + k0 + k;
+                                if (jl == hl) continue;
+                                const double4 pj = s_p[k];
+                                // cell_list+forces.c:120-130, same operation order
+                                double dx = minimg_md(pi.x - pj.x);
+                                double dy = minimg_md(pi.y - pj.y);
+                                double dz = minimg_md(pi.z - pj.z);
+                                double dr2 = (dx * dx + dy * dy + dz * dz) * L * L;
+                                if (dr2 < K_CELL_CUT_SQ) {
+                                    if (count < cap) nl[(size_t)count * stride + g] = R.off + jl;
+                                    count++;
+                                }
+                            }
+                    }
+                }
+        if (act) {
+            if (count > cap) atomicOr(&chain[r].err, 1);
+            nn[g] = min(count, cap);
+        }
+    }
+}
+
+// Reference-order export of x_u and type for in-situ analysis.
+__global__ void __launch_bounds__(BLK) k_export(const RepC *__restrict__ reps, const int *__restrict__ blk_rep,
+                                                const int *__restrict__ blk_first, const int *__restrict__ id,
+                                                const double *__restrict__ xu, const double *__restrict__ yu,
+                                                const double *__restrict__ zu, const int *__restrict__ type,
+                                                double *ex, double *ey, double *ez, int *et, int r_only) {
+    const int b = blockIdx.x, r = blk_rep[b];
+    if (r_only >= 0 && r != r_only) return;
+    const RepC R = reps[r];
+    const int i = blk_first[b] + threadIdx.x;
+    if (i >= R.n) return;
+    const int g = R.off + i, o = R.off + id[g];
+    ex[o] = xu[g]; ey[o] = yu[g]; ez[o] = zu[g]; et[o] = type[g];
+}
+
+// ---- forces
+template <bool FP32>
+struct PairAcc;
+
+template <>
+struct PairAcc<false> {  // FP64: the reference's per-pair arithmetic
+    double fx = 0, fy = 0, fz = 0, pe = 0, vir = 0;
+    __device__ __forceinline__ void add(double xi, double yi, double zi, int ti, const double4 pj, double L) {
+        double dx = minimg_md(xi - pj.x);
+        double dy = minimg_md(yi - pj.y);
+        double dz = minimg_md(zi - pj.z);
+        double dr2 = (dx * dx + dy * dy + dz * dz) * L * L;  // cell_list+forces.c:183
+        const int tj = (int)pj.w;
+        double eps = eps_ab(ti, tj), sig = sig_ab(ti, tj);
+        double r_over_sig_2 = dr2 / (sig * sig);
+        if (r_over_sig_2 < K_LCUTSQR) {  // :189-203
+            double s2 = 1.0 / r_over_sig_2;
+            double s6 = s2 * s2 * s2;
+            pe += 4.0 * eps * ((s6 * (s6 - 1.0)) + K_C0 + K_C2 * r_over_sig_2);
+            double grad = 8.0 * eps * (6.0 * s6 * (s6 - 0.5) / dr2 - K_C2 / (sig * sig));
+            fx += grad * dx * L;
+            fy += grad * dy * L;
+            fz += grad * dz * L;
+            vir += grad * dr2;
+        }
+    }
+    __device__ __forceinline__ void result(double &ox, double &oy, double &oz) const { ox = fx; oy = fy; oz = fz; }
+};
+
+template <>
+struct PairAcc<true> {  // FP32 pair forces, FP64 minimum image, Kahan sums per particle
+    float fx = 0, fy = 0, fz = 0, cx = 0, cy = 0, cz = 0;
+    double pe = 0, vir = 0;
+    __device__ __forceinline__ static void kahan(float &s, float &c, float v) {
+        float y = v - c;
+        float t = s + y;
+        c = (t - s) - y;
+        s = t;
+    }
+    __device__ __forceinline__ void add(double xi, double yi, double zi, int ti, const double4 pj, double L) {
+        const float dx = (float)(minimg_md(xi - pj.x) * L);
+        const float dy = (float)(minimg_md(yi - pj.y) * L);
+        const float dz = (float)(minimg_md(zi - pj.z) * L);
+        const int tj = (int)pj.w, ab = ti + tj;
+        const float isig2 = ab == 0 ? 1.0f : (ab == 1 ? (float)(1.0 / (0.80 * 0.80)) : (float)(1.0 / (0.88 * 0.88)));
+        const float eps = ab == 0 ? 1.0f : (ab == 1 ? 1.5f : 0.5f);
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        const float rs2 = r2 * isig2;
+        if (rs2 < 6.25f) {
+            const float s2 = 1.0f / rs2;
+            const float s6 = s2 * s2 * s2;
+            pe += (double)(4.0f * eps * ((s6 * (s6 - 1.0f)) + (float)K_C0 + (float)K_C2 * rs2));
+            const float grad = 8.0f * eps * (6.0f * s6 * (s6 - 0.5f) / r2 - (float)K_C2 * isig2);
+            kahan(fx, cx, grad * dx);
+            kahan(fy, cy, grad * dy);
+            kahan(fz, cz, grad * dz);
+            vir += (double)(grad * r2);
+        }
+    }
+    __device__ __forceinline__ void result(double &ox, double &oy, double &oz) const {
+        ox = (double)fx - (double)cx;
+        oy = (double)fy - (double)cy;
+        oz = (double)fz - (double)cz;
+    }
+};
+
+template <bool FP32, bool KICK>
+__global__ void __launch_bounds__(BLK) k_force(const RepC *__restrict__ reps, ChainS *chain,
+                                               const int *__restrict__ blk_rep, const int *__restrict__ blk_first,
+                                               const int *__restrict__ blk_idx, const double4 *__restrict__ pos4,
+                                               const int *__restrict__ nn, const int *__restrict__ nl, int stride,
+                                               const int *__restrict__ act, const int8_t *__restrict__ kx,
+                                               const int8_t *__restrict__ ky, const int8_t *__restrict__ kz,
+                                               double *fx, double *fy, double *fz, double *px, double *py, double *pz,
+                                               double *partial) {
+    __shared__ double s_red[4][BLK / 64];
+    __shared__ int s_last;
+    const int b = blockIdx.x, r = blk_rep[b];
+    const RepC R = reps[r];
+    const int i = blk_first[b] + threadIdx.x;
+    double v_p2 = 0.0, v_pe = 0.0, v_vir = 0.0, v_px = 0.0;
+    if (i < R.n) {
+        const int g = R.off + i;
+        const double4 pi = pos4[g];
+        const int ti = (int)pi.w;
+        PairAcc<FP32> acc;
+        const int m = nn[g];
+        for (int k = 0; k < m; k++) acc.add(pi.x, pi.y, pi.z, ti, pos4[nl[(size_t)k * stride + g]], R.L);
+        double Fx, Fy, Fz;
+        acc.result(Fx, Fy, Fz);
+        const int a = act[g];
+        if (a != -1) {  // cell_list+forces.c:207-213
+            Fx += R.f0 * (double)kx[R.off_act + a];
+            Fy += R.f0 * (double)ky[R.off_act + a];
+            Fz += R.f0 * (double)kz[R.off_act + a];
+        }
+        fx[g] = Fx; fy[g] = Fy; fz[g] = Fz;
+        v_pe = acc.pe;
+        v_vir = acc.vir;
+        if (KICK) {  // U2 after the forces, time_evolve.c:516
+            double vx = px[g] + Fx * K_HALF_DT, vy = py[g] + Fy * K_HALF_DT, vz = pz[g] + Fz * K_HALF_DT;
+            px[g] = vx; py[g] = vy; pz[g] = vz;
+            v_p2 = vx * vx + vy * vy + vz * vz;
+            v_px = vx;
+        }
+    }
+    double v[4] = {v_p2, v_pe, v_vir, v_px};
+    for (int q = 0; q < 4; q++) {
+        double w = ac4_wave_sum(v[q]);
+        if ((threadIdx.x & 63) == 0) s_red[q][threadIdx.x >> 6] = w;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 0; q < 4; q++) {
+            double s = 0.0;
+            for (int w = 0; w < BLK / 64; w++) s += s_red[q][w];
+            partial[4 * (size_t)b + q] = s;
+        }
+        __threadfence();
+        unsigned int prev = atomicAdd(&chain[r].done_blocks, 1u);
+        s_last = (prev == (unsigned int)(R.nblk - 1));
+    }
+    __syncthreads();
+    if (!s_last || threadIdx.x != 0) return;
+    // last block of replica r: fixed-order reduction + chain end
+    __threadfence();
+    ChainS &c = chain[r];
+    double S = 0.0, pe = 0.0, vir = 0.0, spx = 0.0;
+    for (int k = 0; k < R.nblk; k++) {
+        const double *pp = partial + 4 * (size_t)(R.blk_off + k);
+        S += pp[0]; pe += pp[1]; vir += pp[2]; spx += pp[3];
+    }
+    pe = 0.5 * pe;                  // every pair seen from both sides
+    vir = (0.5 * vir) / (3.0e0);   // cell_list+forces.c:220
+    c.pe = pe;
+    c.vir = vir;
+    c.done_blocks = 0u;
+    if (!KICK) return;
+    const bool rebuilt = need_rebuild(R, c);
+    if (c.mode != 0) {
+        // coldstart/breakxal of MDcode/initialise.c:166-201: momenta rescaled
+        // to a ramped target temperature every step, no thermostat chain
+        const double f = c.ramp_n > 1 ? c.ramp0 + (c.ramp1 - c.ramp0) * (double)c.ramp_k / (double)(c.ramp_n - 1)
+                                      : c.ramp1;
+        const double s2 = S > 0.0 ? sqrt(R.T * f * R.dof / S) : 1.0;
+        c.p_scale = s2;
+        c.S_end = S * s2 * s2;
+        c.ke = c.S_end * 0.5;
+        c.inst_T = 2.0 * c.ke / R.dof;
+        c.drift = spx * s2;
+        c.heff = c.ke + pe;
+        if (++c.ramp_k >= c.ramp_n) c.mode = 0;
+        if (rebuilt) c.rebuilds += 1;
+        c.maxbits = 0ull;
+        c.force_rebuild = 0;
+        return;
+    }
+    double eta[3] = {c.eta[0], c.eta[1], c.eta[2]}, peta[3] = {c.peta[0], c.peta[1], c.peta[2]}, S2;
+    nhc_half(c.S_end, R.T, R.dof, c.q, eta, peta, &S2);   // start of step (same values as k_first_half)
+    double s2 = nhc_half(S, R.T, R.dof, c.q, eta, peta, &S2);  // end of step, time_evolve.c:517-519
+    for (int k = 0; k < 3; k++) { c.eta[k] = eta[k]; c.peta[k] = peta[k]; }
+    c.S_end = S2;
+    c.p_scale = s2;
+    c.ke = S2 * 0.5;                 // time_evolve.c:520
+    c.inst_T = 2.0 * c.ke / R.dof;   // :524
+    c.drift = spx * s2;
+    double heff = c.ke + pe + eta[0] * R.T * R.dof;  // print.c:85-87
+    for (int k = 0; k < 3; k++) heff = heff + peta[k] * peta[k] / (2.0 * c.q[k]);
+    for (int k = 1; k < 3; k++) heff = heff + R.T * eta[k];
+    c.heff = heff;
+    if (rebuilt) c.rebuilds += 1;
+    c.maxbits = 0ull;
+    c.force_rebuild = 0;
+    c.step += 1;
+}
+
+// ------------------------------------------------------------- host side
+struct ac4_md {
+    int R = 0, prec = 0, ntot = 0, nblk = 0, ncell_tot = 0, nact_tot = 0, cap = 192, stride = 0, max_ncell = 0;
+    std::vector<RepC> reps_h;
+    RepC *reps = nullptr;
+    ChainS *chain = nullptr;
+    int *blk_rep = nullptr, *blk_first = nullptr, *blk_idx = nullptr, *cblk_rep = nullptr, *cblk_cell = nullptr;
+    double *partial = nullptr;
+    PartArrays P{}, A{};  // primary and permutation scratch
+    double *fx = nullptr, *fy = nullptr, *fz = nullptr, *xo = nullptr, *yo = nullptr, *zo = nullptr;
+    double4 *pos4 = nullptr;
+    int *rank = nullptr, *nn = nullptr, *nl = nullptr, *cnt = nullptr, *start = nullptr, *members = nullptr;
+    int8_t *kx = nullptr, *ky = nullptr, *kz = nullptr;
+    double *ex = nullptr, *ey = nullptr, *ez = nullptr;  // reference-order export
+    int *et = nullptr;
+    hipStream_t stream = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    bool have_forces = false;
+};
+
+template <class T>
+static int dalloc(T **p, size_t n) {
+    if (n == 0) n = 1;
+    AC4_HIP(hipMalloc((void **)p, n * sizeof(T)));
+    AC4_HIP(hipMemset(*p, 0, n * sizeof(T)));
+    return AC4_OK;
+}
+
+static int alloc_parts(PartArrays &a, size_t n) {
+    int rc;
+    double **d[] = {&a.x, &a.y, &a.z, &a.xu, &a.yu, &a.zu, &a.px, &a.py, &a.pz};
+    for (double **q : d)
+        if ((rc = dalloc(q, n)) != AC4_OK) return rc;
+    int **i[] = {&a.type, &a.act, &a.id, &a.cell_ref, &a.cell_of};
+    for (int **q : i)
+        if ((rc = dalloc(q, n)) != AC4_OK) return rc;
+    return AC4_OK;
+}
+static void free_parts(PartArrays &a) {
+    void *q[] = {a.x, a.y, a.z, a.xu, a.yu, a.zu, a.px, a.py, a.pz, a.type, a.act, a.id, a.cell_ref, a.cell_of};
+    for (void *v : q)
+        if (v) (void)hipFree(v);
+    a = PartArrays{};
+}
+
+extern "C" int ac4_md_create(int n_rep, const ac4_replica_params *p, int precision, ac4_md **out) {
+    AC4_REQUIRE(n_rep >= 1 && p && out, AC4_EARG, "ac4_md_create: bad arguments");
+    AC4_REQUIRE(precision == AC4_FP64 || precision == AC4_FP32, AC4_EARG, "ac4_md_create: precision must be 0 or 1");
+    int ndev = 0;
+    AC4_REQUIRE(hipGetDeviceCount(&ndev) == hipSuccess && ndev > 0, AC4_EHIP,
+                "ac4_md_create: no HIP device visible (libac4 has no CPU path)");
+    ac4_md *md = new ac4_md();
+    md->R = n_rep;
+    md->prec = precision;
+    int off = 0, off_act = 0, off_cell = 0, blk_off = 0;
+    std::vector<int> brep, bfirst, bidx, crep, ccell;
+    for (int r = 0; r < n_rep; r++) {
+        RepC R;
+        memset(&R, 0, sizeof R);
+        const ac4_replica_params &q = p[r];
+        if (q.n < 2 || !(q.density > 0) || !(q.T > 0)) { delete md; ac4_set_error("replica %d: bad n/density/T", r); return AC4_EARG; }
+        R.n = q.n;
+        R.off = off;
+        R.nact = q.n_act < 0 ? (int)((double)q.n * 0.1000) : q.n_act;
+        if (R.nact > q.n) { delete md; ac4_set_error("replica %d: n_act > n", r); return AC4_EARG; }
+        R.off_act = off_act;
+        R.L = pow((double)q.n / q.density, 0.33333333);  // MDcode/main.c:465
+        R.invL = 1.0 / R.L;                              // :466
+        if (R.L <= 2.0 * K_CELL_CUT) { delete md; ac4_set_error("replica %d: box L=%g too small for the 2.85 list cutoff", r, R.L); return AC4_EARG; }
+        // MDcode/Parallel.c:72-80 with one process (Lpp = L): the reference grid
+        const double Lpp = R.L / (double)1;
+        R.nc_ref = (int)((double)((int)Lpp) / K_CELL_CUT);
+        if (R.nc_ref < 1) R.nc_ref = 1;
+        R.inv_cell_ref = R.L / (Lpp / (double)R.nc_ref);
+        // list grid: as many cells as fit with side >= 2.85
+        R.nc = (int)(R.L / K_CELL_CUT);
+        R.ncell = R.nc * R.nc * R.nc;
+        R.off_cell = off_cell;
+        R.tau = q.tau_per;
+        R.T = q.T;
+        R.f0 = q.f0;
+        R.density = q.density;
+        R.dof = 3.0 * (double)q.n - 3.0;  // global.h:35
+        R.key0 = (uint32_t)q.seed;
+        R.key1 = (uint32_t)(q.seed >> 32);
+        R.nblk = (q.n + BLK - 1) / BLK;
+        R.blk_off = blk_off;
+        for (int k = 0; k < R.nblk; k++) { brep.push_back(r); bfirst.push_back(k * BLK); bidx.push_back(k); }
+        for (int c = 0; c < R.ncell; c++) { crep.push_back(r); ccell.push_back(c); }
+        off += q.n;
+        off_act += R.nact;
+        off_cell += R.ncell;
+        blk_off += R.nblk;
+        md->max_ncell = std::max(md->max_ncell, R.ncell);
+        md->reps_h.push_back(R);
+    }
+    md->ntot = off;
+    md->nact_tot = off_act;
+    md->ncell_tot = off_cell;
+    md->nblk = blk_off;
+    md->stride = off;
+    int rc = AC4_OK;
+#define ALLOC(ptr, n) if ((rc = dalloc(&md->ptr, (size_t)(n))) != AC4_OK) { ac4_md_destroy(md); return rc; }
+    ALLOC(reps, n_rep) ALLOC(chain, n_rep) ALLOC(blk_rep, md->nblk) ALLOC(blk_first, md->nblk) ALLOC(blk_idx, md->nblk)
+    ALLOC(cblk_rep, off_cell) ALLOC(cblk_cell, off_cell) ALLOC(partial, 4 * (size_t)md->nblk)
+    ALLOC(fx, off) ALLOC(fy, off) ALLOC(fz, off) ALLOC(xo, off) ALLOC(yo, off) ALLOC(zo, off) ALLOC(pos4, off)
+    ALLOC(rank, off) ALLOC(nn, off) ALLOC(nl, (size_t)md->cap * off) ALLOC(cnt, off_cell) ALLOC(start, off_cell)
+    ALLOC(members, off) ALLOC(kx, off_act) ALLOC(ky, off_act) ALLOC(kz, off_act)
+#undef ALLOC
+    if ((rc = alloc_parts(md->P, off)) != AC4_OK || (rc = alloc_parts(md->A, off)) != AC4_OK) { ac4_md_destroy(md); return rc; }
+    if (hipStreamCreate(&md->stream) != hipSuccess) { ac4_md_destroy(md); ac4_set_error("hipStreamCreate failed"); return AC4_EHIP; }
+    AC4_HIP(hipMemcpy(md->reps, md->reps_h.data(), sizeof(RepC) * n_rep, hipMemcpyHostToDevice));
+    AC4_HIP(hipMemcpy(md->blk_rep, brep.data(), sizeof(int) * md->nblk, hipMemcpyHostToDevice));
+    AC4_HIP(hipMemcpy(md->blk_first, bfirst.data(), sizeof(int) * md->nblk, hipMemcpyHostToDevice));
+    AC4_HIP(hipMemcpy(md->blk_idx, bidx.data(), sizeof(int) * md->nblk, hipMemcpyHostToDevice));
+    AC4_HIP(hipMemcpy(md->cblk_rep, crep.data(), sizeof(int) * off_cell, hipMemcpyHostToDevice));
+    AC4_HIP(hipMemcpy(md->cblk_cell, ccell.data(), sizeof(int) * off_cell, hipMemcpyHostToDevice));
+    std::vector<ChainS> ch(n_rep);
+    for (int r = 0; r < n_rep; r++) {
+        memset(&ch[r], 0, sizeof(ChainS));
+        const double tau = .20, T = md->reps_h[r].T;  // time_evolve.c:477-482
+        for (int k = 0; k < 3; k++) ch[r].q[k] = T * tau * tau;
+        ch[r].q[0] = md->reps_h[r].dof * T * tau * tau;
+        ch[r].p_scale = 1.0;
+        ch[r].force_rebuild = 1;
+    }
+    AC4_HIP(hipMemcpy(md->chain, ch.data(), sizeof(ChainS) * n_rep, hipMemcpyHostToDevice));
+    std::vector<int> neg(off, -1), ids(off);
+    for (int r = 0; r < n_rep; r++)
+        for (int i = 0; i < md->reps_h[r].n; i++) ids[md->reps_h[r].off + i] = i;
+    AC4_HIP(hipMemcpy(md->P.act, neg.data(), sizeof(int) * off, hipMemcpyHostToDevice));
+    AC4_HIP(hipMemcpy(md->P.id, ids.data(), sizeof(int) * off, hipMemcpyHostToDevice));
+    *out = md;
+    return AC4_OK;
+}
+
+extern "C" int ac4_md_destroy(ac4_md *md) {
+    if (!md) return AC4_OK;
+    if (md->stream) (void)hipStreamSynchronize(md->stream);
+    if (md->gexec) (void)hipGraphExecDestroy(md->gexec);
+    if (md->graph) (void)hipGraphDestroy(md->graph);
+    void *ptrs[] = {md->reps, md->chain, md->blk_rep, md->blk_first, md->blk_idx, md->cblk_rep, md->cblk_cell,
+                    md->partial, md->fx, md->fy, md->fz, md->xo, md->yo, md->zo, md->pos4, md->rank, md->nn, md->nl,
+                    md->cnt, md->start, md->members, md->kx, md->ky, md->kz, md->ex, md->ey, md->ez, md->et};
+    for (void *q : ptrs)
+        if (q) (void)hipFree(q);
+    free_parts(md->P);
+    free_parts(md->A);
+    if (md->stream) (void)hipStreamDestroy(md->stream);
+    delete md;
+    return AC4_OK;
+}
+
+extern "C" int ac4_md_total_particles(const ac4_md *md) { return md ? md->ntot : 0; }
+
+static int check_rep(const ac4_md *md, int r) {
+    AC4_REQUIRE(md && r >= 0 && r < md->R, AC4_EARG, "bad handle or replica index %d", r);
+    return AC4_OK;
+}
+
+template <class T>
+static int h2d(T *dst, const T *src, size_t n, hipStream_t s) {
+    AC4_HIP(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, s));
+    return AC4_OK;
+}
+template <class T>
+static int d2h(T *dst, const T *src, size_t n, hipStream_t s) {
+    AC4_HIP(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToHost, s));
+    return AC4_OK;
+}
+#define TRY(e) do { int rc_ = (e); if (rc_ != AC4_OK) return rc_; } while (0)
+
+static int get_chain(ac4_md *md, int r, ChainS *c) {
+    TRY(d2h(c, md->chain + r, 1, md->stream));
+    AC4_HIP(hipStreamSynchronize(md->stream));
+    return AC4_OK;
+}
+static int put_chain(ac4_md *md, int r, const ChainS *c) {
+    TRY(h2d(md->chain + r, c, 1, md->stream));
+    AC4_HIP(hipStreamSynchronize(md->stream));
+    return AC4_OK;
+}
+static int get_chains(ac4_md *md, std::vector<ChainS> &c) {
+    c.resize(md->R);
+    TRY(d2h(c.data(), md->chain, md->R, md->stream));
+    AC4_HIP(hipStreamSynchronize(md->stream));
+    return AC4_OK;
+}
+static int put_chains(ac4_md *md, const std::vector<ChainS> &c) {
+    TRY(h2d(md->chain, c.data(), md->R, md->stream));
+    AC4_HIP(hipStreamSynchronize(md->stream));
+    return AC4_OK;
+}
+
+// slot -> reference index map of replica r
+static int get_ids(ac4_md *md, int r, std::vector<int> &id) {
+    const RepC &R = md->reps_h[r];
+    id.resize(R.n);
+    TRY(d2h(id.data(), md->P.id + R.off, R.n, md->stream));
+    AC4_HIP(hipStreamSynchronize(md->stream));
+    return AC4_OK;
+}
+template <class T>
+static int fetch_unpermuted(ac4_md *md, int r, const T *dev, T *out, const std::vector<int> &id) {
+    const RepC &R = md->reps_h[r];
+    std::vector<T> tmp(R.n);
+    TRY(d2h(tmp.data(), dev + R.off, R.n, md->stream));
+    AC4_HIP(hipStreamSynchronize(md->stream));
+    for (int i = 0; i < R.n; i++) out[id[i]] = tmp[i];
+    return AC4_OK;
+}
+
+extern "C" int ac4_md_set_state(ac4_md *md, int r, const double *x, const double *y, const double *z,
+                                const double *xu, const double *yu, const double *zu, const double *px,
+                                const double *py, const double *pz, const int *type, const int *act_tag,
+                                const int *kx, const int *ky, const int *kz) {
+    TRY(check_rep(md, r));
+    AC4_REQUIRE(x && y && z && xu && yu && zu && px && py && pz && type, AC4_EARG, "set_state: NULL array");
+    const RepC &R = md->reps_h[r];
+    const int n = R.n, g = R.off;
+    std::vector<double4> p4(n);
+    std::vector<int> ids(n);
+    double keblk = 0.0;  // time_evolve.c:586-589 order
+    for (int i = 0; i < n; i++) {
+        AC4_REQUIRE(type[i] == 0 || type[i] == 1, AC4_EARG, "set_state: type[%d]=%d not in {0,1}", i, type[i]);
+        AC4_REQUIRE(x[i] >= 0.0 && x[i] < 1.0 && y[i] >= 0.0 && y[i] < 1.0 && z[i] >= 0.0 && z[i] < 1.0, AC4_EARG,
+                    "set_state: wrapped coordinate of particle %d outside [0,1)", i);
+        p4[i] = make_double4(x[i], y[i], z[i], (double)type[i]);
+        keblk += px[i] * px[i] + py[i] * py[i] + pz[i] * pz[i];
+        ids[i] = i;
+    }
+    hipStream_t s = md->stream;
+    // a loaded state is stored in reference order (slot = index)
+    TRY(h2d(md->P.x + g, x, n, s)); TRY(h2d(md->P.y + g, y, n, s)); TRY(h2d(md->P.z + g, z, n, s));
+    TRY(h2d(md->P.xu + g, xu, n, s)); TRY(h2d(md->P.yu + g, yu, n, s)); TRY(h2d(md->P.zu + g, zu, n, s));
+    TRY(h2d(md->xo + g, xu, n, s)); TRY(h2d(md->yo + g, yu, n, s)); TRY(h2d(md->zo + g, zu, n, s));
+    TRY(h2d(md->P.px + g, px, n, s)); TRY(h2d(md->P.py + g, py, n, s)); TRY(h2d(md->P.pz + g, pz, n, s));
+    TRY(h2d(md->P.type + g, type, n, s));
+    TRY(h2d(md->P.id + g, ids.data(), n, s));
+    TRY(h2d(md->pos4 + g, p4.data(), n, s));
+    std::vector<int> at;
+    if (act_tag) {
+        for (int i = 0; i < n; i++)
+            AC4_REQUIRE(act_tag[i] >= -1 && act_tag[i] < R.nact, AC4_EARG, "set_state: actTag[%d]=%d out of range", i, act_tag[i]);
+        at.assign(act_tag, act_tag + n);
+    } else {  // keep the current tags, brought back to reference order
+        std::vector<int> cur;
+        TRY(get_ids(md, r, cur));
+        at.resize(n);
+        TRY(fetch_unpermuted(md, r, (const int *)md->P.act, at.data(), cur));
+    }
+    TRY(h2d(md->P.act + g, at.data(), n, s));
+    if (kx && ky && kz && R.nact > 0) {
+        std::vector<int8_t> a(R.nact), b(R.nact), c(R.nact);
+        for (int i = 0; i < R.nact; i++) { a[i] = (int8_t)kx[i]; b[i] = (int8_t)ky[i]; c[i] = (int8_t)kz[i]; }
+        TRY(h2d(md->kx + R.off_act, a.data(), R.nact, s));
+        TRY(h2d(md->ky + R.off_act, b.data(), R.nact, s));
+        TRY(h2d(md->kz + R.off_act, c.data(), R.nact, s));
+        AC4_HIP(hipStreamSynchronize(s));
+    }
+    ChainS c;
+    TRY(get_chain(md, r, &c));  // also completes the copies above
+    c.S_end = keblk;
+    c.p_scale = 1.0;
+    c.force_rebuild = 1;
+    c.maxbits = 0ull;
+    c.step = 0;  // a loaded state starts a new run (MDcode/main.c:238-240: start = 0)
+    c.mode = 0;
+    TRY(put_chain(md, r, &c));
+    md->have_forces = false;
+    return AC4_OK;
+}
+
+extern "C" int ac4_md_get_state(ac4_md *md, int r, double *x, double *y, double *z, double *xu, double *yu,
+                                double *zu, double *px, double *py, double *pz, int *kx, int *ky, int *kz) {
+    TRY(check_rep(md, r));
+    const RepC &R = md->reps_h[r];
+    std::vector<int> id;
+    TRY(get_ids(md, r, id));
+    double *outs[] = {x, y, z, xu, yu, zu, px, py, pz};
+    const double *devs[] = {md->P.x, md->P.y, md->P.z, md->P.xu, md->P.yu, md->P.zu, md->P.px, md->P.py, md->P.pz};
+    for (int k = 0; k < 9; k++)
+        if (outs[k]) TRY(fetch_unpermuted(md, r, devs[k], outs[k], id));
+    ChainS ch;
+    TRY(get_chain(md, r, &ch));
+    const double ps = ch.p_scale;  // pending end-of-step U3 factor
+    if (ps != 1.0)
+        for (int i = 0; i < R.n; i++) {
+            if (px) px[i] = px[i] * ps;
+            if (py) py[i] = py[i] * ps;
+            if (pz) pz[i] = pz[i] * ps;
+        }
+    if ((kx || ky || kz) && R.nact > 0) {
+        std::vector<int8_t> a(R.nact), b(R.nact), c(R.nact);
+        TRY(d2h(a.data(), md->kx + R.off_act, R.nact, md->stream));
+        TRY(d2h(b.data(), md->ky + R.off_act, R.nact, md->stream));
+        TRY(d2h(c.data(), md->kz + R.off_act, R.nact, md->stream));
+        AC4_HIP(hipStreamSynchronize(md->stream));
+        for (int i = 0; i < R.nact; i++) {
+            if (kx) kx[i] = a[i];
+            if (ky) ky[i] = b[i];
+            if (kz) kz[i] = c[i];
+        }
+    }
+    return AC4_OK;
+}
+
+extern "C" int ac4_md_get_forces(ac4_md *md, int r, double *fx, double *fy, double *fz) {
+    TRY(check_rep(md, r));
+    std::vector<int> id;
+    TRY(get_ids(md, r, id));
+    if (fx) TRY(fetch_unpermuted(md, r, (const double *)md->fx, fx, id));
+    if (fy) TRY(fetch_unpermuted(md, r, (const double *)md->fy, fy, id));
+    if (fz) TRY(fetch_unpermuted(md, r, (const double *)md->fz, fz, id));
+    return AC4_OK;
+}
+
+extern "C" int ac4_md_set_chain(ac4_md *md, int r, const double *c9) {
+    TRY(check_rep(md, r));
+    AC4_REQUIRE(c9, AC4_EARG, "set_chain: NULL");
+    ChainS c;
+    TRY(get_chain(md, r, &c));
+    for (int k = 0; k < 3; k++) { c.eta[k] = c9[k]; c.peta[k] = c9[3 + k]; c.q[k] = c9[6 + k]; }
+    return put_chain(md, r, &c);
+}
+
+extern "C" int ac4_md_get_chain(ac4_md *md, int r, double *c9) {
+    TRY(check_rep(md, r));
+    ChainS c;
+    TRY(get_chain(md, r, &c));
+    for (int k = 0; k < 3; k++) { c9[k] = c.eta[k]; c9[3 + k] = c.peta[k]; c9[6 + k] = c.q[k]; }
+    return AC4_OK;
+}
+
+extern "C" int ac4_md_set_rebuild_criterion(ac4_md *md, int criterion) {
+    AC4_REQUIRE(md && (criterion == 0 || criterion == 1), AC4_EARG, "criterion must be 0 (safe) or 1 (reference)");
+    std::vector<ChainS> c;
+    TRY(get_chains(md, c));
+    for (auto &e : c) e.crit = criterion;
+    return put_chains(md, c);
+}
+
+static int launch_rebuild(ac4_md *md) {
+    hipStream_t s = md->stream;
+    const int cy = (md->max_ncell + BLK - 1) / BLK, sy = (md->max_ncell + BLK / 64 - 1) / (BLK / 64);
+    hipLaunchKernelGGL(k_cell_zero, dim3(md->R, cy), dim3(BLK), 0, s, md->reps, md->chain, md->cnt);
+    hipLaunchKernelGGL(k_bin, dim3(md->nblk), dim3(BLK), 0, s, md->reps, md->chain, md->blk_rep, md->blk_first,
+                       md->P.x, md->P.y, md->P.z, md->cnt, md->P.cell_of, md->rank, md->P.cell_ref);
+    hipLaunchKernelGGL(k_cell_scan, dim3(md->R), dim3(BLK), 0, s, md->reps, md->chain, md->cnt, md->start);
+    hipLaunchKernelGGL(k_scatter, dim3(md->nblk), dim3(BLK), 0, s, md->reps, md->chain, md->blk_rep, md->blk_first,
+                       md->P.cell_of, md->rank, md->start, md->members);
+    hipLaunchKernelGGL(k_cell_sort, dim3(md->R, sy), dim3(BLK), 0, s, md->reps, md->chain, md->cnt, md->start,
+                       md->P.id, md->members);
+    hipLaunchKernelGGL(k_gather, dim3(md->nblk), dim3(BLK), 0, s, md->reps, md->chain, md->blk_rep, md->blk_first,
+                       md->members, md->P, md->A);
+    hipLaunchKernelGGL(k_commit, dim3(md->nblk), dim3(BLK), 0, s, md->reps, md->chain, md->blk_rep, md->blk_first,
+                       md->A, md->P, md->pos4, md->xo, md->yo, md->zo);
+    hipLaunchKernelGGL(k_nlist, dim3(md->ncell_tot), dim3(NL_WAVE), 0, s, md->reps, md->chain, md->cblk_rep,
+                       md->cblk_cell, md->pos4, md->cnt, md->start, md->nn, md->nl, md->stride, md->cap);
+    AC4_HIP(hipGetLastError());
+    return AC4_OK;
+}
+
+template <bool KICK>
+static int launch_force(ac4_md *md) {
+    hipStream_t s = md->stream;
+    if (md->prec == AC4_FP32)
+        hipLaunchKernelGGL((k_force<true, KICK>), dim3(md->nblk), dim3(BLK), 0, s, md->reps, md->chain, md->blk_rep,
+                           md->blk_first, md->blk_idx, md->pos4, md->nn, md->nl, md->stride, md->P.act, md->kx,
+                           md->ky, md->kz, md->fx, md->fy, md->fz, md->P.px, md->P.py, md->P.pz, md->partial);
+    else
+        hipLaunchKernelGGL((k_force<false, KICK>), dim3(md->nblk), dim3(BLK), 0, s, md->reps, md->chain,
+                           md->blk_rep, md->blk_first, md->blk_idx, md->pos4, md->nn, md->nl, md->stride, md->P.act,
+                           md->kx, md->ky, md->kz, md->fx, md->fy, md->fz, md->P.px, md->P.py, md->P.pz,
+                           md->partial);
+    AC4_HIP(hipGetLastError());
+    return AC4_OK;
+}
+
+static void launch_shuffle(ac4_md *md, int force) {
+    hipLaunchKernelGGL(k_shuffle, dim3(md->R), dim3(64), 0, md->stream, md->reps, md->chain, md->kx, md->ky, md->kz,
+                       force);
+}
+static void launch_first_half(ac4_md *md) {
+    hipLaunchKernelGGL(k_first_half, dim3(md->nblk), dim3(BLK), 0, md->stream, md->reps, md->chain, md->blk_rep,
+                       md->blk_first, md->P.x, md->P.y, md->P.z, md->P.xu, md->P.yu, md->P.zu, md->P.px, md->P.py,
+                       md->P.pz, md->fx, md->fy, md->fz, md->xo, md->yo, md->zo, md->P.type, md->pos4, md->chain);
+}
+
+static int launch_step(ac4_md *md) {
+    launch_shuffle(md, 0);
+    launch_first_half(md);
+    AC4_HIP(hipGetLastError());
+    TRY(launch_rebuild(md));
+    return launch_force<true>(md);
+}
+
+static int check_err(ac4_md *md) {
+    std::vector<ChainS> c;
+    TRY(get_chains(md, c));
+    for (int r = 0; r < md->R; r++) {
+        AC4_REQUIRE(!(c[r].err & 1), AC4_ECAP, "replica %d: neighbour list exceeded %d entries", r, md->cap);
+        AC4_REQUIRE(!(c[r].err & 2), AC4_ESTATE, "replica %d: non-finite or runaway coordinate (run blew up)", r);
+    }
+    return AC4_OK;
+}
+
+static int force_rebuild_all(ac4_md *md) {
+    std::vector<ChainS> c;
+    TRY(get_chains(md, c));
+    for (auto &e : c) e.force_rebuild = 1;
+    TRY(put_chains(md, c));
+    TRY(launch_rebuild(md));
+    TRY(get_chains(md, c));
+    for (auto &e : c) { e.force_rebuild = 0; e.maxbits = 0ull; }
+    return put_chains(md, c);
+}
+
+extern "C" int ac4_md_update_cell_list(ac4_md *md, int r_out, int *cell_out, int *full_count_out) {
+    AC4_REQUIRE(md, AC4_EARG, "NULL handle");
+    TRY(force_rebuild_all(md));
+    if (r_out >= 0 && r_out < md->R) {
+        std::vector<int> id;
+        TRY(get_ids(md, r_out, id));
+        if (cell_out) TRY(fetch_unpermuted(md, r_out, (const int *)md->P.cell_ref, cell_out, id));
+        if (full_count_out) TRY(fetch_unpermuted(md, r_out, (const int *)md->nn, full_count_out, id));
+    }
+    return check_err(md);
+}
+
+extern "C" int ac4_md_calculate_forces(ac4_md *md, double *out2) {
+    AC4_REQUIRE(md, AC4_EARG, "NULL handle");
+    std::vector<ChainS> c;
+    TRY(get_chains(md, c));
+    bool stale = false;
+    for (auto &e : c) stale |= e.force_rebuild != 0;
+    if (stale) TRY(force_rebuild_all(md));
+    TRY(launch_force<false>(md));
+    TRY(get_chains(md, c));
+    if (out2)
+        for (int r = 0; r < md->R; r++) { out2[2 * r] = c[r].pe; out2[2 * r + 1] = c[r].vir; }
+    md->have_forces = true;
+    return check_err(md);
+}
+
+extern "C" int ac4_md_step(ac4_md *md, int nsteps) {
+    AC4_REQUIRE(md && nsteps >= 0, AC4_EARG, "bad arguments");
+    if (!md->have_forces) TRY(ac4_md_calculate_forces(md, nullptr));
+    for (int k = 0; k < nsteps; k++) TRY(launch_step(md));
+    return AC4_OK;
+}
+
+extern "C" int ac4_md_step_graph(ac4_md *md, int nsteps) {
+    AC4_REQUIRE(md && nsteps >= 0, AC4_EARG, "bad arguments");
+    if (!md->have_forces) TRY(ac4_md_calculate_forces(md, nullptr));
+    if (!md->gexec) {
+        AC4_HIP(hipStreamBeginCapture(md->stream, hipStreamCaptureModeThreadLocal));
+        int rc = launch_step(md);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(md->stream, &g);
+        if (rc != AC4_OK) return rc;
+        AC4_HIP(e);
+        md->graph = g;
+        AC4_HIP(hipGraphInstantiate(&md->gexec, g, nullptr, nullptr, 0));
+    }
+    for (int k = 0; k < nsteps; k++) AC4_HIP(hipGraphLaunch(md->gexec, md->stream));
+    return AC4_OK;
+}
+
+extern "C" int ac4_md_relax(ac4_md *md, int nsteps, double f_start, double f_end) {
+    AC4_REQUIRE(md && nsteps > 0 && f_start > 0 && f_end > 0, AC4_EARG, "relax: bad arguments");
+    if (!md->have_forces) TRY(ac4_md_calculate_forces(md, nullptr));
+    std::vector<ChainS> c;
+    TRY(get_chains(md, c));
+    for (auto &e : c) { e.mode = 1; e.ramp_n = nsteps; e.ramp_k = 0; e.ramp0 = f_start; e.ramp1 = f_end; }
+    TRY(put_chains(md, c));
+    for (int k = 0; k < nsteps; k++) TRY(launch_step(md));
+    return check_err(md);
+}
+
+// Eager steps with HIP events (on the handle's stream) around each launch
+// group; ms4 = average ms per step of {first_half, rebuild group, force, shuffle}.
+extern "C" int ac4_md_kernel_times(ac4_md *md, int nsteps, double *ms4) {
+    AC4_REQUIRE(md && nsteps > 0 && ms4, AC4_EARG, "kernel_times: bad arguments");
+    if (!md->have_forces) TRY(ac4_md_calculate_forces(md, nullptr));
+    hipStream_t s = md->stream;
+    hipEvent_t ev[5];
+    for (auto &e : ev) AC4_HIP(hipEventCreate(&e));
+    double acc[4] = {0, 0, 0, 0};
+    for (int k = 0; k < nsteps; k++) {
+        AC4_HIP(hipEventRecord(ev[0], s));
+        launch_shuffle(md, 0);
+        AC4_HIP(hipEventRecord(ev[1], s));
+        launch_first_half(md);
+        AC4_HIP(hipEventRecord(ev[2], s));
+        TRY(launch_rebuild(md));
+        AC4_HIP(hipEventRecord(ev[3], s));
+        TRY(launch_force<true>(md));
+        AC4_HIP(hipEventRecord(ev[4], s));
+        AC4_HIP(hipEventSynchronize(ev[4]));
+        float t[4];
+        for (int q = 0; q < 4; q++) AC4_HIP(hipEventElapsedTime(&t[q], ev[q], ev[q + 1]));
+        acc[3] += t[0]; acc[0] += t[1]; acc[1] += t[2]; acc[2] += t[3];
+    }
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    for (int q = 0; q < 4; q++) ms4[q] = acc[q] / nsteps;
+    return check_err(md);
+}
+
+extern "C" int ac4_md_shuffle(ac4_md *md) {
+    AC4_REQUIRE(md, AC4_EARG, "NULL handle");
+    launch_shuffle(md, 1);
+    AC4_HIP(hipGetLastError());
+    AC4_HIP(hipStreamSynchronize(md->stream));
+    return AC4_OK;
+}
+
+extern "C" int ac4_md_shuffle_uniforms(ac4_md *md, int r, double *u) {
+    TRY(check_rep(md, r));
+    AC4_REQUIRE(u, AC4_EARG, "NULL output");
+    ChainS c;
+    TRY(get_chain(md, r, &c));
+    const RepC &R = md->reps_h[r];
+    for (int comp = 0; comp < 3; comp++)
+        for (int i = 0; i < R.nact; i++)
+            u[comp * R.nact + i] = ac4_shuffle_uniform(R.key0, R.key1, (uint64_t)c.shuffles, comp, i);
+    return AC4_OK;
+}
+
+extern "C" int ac4_md_stats(ac4_md *md, int r, double *out8) {
+    TRY(check_rep(md, r));
+    ChainS c;
+    TRY(get_chain(md, r, &c));
+    const RepC &R = md->reps_h[r];
+    out8[0] = c.pe;
+    out8[1] = c.vir;
+    out8[2] = c.ke;
+    out8[3] = c.inst_T;
+    out8[4] = R.density * c.inst_T + c.vir / (R.L * R.L * R.L);  // print.c:83
+    out8[5] = c.drift;
+    out8[6] = c.heff;
+    out8[7] = (double)c.step;
+    return AC4_OK;
+}
+
+extern "C" int ac4_md_rebuilds(ac4_md *md, int r) {
+    TRY(check_rep(md, r));
+    ChainS c;
+    TRY(get_chain(md, r, &c));
+    return c.rebuilds;
+}
+
+extern "C" int ac4_md_sync(ac4_md *md) {
+    AC4_REQUIRE(md, AC4_EARG, "NULL handle");
+    AC4_HIP(hipStreamSynchronize(md->stream));
+    return check_err(md);
+}
+
+extern "C" int ac4_philox4x32(const uint32_t *ctr4, const uint32_t *key2, uint32_t *out4) {
+    AC4_REQUIRE(ctr4 && key2 && out4, AC4_EARG, "NULL argument");
+    ac4_philox(ctr4, key2, out4);
+    return AC4_OK;
+}
+
+// Reference-order device copies of x_u and type of replica r for in-situ
+// analysis (ac4_analysis.hip); the copies are complete when this returns.
+extern "C" int ac4_md_device_xu(ac4_md *md, int r, const double **xu, const double **yu, const double **zu,
+                                const int **type, int *n) {
+    TRY(check_rep(md, r));
+    if (!md->ex) {
+        TRY(dalloc(&md->ex, md->ntot)); TRY(dalloc(&md->ey, md->ntot)); TRY(dalloc(&md->ez, md->ntot));
+        TRY(dalloc(&md->et, md->ntot));
+    }
+    hipLaunchKernelGGL(k_export, dim3(md->nblk), dim3(BLK), 0, md->stream, md->reps, md->blk_rep, md->blk_first,
+                       md->P.id, md->P.xu, md->P.yu, md->P.zu, md->P.type, md->ex, md->ey, md->ez, md->et, r);
+    AC4_HIP(hipGetLastError());
+    AC4_HIP(hipStreamSynchronize(md->stream));
+    const RepC &R = md->reps_h[r];
+    *xu = md->ex + R.off;
+    *yu = md->ey + R.off;
+    *zu = md->ez + R.off;
+    *type = md->et + R.off;
+    *n = R.n;
+    return AC4_OK;
+}

@@ -1,0 +1,216 @@
+/*
+ * ac4.h -- C ABI of the MI355X-native activeChi4 hot path (libac4.so).
+ *
+ * Plain pointers and sizes only.  Every entry point returns an int status:
+ * AC4_OK (0) or a negative AC4_E* code; ac4_last_error() gives the message.
+ * Nothing calls exit().  Host arrays are caller-owned and copied in/out;
+ * device state is owned by the handle.  One host thread drives one handle.
+ *
+ * The reference has no plugin or FFI layer: its boundary is two executables
+ * (MDcode `run`, Analysis `analyse`) plus text files, and inside them the
+ * global-state functions cited below.  Each entry point names the reference
+ * routine it replaces (paths relative to the reference checkout).
+ */
+#ifndef AC4_H
+#define AC4_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AC4_OK 0
+#define AC4_EARG -1      /* bad argument */
+#define AC4_EHIP -2      /* HIP runtime error */
+#define AC4_ECAP -3      /* capacity exceeded (neighbour list, cell) */
+#define AC4_ESTATE -4    /* call out of order / missing state */
+#define AC4_EIO -5       /* file I/O or parse error */
+
+#define AC4_FP64 0       /* plain mode: every force in FP64 */
+#define AC4_FP32 1       /* pair forces in FP32, compensated per-particle sums */
+
+#define AC4_NQ 162       /* lattice q vectors of Analysis/S4Qt.c:297-356 */
+
+const char *ac4_last_error(void);
+/* Number of visible HIP devices (0 when none): lets callers fail loudly. */
+int ac4_device_count(void);
+
+/* ------------------------------------------------------------------ MD --
+ * One handle holds R >= 1 independent replicas (R = 1 is the plain single
+ * run).  Replicas are packed into every launch: one launch sequence per step
+ * advances all of them.  A replica's results do not depend on which other
+ * replicas share the handle (per-replica block partition, fixed-order
+ * reductions, counter-based RNG keyed by the replica's seed).              */
+typedef struct {
+    int n;            /* particles (MDcode/global.h:15, compile-time there) */
+    double density;   /* DENSITY, para_file line 4 (MDcode/main.c:421-422) */
+    double T;         /* thermostat temperature, para_file line 4 */
+    double f0;        /* active force magnitude, para_file line 6 (:423-425) */
+    int n_act;        /* active particles; < 0 -> (int)(n*0.1) (global.h:55-56) */
+    int tau_per;      /* steps between direction shuffles (global.h:58) */
+    uint64_t seed;    /* Philox-4x32-10 key of this replica's RNG stream */
+} ac4_replica_params;
+
+typedef struct ac4_md ac4_md;
+
+/* Allocates device state for all replicas.  precision: AC4_FP64 / AC4_FP32.
+ * Replaces the set-up of MDcode/initialise.c:203-227 + Parallel.c:6-106
+ * (the whole box of every replica is resident on one GPU). */
+int ac4_md_create(int n_rep, const ac4_replica_params *p, int precision,
+                  ac4_md **out);
+int ac4_md_destroy(ac4_md *md);
+/* Number of particles of every replica, summed. */
+int ac4_md_total_particles(const ac4_md *md);
+
+/* State of replica r in the reference's particle order (SPHERE fields of
+ * MDcode/global.h:62-68): wrapped box-scaled x,y,z in [0,1), unwrapped
+ * box-scaled x_u,y_u,z_u, momenta p (m = 1), type (0 = A, 1 = B), actTag
+ * (-1 or active index), and the active directions kx,ky,kz (+-1, length
+ * n_act; MDcode/active.c:30-41).  set_state also resets the neighbour-list
+ * reference positions (xold = x_u), marks the list stale and restarts the
+ * replica's step counter at 0 (the shuffle schedule counts from there).  Pass NULL for
+ * act_tag/k* to keep the current ones. */
+int ac4_md_set_state(ac4_md *md, int r, const double *x, const double *y,
+                     const double *z, const double *xu, const double *yu,
+                     const double *zu, const double *px, const double *py,
+                     const double *pz, const int *type, const int *act_tag,
+                     const int *kx, const int *ky, const int *kz);
+int ac4_md_get_state(ac4_md *md, int r, double *x, double *y, double *z,
+                     double *xu, double *yu, double *zu, double *px,
+                     double *py, double *pz, int *kx, int *ky, int *kz);
+/* Forces of replica r (real units) from the last force evaluation. */
+int ac4_md_get_forces(ac4_md *md, int r, double *fx, double *fy, double *fz);
+/* Nose-Hoover chain {eta[3], p_eta[3], Q[3]} of replica r
+ * (MDcode/time_evolve.c:476-493; the caller draws p_eta). */
+int ac4_md_set_chain(ac4_md *md, int r, const double *chain9);
+int ac4_md_get_chain(ac4_md *md, int r, double *chain9);
+
+/* update_cell_list() (MDcode/cell_list+forces.c:58-148): bins every replica
+ * with the reference's cell grid (Parallel.c:72-80) and builds the neighbour
+ * lists at ((dx*dx+dy*dy)+dz*dz)*L*L < 2.85^2.  Optional outputs for replica
+ * r_out (reference particle order): cell index and the full neighbour count
+ * (number of list partners, both directions). */
+int ac4_md_update_cell_list(ac4_md *md, int r_out, int *cell_out,
+                            int *full_count_out);
+/* calculate_forces() (MDcode/cell_list+forces.c:153-227) on the current
+ * lists; out2 per replica = {pe, virial/3} (reference sign conventions). */
+int ac4_md_calculate_forces(ac4_md *md, double *out2);
+/* nsteps x { ShuffleActDirec() when step % tau_per == 0 (MDcode/main.c:245-247);
+ * AdvanceTimeNVTNoseHoover() (MDcode/time_evolve.c:495-526) } for every
+ * replica.  Lists are rebuilt on device when 4*maxD^2 > skin^2 (safe form of
+ * time_evolve.c:508); criterion = 1 selects the reference's 3*maxD^2.
+ * Requires forces (ac4_md_calculate_forces or a previous step). */
+int ac4_md_step(ac4_md *md, int nsteps);
+/* Same as ac4_md_step but records the per-step launch sequence once in a
+ * hipGraph and replays it (batched serving path). */
+int ac4_md_step_graph(ac4_md *md, int nsteps);
+int ac4_md_set_rebuild_criterion(ac4_md *md, int criterion);
+/* coldstart()/breakxal() (MDcode/initialise.c:166-201): nsteps of velocity
+ * Verlet with the momenta rescaled every step to a target temperature ramped
+ * linearly from f_start*T to f_end*T (each replica its own T); no thermostat
+ * chain, no direction shuffles, step counter unchanged.  Used to bring a
+ * lattice start to temperature before Nose-Hoover dynamics. */
+int ac4_md_relax(ac4_md *md, int nsteps, double f_start, double f_end);
+/* ShuffleActDirec() (MDcode/active.c:45-85) now, for every replica, with the
+ * device Philox stream; shuffle counter advances by one. */
+int ac4_md_shuffle(ac4_md *md);
+/* The uniforms the next shuffle of replica r will draw: 3*n_act doubles
+ * (component-major), for checking the swap rule against the oracle. */
+int ac4_md_shuffle_uniforms(ac4_md *md, int r, double *u);
+/* print_stuff() quantities (MDcode/print.c:60-98) for replica r:
+ * out8 = {pe, virial, ke, inst_T, pressure, drift (sum px), Heff, steps}. */
+int ac4_md_stats(ac4_md *md, int r, double *out8);
+/* Neighbour-list rebuilds performed by replica r so far. */
+int ac4_md_rebuilds(ac4_md *md, int r);
+/* Measurement hook (no reference counterpart): nsteps eager steps timed with
+ * HIP events on the handle's stream around each launch group; ms4 = average
+ * milliseconds per step of {first half, rebuild group, force, shuffle}. */
+int ac4_md_kernel_times(ac4_md *md, int nsteps, double *ms4);
+/* Waits for all device work of the handle; reports deferred device errors
+ * (list/cell overflow). */
+int ac4_md_sync(ac4_md *md);
+/* Raw Philox-4x32-10 blocks (known-answer tests): out4 = philox(ctr4, key2). */
+int ac4_philox4x32(const uint32_t *ctr4, const uint32_t *key2, uint32_t *out4);
+
+/* ------------------------------------------------------------ Analysis --
+ * Streaming per-frame analysis over all time origins (Analysis/main.c:53-62).
+ * The (origin, lag) walk of Analysis/msd_pos.c:44-107 / overlap.c:42-106 is
+ * computed on the host from the frame index list; frames are pushed in file
+ * order, origin frames stay resident, every pushed frame is compared on the
+ * GPU with all origins whose walk it completes.                            */
+typedef struct ac4_ana ac4_ana;
+
+#define AC4_OBS_MSD 1
+#define AC4_OBS_OVERLAP 2
+#define AC4_OBS_S4 4
+#define AC4_OBS_FS 8
+
+/* fidx: frame step indices (time_index_file.dat, read_all_data.c:14-31);
+ * gap: origin spacing (0 -> detected as read_all_data.c:36-48);
+ * factor: lag growth (1.4, Analysis/main.c:4); a: overlap length (0.3);
+ * obs: AC4_OBS_* mask; type_flag: the reference's `flag` argument
+ * (1 -> exclude type-1 particles, msd_pos.c:62, overlap.c:59, Fsqt.c:104). */
+int ac4_ana_create(int n, double L, int nframes, const int *fidx, int gap,
+                   double factor, double a, int obs, int type_flag,
+                   ac4_ana **out);
+int ac4_ana_destroy(ac4_ana *an);
+int ac4_ana_gap(const ac4_ana *an);
+/* Frame at position `pos` of the index list (LoadFramePP, read_all_data.c:129-150),
+ * unwrapped box-scaled coordinates, host arrays of n. Frames must arrive in
+ * increasing pos. */
+int ac4_ana_push_frame(ac4_ana *an, int pos, const double *x, const double *y,
+                       const double *z, const int *type);
+/* Same from device arrays (in-situ analysis of a running MD handle). */
+int ac4_ana_push_frame_device(ac4_ana *an, int pos, const double *dx,
+                              const double *dy, const double *dz,
+                              const int *dtype);
+/* Pushes replica r's current x_u of an MD handle as frame `pos` (in-situ,
+ * no host round trip). */
+int ac4_ana_push_md(ac4_ana *an, int pos, ac4_md *md, int r);
+/* Accumulates the per-(origin,lag) results into slots in the reference's
+ * origin order (msd_pos.c:87-93, overlap.c:89-94, S4Qt.c:228-236). */
+int ac4_ana_finalize(ac4_ana *an);
+/* Slot arrays (length nslots; slot 0 unused as in the reference):
+ * MSD: m0=<r^2>, m1=<sum_a da^4>, m2=<r^4> sums, counters, delta_t. */
+int ac4_ana_nslots(const ac4_ana *an, int obs);
+int ac4_ana_get_msd(ac4_ana *an, double *m0, double *m1, double *m2,
+                    int *counters, double *delta_t, int *num_intervals);
+int ac4_ana_get_overlap(ac4_ana *an, double *c, double *x4, int *counters,
+                        double *delta_t, int *num_intervals);
+/* S4 / Fs slot arrays are [nslots][162]. qval[162] = Q column (|q|^2). */
+int ac4_ana_get_phase(ac4_ana *an, int obs, double *cs, double *ss,
+                      double *cs2, double *ss2, int *counters,
+                      double *delta_t, double *qval);
+/* Per-pair raw results in walk order (rule of `obs`): MSD 3 sums, overlap
+ * count, S4 masked count, S4/Fs {cos[162], sin[162]}. */
+int ac4_ana_npairs(const ac4_ana *an, int obs);
+int ac4_ana_get_pairs(ac4_ana *an, int obs, double *vals, int *ivals);
+/* Writes msd.dat, relax_a_%.2lf.dat, S4qt_a_%.2lf.dat, Fsqt_a_%.2lf.dat into
+ * sub_dir with the reference's formats (msd_pos.c:112-120, overlap.c:108-121,
+ * S4Qt.c:251-271, Fsqt.c:257-278). */
+int ac4_ana_write(ac4_ana *an, const char *sub_dir);
+
+/* ------------------------------------------------------ host-side logic */
+/* prepareSavingArray (MDcode/main.c:62-119) incl. its shellSort (:17-41):
+ * returns the frame count; out[0..count] as the run loop reads it. */
+int ac4_schedule(uint64_t run_length, int n_origins, double factor,
+                 uint64_t *out, int cap);
+/* Seeded initial state in the reference's SPHERE layout: simple-cubic sites
+ * chosen at random (MDcode/initialise.c:34-62), 80:20 species (:63-72),
+ * Maxwell-Boltzmann momenta at T with zero drift, n_act active particles with
+ * the initial +-(1,1,1) directions of MDcode/active.c:24-41.  Replaces the
+ * time()-seeded GetFccGrid/selectRandomActiveParticles; xu/act_tag/k* may be
+ * NULL. */
+int ac4_init_lattice(int n, double T, int n_act, uint64_t seed, double *x, double *y, double *z, double *xu,
+                     double *yu, double *zu, double *px, double *py, double *pz, int *type, int *act_tag, int *kx,
+                     int *ky, int *kz);
+/* read_all_data.c:36-48 */
+int ac4_origin_gap(const int *fidx, int nframes);
+/* Ensemble chi4 over replicas (no reference counterpart, north-star
+ * capability): q[r*nor+o]; out2 = {pooled Nbar*(<Q^2>-<Q>^2), mean_r N_r*Var_o Q_r}. */
+int ac4_chi4_ensemble(int nrep, int nor, const double *q, const int *n_r,
+                      double *out2);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

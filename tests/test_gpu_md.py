@@ -1,0 +1,181 @@
+"""GPU parity of the MD hot path (libac4.so through the C ABI) against the
+oracle: bit-exact cells and neighbour counts, FP64 forces within 1e-12,
+trajectories, shuffle rule, batch independence, graph replay."""
+import numpy as np
+import pytest
+
+import oracle_lib as OL
+from test_oracle_md import lattice_state, _rel_force_err
+
+pytestmark = pytest.mark.gpu
+
+
+def make_md(ac4, states, f0=1.5, T=0.6, tau=0, prec=0, seeds=None):
+    ps = []
+    for k, (s, L) in enumerate(states):
+        ps.append(ac4.ReplicaParams(len(s["x"]), 1.2, T, f0, -1, tau, (seeds or {}).get(k, 1000 + k)))
+    md = ac4.MD(ps, precision=prec)
+    for k, (s, L) in enumerate(states):
+        md.set_state(k, s["x"], s["y"], s["z"], s["xu"], s["yu"], s["zu"], s["px"], s["py"], s["pz"], s["type"],
+                     s["act_tag"], s["kx"], s["ky"], s["kz"])
+    return md
+
+
+_RELAXED = {}
+
+
+def relaxed_state(ac4, n, seed, T=0.6, f0=1.5):
+    """Lattice start brought to temperature on the GPU (ac4_md_relax), as the
+    reference does with coldstart/breakxal before Nose-Hoover dynamics."""
+    key = (n, seed, T, f0)
+    if key not in _RELAXED:
+        s, L = lattice_state(n, T=T, seed=seed)
+        md = make_md(ac4, [(s, L)], f0=f0, T=T)
+        md.relax(300, 1.5, 1.0)
+        g = md.get_state(0)
+        st = dict(s)
+        st.update({k: g[k] for k in ("x", "y", "z", "xu", "yu", "zu", "px", "py", "pz")})
+        md.close()
+        _RELAXED[key] = (st, L)
+    return _RELAXED[key]
+
+
+def make_relaxed_md(ac4, specs, tau, prec, f0=1.5, T=0.6):
+    """specs: list of (n, state seed, rng seed)."""
+    states = [relaxed_state(ac4, n, sd, T, f0) for n, sd, _ in specs]
+    md = make_md(ac4, states, f0=f0, T=T, tau=tau, prec=prec, seeds={k: rs for k, (_, _, rs) in enumerate(specs)})
+    for k, (n, sd, rs) in enumerate(specs):
+        md.init_chain(k, seed=sd)
+    return md
+
+
+def test_relax_reaches_target_temperature(gpu):
+    s, L = lattice_state(1500, T=0.5, seed=8)
+    md = make_md(gpu, [(s, L)], f0=1.0, T=0.5)
+    md.relax(200, 2.0, 1.0)
+    st = md.stats(0)
+    assert st["T"] == pytest.approx(0.5, rel=1e-12) and st["steps"] == 0
+    md.init_chain(0)
+    md.step(200)
+    md.sync()
+    assert 0.3 < md.stats(0)["T"] < 0.8
+
+
+@pytest.mark.parametrize("n,jit", [(1000, 0.05), (1000, 0.4), (2000, 0.05), (10000, 0.1), (875, 0.2)])
+def test_cells_and_full_counts_bit_exact(gpu, n, jit):
+    s, L = lattice_state(n, jitter=jit, seed=n + 1)
+    md = make_md(gpu, [(s, L)])
+    cell, cnt = md.update_cell_list(0)
+    _, _, ocell = OL.half_list(s["x"], s["y"], s["z"], L)
+    assert np.array_equal(cell, ocell)
+    assert np.array_equal(cnt, OL.full_counts(s["x"], s["y"], s["z"], L))
+
+
+@pytest.mark.parametrize("n,f0", [(1000, 1.5), (2000, 0.0), (4000, 2.0)])
+def test_forces_fp64_match_oracle(gpu, n, f0):
+    s, L = lattice_state(n, jitter=0.06, seed=n + 7)
+    md = make_md(gpu, [(s, L)], f0=f0)
+    pv = md.calculate_forces()[0]
+    cnt, lst, _ = OL.half_list(s["x"], s["y"], s["z"], L)
+    fx, fy, fz, o = OL.forces_list(s, L, f0, cnt, lst)
+    assert _rel_force_err(md.forces(0), (fx, fy, fz)) < 1e-12
+    assert pv[0] == pytest.approx(o[0], rel=1e-12)
+    assert pv[1] == pytest.approx(o[1], rel=1e-12)
+
+
+def test_forces_fp32_error_bound(gpu):
+    errs = []
+    for seed in range(4):
+        s, L = lattice_state(3000, jitter=0.06, seed=50 + seed)
+        md = make_md(gpu, [(s, L)], prec=1)
+        md.calculate_forces()
+        cnt, lst, _ = OL.half_list(s["x"], s["y"], s["z"], L)
+        fx, fy, fz, o = OL.forces_list(s, L, 1.5, cnt, lst)
+        errs.append(_rel_force_err(md.forces(0), (fx, fy, fz)))
+    assert max(errs) <= 1e-5, errs
+
+
+def _chain0(n, T):
+    q = np.zeros(3)
+    OL.load().or_nhc_masses(n, T, OL.dp(q))
+    return np.concatenate([np.zeros(3), np.sqrt(T * q) * np.array([0.4, -0.3, 0.2]), q])
+
+
+def test_nhc_trajectory_matches_oracle(gpu):
+    n, T, f0 = 1000, 0.6, 1.5
+    s, L = lattice_state(n, T=T, jitter=0.02, seed=21)
+    # relax on the oracle so the compared window is ordinary dynamics
+    s, ch, _ = OL.nhc_steps(s, L, T, f0, 200, _chain0(n, T))
+    s = {k: v for k, v in s.items() if k not in ("fx", "fy", "fz")}
+    md = make_md(gpu, [(s, L)], f0=f0, T=T)
+    md.set_chain(0, ch)
+    md.step(60)
+    g = md.get_state(0)
+    o, och, oout = OL.nhc_steps(s, L, T, f0, 60, ch)
+    for k in ("xu", "yu", "zu"):
+        assert np.abs(g[k] - o[k]).max() < 1e-11
+    for k in ("px", "py", "pz"):
+        assert np.abs(g[k] - o[k]).max() < 1e-9
+    assert np.allclose(md.get_chain(0), och, rtol=1e-10, atol=1e-12)
+    st = md.stats(0)
+    assert st["Heff"] == pytest.approx(oout[4], rel=1e-10)
+    assert st["ke"] == pytest.approx(oout[2], rel=1e-10)
+    assert st["pe"] == pytest.approx(oout[0], rel=1e-10)
+    assert md.rebuilds(0) >= 1
+
+
+def test_shuffle_follows_reference_swap_rule(gpu):
+    s, L = lattice_state(2000, seed=4)
+    md = make_md(gpu, [(s, L)], tau=200)
+    before = md.get_state(0)
+    u = md.shuffle_uniforms(0)
+    md.shuffle()
+    after = md.get_state(0)
+    for c, k in enumerate(("kx", "ky", "kz")):
+        ref = np.array(before[k], dtype=np.int32)
+        OL.load().or_shuffle(len(ref), OL.ip(ref), OL.dp(np.ascontiguousarray(u[c])))
+        assert np.array_equal(after[k], ref)
+        assert after[k].sum() == 0 and set(after[k].tolist()) <= {-1, 1}
+    assert not np.array_equal(after["kx"], before["kx"])
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_replica_alone_equals_in_batch(gpu, prec):
+    alone = make_relaxed_md(gpu, [(1000, 31, 77)], tau=20, prec=prec)
+    alone.step(150)
+    alone.sync()
+    mixed = make_relaxed_md(gpu, [(1500, 32, 5), (1000, 31, 77), (2600, 33, 9)], tau=20, prec=prec)
+    mixed.step(150)
+    mixed.sync()
+    a, m = alone.get_state(0), mixed.get_state(1)
+    for k in ("xu", "yu", "zu", "px", "py", "pz", "kx"):
+        assert np.array_equal(a[k], m[k]), k
+    assert alone.stats(0)["Heff"] == mixed.stats(1)["Heff"]
+    assert alone.rebuilds(0) == mixed.rebuilds(1) > 0
+
+
+def test_graph_replay_equals_eager(gpu):
+    spec = [(1200, 41, 1), (2000, 42, 2)]
+    e = make_relaxed_md(gpu, spec, tau=25, prec=1)
+    g = make_relaxed_md(gpu, spec, tau=25, prec=1)
+    e.step(100)
+    e.sync()
+    g.step(60, graph=True)
+    g.sync()
+    g.step(40, graph=True)
+    g.sync()
+    for r in range(2):
+        se, sg = e.get_state(r), g.get_state(r)
+        for k in ("xu", "px", "kz"):
+            assert np.array_equal(se[k], sg[k])
+
+
+def test_stats_consistent_with_state(gpu):
+    md = make_relaxed_md(gpu, [(1000, 51, 3)], tau=50, prec=0)
+    md.step(30)
+    md.sync()
+    st, g = md.stats(0), md.get_state(0)
+    ke = 0.5 * (g["px"] ** 2 + g["py"] ** 2 + g["pz"] ** 2).sum()
+    assert st["ke"] == pytest.approx(ke, rel=1e-12)
+    assert st["drift"] == pytest.approx(g["px"].sum(), abs=1e-9)
+    assert st["steps"] == 30

@@ -1,0 +1,83 @@
+"""C-ABI library checks that need no GPU: exported symbols, loud failure
+without a device, host-side logic (schedule, origin gap, walk) against the
+oracle, Philox known-answer vectors."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle_lib as OL
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared():
+    names = set()
+    for h in os.listdir(os.path.join(ROOT, "include")):
+        if h.endswith(".h"):
+            txt = open(os.path.join(ROOT, "include", h)).read()
+            txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+            names |= set(re.findall(r"\b(ac4_\w+)\s*\(", txt))
+    return names
+
+
+def test_library_exports_every_declared_symbol(ac4):
+    out = subprocess.check_output(["nm", "-D", "--defined-only", ac4.LIB_PATH]).decode()
+    exported = set(re.findall(r" T (ac4_\w+)", out))
+    missing = declared() - exported
+    assert not missing, missing
+    assert len(declared()) >= 35
+
+
+def test_no_silent_cpu_path(ac4):
+    if ac4.lib.ac4_device_count() > 0:
+        pytest.skip("a GPU is visible")
+    p = ac4.ReplicaParams(1000, 1.2, 0.6, 1.5, -1, 200, 1)
+    with pytest.raises(ac4.AC4Error, match="no HIP device"):
+        ac4.MD([p])
+    with pytest.raises(ac4.AC4Error, match="no HIP device"):
+        ac4.Analysis(100, 4.4, [0, 1, 2, 3, 4, 5, 20, 21, 22, 23, 24, 25])
+
+
+@pytest.mark.parametrize("ctr,key,want", [
+    ([0, 0, 0, 0], [0, 0], [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]),
+    ([0xffffffff] * 4, [0xffffffff] * 2, [0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd]),
+    ([0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344], [0xa4093822, 0x299f31d0],
+     [0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1]),
+])
+def test_philox_known_answers(ac4, ctr, key, want):
+    # Random123 philox4x32-10 known-answer vectors (Salmon et al. 2011)
+    assert ac4.philox(ctr, key).tolist() == want
+
+
+@pytest.mark.parametrize("run", [400, 1000, 10000, 20000, 50000, 100000, 123457])
+def test_product_schedule_matches_oracle(ac4, oracle, run):
+    a, araw = ac4.schedule(run)
+    b, braw = OL.schedule(run)
+    assert np.array_equal(a, b)
+    assert int(araw[len(a)]) == int(braw[len(b)])
+
+
+def test_product_walk_matches_oracle(ac4, oracle):
+    import trajgen
+    for fr, gap in ((trajgen.case_frames(3, 30), 30), (ac4.schedule(10000)[0].astype(np.int64).tolist(), 50)):
+        f = np.array(fr, dtype=np.int32)
+        assert ac4.origin_gap(f) == gap == oracle.or_origin_gap(OL.ip(f), len(f))
+
+
+def test_init_lattice_invariants(ac4):
+    n, T = 1000, 0.5
+    s = ac4.init_lattice(n, T, 42)
+    assert (s["type"] == 1).sum() == n - int(0.8 * n)
+    p2 = (s["px"] ** 2 + s["py"] ** 2 + s["pz"] ** 2).sum()
+    assert p2 / (3 * n - 3) == pytest.approx(T, rel=1e-12)
+    assert abs(s["px"].sum()) < 1e-9 and abs(s["py"].sum()) < 1e-9
+    assert sorted(s["act_tag"][s["act_tag"] >= 0].tolist()) == list(range(100))
+    assert s["kx"].sum() == 0 and set(s["kx"].tolist()) == {1, -1}
+    pos = np.stack([s["x"], s["y"], s["z"]], 1)
+    assert len({tuple(p) for p in pos.tolist()}) == n  # distinct lattice sites
+    assert np.all((pos >= 0) & (pos < 1))
+    s2 = ac4.init_lattice(n, T, 42)
+    assert np.array_equal(s["x"], s2["x"]) and np.array_equal(s["px"], s2["px"])
