@@ -39,6 +39,12 @@ class AC4Error(RuntimeError):
     pass
 
 
+class Job(ctypes.Structure):
+    """ac4_job of include/ac4.h."""
+    _fields_ = [("n", ctypes.c_int), ("T", ctypes.c_double), ("f0", ctypes.c_double), ("tau_p", ctypes.c_double),
+                ("steps", ctypes.c_long), ("seed", ctypes.c_uint64), ("t_arrive", ctypes.c_double)]
+
+
 _SIGS = {
     "ac4_last_error": (ctypes.c_char_p, []),
     "ac4_device_count": (ctypes.c_int, []),
@@ -83,6 +89,28 @@ _SIGS = {
     "ac4_schedule": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int, ctypes.c_double, _U64, ctypes.c_int]),
     "ac4_origin_gap": (ctypes.c_int, [_I, ctypes.c_int]),
     "ac4_chi4_ensemble": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _D, _I, _D]),
+    "ac4_set_tuning": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
+    "ac4_md_create_slots": (ctypes.c_int, [ctypes.c_int, _I, ctypes.c_int, ctypes.POINTER(_VP)]),
+    "ac4_md_assign": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.POINTER(ReplicaParams)]),
+    "ac4_md_release": (ctypes.c_int, [_VP, ctypes.c_int]),
+    "ac4_md_slot_n": (ctypes.c_int, [_VP, ctypes.c_int]),
+    "ac4_md_slot_offset": (ctypes.c_int, [_VP, ctypes.c_int]),
+    "ac4_md_load_slot": (ctypes.c_int, [_VP, ctypes.c_int] + [_D] * 9 + [_I] * 5),
+    "ac4_md_relax_slot": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double]),
+    "ac4_md_export_all": (ctypes.c_int, [_VP, _D, _D, _D]),
+    "ac4_md_overlap_enable": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                             ctypes.c_double]),
+    "ac4_md_overlap_sample": (ctypes.c_int, [_VP]),
+    "ac4_md_overlap_get": (ctypes.c_int, [_VP, ctypes.c_int, _D, _D, _I]),
+    "ac4_server_create": (ctypes.c_int, [ctypes.c_int, _I, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                         ctypes.POINTER(_VP)]),
+    "ac4_server_destroy": (ctypes.c_int, [_VP]),
+    "ac4_server_submit": (ctypes.c_int, [_VP, ctypes.POINTER(Job)]),
+    "ac4_server_tick": (ctypes.c_int, [_VP, ctypes.c_double]),
+    "ac4_server_pending": (ctypes.c_int, [_VP]),
+    "ac4_server_job": (ctypes.c_int, [_VP, ctypes.c_int, _D, _D, ctypes.POINTER(ctypes.c_long), _I, _I]),
+    "ac4_server_result": (ctypes.c_int, [_VP, ctypes.c_int, _D, _D, _D]),
+    "ac4_server_particle_steps": (ctypes.c_longlong, [_VP]),
     "ac4_init_lattice": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_uint64]
                          + [_D] * 9 + [_I] * 5),
 }
@@ -250,6 +278,20 @@ class MD:
     def sync(self):
         _check(lib.ac4_md_sync(self.h))
 
+    def overlap_enable(self, sample_every, origin_every, n_origin, n_lags, a=0.3):
+        self.ov = (sample_every, n_lags)
+        _check(lib.ac4_md_overlap_enable(self.h, sample_every, origin_every, n_origin, n_lags, a))
+
+    def overlap_sample(self):
+        _check(lib.ac4_md_overlap_sample(self.h))
+
+    def overlap(self, r):
+        nl = self.ov[1] + 1
+        q, q2 = np.empty(nl), np.empty(nl)
+        c = np.empty(nl, dtype=np.int32)
+        _check(lib.ac4_md_overlap_get(self.h, r, _dp(q), _dp(q2), _ip(c)))
+        return q, q2, c
+
     def kernel_times(self, nsteps):
         """Average ms per step of {first_half, rebuild, force, shuffle} (HIP events)."""
         o = np.empty(4)
@@ -340,6 +382,69 @@ class Analysis:
 
     def write(self, sub_dir):
         _check(lib.ac4_ana_write(self.h, sub_dir.encode()))
+
+
+class Server:
+    """Request loop over device-resident slots (ac4_server_* of include/ac4.h)."""
+
+    def __init__(self, slot_caps, precision=FP32, chunk=100, relax=300):
+        load()
+        caps = _i32(slot_caps)
+        h = _VP()
+        _check(lib.ac4_server_create(len(caps), _ip(caps), precision, chunk, relax, ctypes.byref(h)))
+        self.h = h
+        self.n_of = {}
+
+    def close(self):
+        if self.h:
+            lib.ac4_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, n, T, f0, tau_p, steps, seed, t_arrive=0.0):
+        j = Job(n, T, f0, tau_p, steps, seed, t_arrive)
+        rc = lib.ac4_server_submit(self.h, ctypes.byref(j))
+        if rc < 0:
+            _check(rc)
+        self.n_of[rc] = n
+        return rc
+
+    def tick(self, now):
+        rc = lib.ac4_server_tick(self.h, now)
+        if rc < 0:
+            _check(rc)
+        return rc
+
+    def pending(self):
+        return lib.ac4_server_pending(self.h)
+
+    def job(self, jid):
+        ts, td = ctypes.c_double(), ctypes.c_double()
+        sd = ctypes.c_long()
+        fr, st = ctypes.c_int(), ctypes.c_int()
+        _check(lib.ac4_server_job(self.h, jid, ctypes.byref(ts), ctypes.byref(td), ctypes.byref(sd),
+                                  ctypes.byref(fr), ctypes.byref(st)))
+        return dict(t_start=ts.value, t_done=td.value, steps_done=sd.value, frames=fr.value, state=st.value)
+
+    def result(self, jid):
+        n = self.n_of[jid]
+        a = [np.empty(n) for _ in range(3)]
+        _check(lib.ac4_server_result(self.h, jid, *[_dp(v) for v in a]))
+        return a
+
+    def particle_steps(self):
+        return lib.ac4_server_particle_steps(self.h)
+
+
+def set_tuning(tpp_max=16, nlist_mode=0):
+    """Kernel shape for handles created afterwards (ac4_set_tuning)."""
+    load()
+    _check(lib.ac4_set_tuning(tpp_max, nlist_mode))
 
 
 def schedule(run_length, n_origins=200, factor=1.4):

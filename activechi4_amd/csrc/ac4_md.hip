@@ -43,9 +43,12 @@
 
 struct RepC {
     int n, off, nact, off_act;
-    int nc, ncell, off_cell, tau;      // list grid: nc = floor(L/2.85) cells per axis
-    int nblk, blk_off, nc_ref, pad1;   // nc_ref: the reference's grid (Parallel.c:72-80)
+    int nc, ncell, off_cell, tau;      // list grid: nc = floor(L/2.85) cells per axis (27-cell stencil)
+    int nblk, blk_off, nc_ref, tpp;    // nc_ref: the reference's grid (Parallel.c:72-80)
+    int fnblk, fblk_off, ltpp, cap_n;  // force-kernel block table (tpp = 1<<ltpp lanes per particle)
+    int cap_cells, pad2;
     double L, invL, inv_cell_ref, T, f0, dof, density;
+    float Lf, Lhf;
     uint32_t key0, key1;
 };
 
@@ -118,19 +121,29 @@ __device__ __forceinline__ bool need_rebuild(const RepC &R, const ChainS &c) {
 }
 
 // ---------------------------------------------------------------- kernels
-__global__ void k_shuffle(const RepC *__restrict__ reps, ChainS *__restrict__ chain, int8_t *kx,
-                          int8_t *ky, int8_t *kz, int force) {
+#define SHUF_LDS 16384
+__global__ void __launch_bounds__(64) k_shuffle(const RepC *__restrict__ reps, ChainS *__restrict__ chain,
+                                                int8_t *kx, int8_t *ky, int8_t *kz, int force) {
+    __shared__ int8_t s_k[3][SHUF_LDS];
     const int r = blockIdx.x;
     const RepC R = reps[r];
     ChainS &c = chain[r];
     if (!force && (c.mode != 0 || R.tau <= 0 || (c.step % R.tau) != 0)) return;
+    const uint64_t sh = (uint64_t)c.shuffles;
+    const bool lds = R.nact <= SHUF_LDS;
+    if (lds)
+        for (int i = threadIdx.x; i < R.nact; i += 64) {
+            s_k[0][i] = kx[R.off_act + i];
+            s_k[1][i] = ky[R.off_act + i];
+            s_k[2][i] = kz[R.off_act + i];
+        }
+    __syncthreads();
     const int comp = threadIdx.x;
     if (comp < 3 && R.nact > 0) {
-        int8_t *k = (comp == 0 ? kx : (comp == 1 ? ky : kz)) + R.off_act;
-        const uint64_t s = (uint64_t)c.shuffles;
+        int8_t *k = lds ? s_k[comp] : (comp == 0 ? kx : (comp == 1 ? ky : kz)) + R.off_act;
         // MDcode/active.c:58-63: for i: j = (int)(Nact*U[0,1)); swap(k[i], k[j])
         for (int i = 0; i < R.nact; i++) {
-            double u = ac4_shuffle_uniform(R.key0, R.key1, s, comp, i);
+            double u = ac4_shuffle_uniform(R.key0, R.key1, sh, comp, i);
             int j = (int)((double)R.nact * u);
             int8_t tmp = k[i];
             k[i] = k[j];
@@ -138,6 +151,12 @@ __global__ void k_shuffle(const RepC *__restrict__ reps, ChainS *__restrict__ ch
         }
     }
     __syncthreads();
+    if (lds)
+        for (int i = threadIdx.x; i < R.nact; i += 64) {
+            kx[R.off_act + i] = s_k[0][i];
+            ky[R.off_act + i] = s_k[1][i];
+            kz[R.off_act + i] = s_k[2][i];
+        }
     if (threadIdx.x == 0) c.shuffles += 1;
 }
 
@@ -148,7 +167,8 @@ __global__ void __launch_bounds__(BLK) k_first_half(const RepC *__restrict__ rep
                                                     const double *__restrict__ fy, const double *__restrict__ fz,
                                                     const double *__restrict__ xo, const double *__restrict__ yo,
                                                     const double *__restrict__ zo, const int *__restrict__ type,
-                                                    double4 *pos4, ChainS *chain_w) {
+                                                    double4 *pos4, ChainS *chain_w, const int *__restrict__ blk_idx,
+                                                    int *cnt) {
     __shared__ double s_scale[2];
     __shared__ double s_max[BLK / 64];
     const int b = blockIdx.x, r = blk_rep[b];
@@ -157,9 +177,11 @@ __global__ void __launch_bounds__(BLK) k_first_half(const RepC *__restrict__ rep
         const ChainS &c = chain[r];
         double eta[3] = {c.eta[0], c.eta[1], c.eta[2]}, peta[3] = {c.peta[0], c.peta[1], c.peta[2]}, S2;
         s_scale[0] = c.p_scale;
-        s_scale[1] = c.mode == 0 ? nhc_half(c.S_end, R.T, R.dof, c.q, eta, peta, &S2) : 1.0;
+        s_scale[1] = (c.mode == 0 && R.n > 0) ? nhc_half(c.S_end, R.T, R.dof, c.q, eta, peta, &S2) : 1.0;
     }
     __syncthreads();
+    // clear this replica's cell counts for a possible rebuild later in the step
+    for (int c = blk_idx[b] * BLK + threadIdx.x; c < R.ncell; c += R.nblk * BLK) cnt[R.off_cell + c] = 0;
     const double ps = s_scale[0], s1 = s_scale[1];
     const int i = blk_first[b] + threadIdx.x;
     double d2 = 0.0;
@@ -334,8 +356,8 @@ __global__ void __launch_bounds__(BLK) k_gather(const RepC *__restrict__ reps, c
 
 __global__ void __launch_bounds__(BLK) k_commit(const RepC *__restrict__ reps, const ChainS *__restrict__ chain,
                                                 const int *__restrict__ blk_rep, const int *__restrict__ blk_first,
-                                                PartArrays src, PartArrays dst, double4 *pos4, double *xo, double *yo,
-                                                double *zo) {
+                                                PartArrays src, PartArrays dst, double4 *pos4, float4 *pos4f,
+                                                double *xo, double *yo, double *zo) {
     const int b = blockIdx.x, r = blk_rep[b];
     const RepC R = reps[r];
     if (!need_rebuild(R, chain[r])) return;
@@ -350,79 +372,306 @@ __global__ void __launch_bounds__(BLK) k_commit(const RepC *__restrict__ reps, c
     dst.type[g] = t; dst.act[g] = src.act[g]; dst.id[g] = src.id[g];
     dst.cell_ref[g] = src.cell_ref[g]; dst.cell_of[g] = src.cell_of[g];
     pos4[g] = make_double4(sx, sy, sz, (double)t);
+    pos4f[g] = make_float4((float)(sx * R.L), (float)(sy * R.L), (float)(sz * R.L), (float)t);
     xo[g] = ux;  // update_cell_list, cell_list+forces.c:141-146
     yo[g] = uy;
     zo[g] = uz;
 }
 
-// Distinct neighbour-cell coordinates along one axis (fewer than 3 cells per
-// axis would otherwise alias the 27-cell stencil, cf. cell_list+forces.c:37-49).
+// Distinct neighbour-cell coordinates along one axis (cells are at least one
+// list radius wide); with fewer than 3 cells per axis every cell is a
+// neighbour once (an aliased stencil counts pairs twice, cf.
+// cell_list+forces.c:37-49 with two cells per axis).
 __device__ __forceinline__ int stencil_axis(int c, int nc, int *out) {
     if (nc >= 3) { out[0] = (c + nc - 1) % nc; out[1] = c; out[2] = (c + 1) % nc; return 3; }
-    if (nc == 2) { out[0] = 0; out[1] = 1; return 2; }
-    out[0] = 0;
-    return 1;
+    for (int d = 0; d < nc; d++) out[d] = d;
+    return nc;
 }
 
-// One wave per cell: the particles of each neighbour cell (contiguous slots
-// after the permutation) are staged in LDS 64 at a time and every lane tests
-// them against its home particle.  Lists are stored neighbour-major
-// (nl[k*stride + slot]) so the force kernel reads them coalesced.
-#define NL_WAVE 64
-__global__ void __launch_bounds__(NL_WAVE) k_nlist(const RepC *__restrict__ reps, ChainS *chain,
-                                                   const int *__restrict__ cblk_rep, const int *__restrict__ cblk_cell,
-                                                   const double4 *__restrict__ pos4, const int *__restrict__ cnt,
-                                                   const int *__restrict__ start, int *nn, int *nl, int stride,
-                                                   int cap) {
-    __shared__ double4 s_p[NL_WAVE];
-    const int r = cblk_rep[blockIdx.x], c = cblk_cell[blockIdx.x];
+// List radius per species pair: its force cutoff 2.5*sigma_ab plus the skin
+// 0.35 (MDcode/global.h:25-31 apply 2.85 to every pair); pairs are stored when
+// ((dx*dx+dy*dy)+dz*dz)*L*L < (2.5 sigma_ab + 0.35)^2.  The reference list
+// (< 2.85^2 for every pair, cell_list+forces.c:129-130) is counted beside it.
+__device__ __forceinline__ double list_cut2(int ab) {
+    const double rc = ab == 0 ? (2.5 * 1.00) + (.35 * 1.00) : (ab == 1 ? (2.5 * 0.80) + (.35 * 1.00) : (2.5 * 0.88) + (.35 * 1.00));
+    return rc * rc;
+}
+
+// Each replica owns cap*n list entries from off*cap.  Neighbour k of local
+// particle i (teams of tpp lanes) lives at ((k/tpp)*n + i)*tpp + k%tpp in that
+// region: a wave's lanes (64/tpp particles x tpp) read 64 consecutive entries
+// for every chunk of tpp neighbours.
+__device__ __forceinline__ size_t nl_at(const RepC &R, int cap, int k, int i) {
+    return (size_t)R.off * cap + (((size_t)(k >> R.ltpp) * R.n + i) << R.ltpp) + (k & (R.tpp - 1));
+}
+
+#define NL_UNROLL 8
+// Thread per particle (slots are in cell order, so the lanes of a wave scan
+// the same few cells and the position loads are shared in L1); candidate
+// positions are loaded NL_UNROLL at a time to keep several loads in flight.
+// Dynamics lists (EXACT=false) screen candidates in FP32 with a margin of
+// 1e-3 sigma^2, far above the FP32 error of r^2 (< 1e-4 for L < 100), so no
+// pair within its list radius is lost.  EXACT=true (update_cell_list) tests
+// every candidate with the reference's FP64 expression and also counts the
+// reference's 2.85 list.
+template <bool EXACT>
+__global__ void __launch_bounds__(BLK) k_nlist(const RepC *__restrict__ reps, ChainS *chain,
+                                               const int *__restrict__ blk_rep, const int *__restrict__ blk_first,
+                                               const double4 *__restrict__ pos4, const float4 *__restrict__ pos4f,
+                                               const int *__restrict__ cell_of, const int *__restrict__ cnt,
+                                               const int *__restrict__ start, int *nn, int *nn_ref, int *nl,
+                                               int stride, int cap) {
+    const int b = blockIdx.x, r = blk_rep[b];
     const RepC R = reps[r];
     if (!need_rebuild(R, chain[r])) return;
-    const int lane = threadIdx.x, nc = R.nc;
-    const int hs = start[R.off_cell + c], hn = cnt[R.off_cell + c];
+    const int i = blk_first[b] + threadIdx.x;
+    if (i >= R.n) return;
+    const int g = R.off + i, nc = R.nc;
+    const double4 pi = pos4[g];
+    const float4 pf = pos4f[g];
+    const int ti = (int)pi.w;
+    const int c = cell_of[g];
     const int cx = c % nc, cy = (c / nc) % nc, cz = c / (nc * nc);
     int ax[3], ay[3], az[3];
     const int nx = stencil_axis(cx, nc, ax), ny = stencil_axis(cy, nc, ay), nz = stencil_axis(cz, nc, az);
     const double L = R.L;
-    for (int h0 = 0; h0 < hn; h0 += NL_WAVE) {
-        const int hl = hs + h0 + lane;  // local slot of the home particle
-        const bool act = h0 + lane < hn;
-        const int g = R.off + hl;
-        double4 pi = make_double4(0, 0, 0, 0);
-        if (act) pi = pos4[g];
-        int count = 0;
-        for (int iz = 0; iz < nz; iz++)
-            for (int iy = 0; iy < ny; iy++)
-                for (int ix = 0; ix < nx; ix++) {
-                    const int c2 = R.off_cell + ax[ix] + ay[iy] * nc + az[iz] * nc * nc;
-                    const int s2 = start[c2], n2 = cnt[c2];
-                    for (int k0 = 0; k0 < n2; k0 += NL_WAVE) {
-                        __syncthreads();
-                        if (k0 + lane < n2) s_p[lane] = pos4[R.off + s2 + k0 + lane];
-                        __syncthreads();
-                        const int m = min(NL_WAVE, n2 - k0);
-                        if (act)
-                            for (int k = 0; k < m; k++) {
-                                const int jl = s2 + k0 + k;
-                                if (jl == hl) continue;
-                                const double4 pj = s_p[k];
-                                // cell_list+forces.c:120-130, same operation order
-                                double dx = minimg_md(pi.x - pj.x);
-                                double dy = minimg_md(pi.y - pj.y);
-                                double dz = minimg_md(pi.z - pj.z);
-                                double dr2 = (dx * dx + dy * dy + dz * dz) * L * L;
-                                if (dr2 < K_CELL_CUT_SQ) {
-                                    if (count < cap) nl[(size_t)count * stride + g] = R.off + jl;
-                                    count++;
-                                }
+    const double cut_a = list_cut2(ti), cut_b = list_cut2(ti + 1);
+    const float cfa = (float)cut_a + 1e-3f, cfb = (float)cut_b + 1e-3f, Lf = R.Lf, Lh = R.Lhf;
+    int count = 0, cref = 0;
+    for (int iz = 0; iz < nz; iz++)
+        for (int iy = 0; iy < ny; iy++)
+            for (int ix = 0; ix < nx; ix++) {
+                const int c2 = R.off_cell + ax[ix] + ay[iy] * nc + az[iz] * nc * nc;
+                const int s2 = R.off + start[c2], e2 = s2 + cnt[c2];
+                for (int k = s2; k < e2; k += NL_UNROLL) {
+                    if (EXACT) {
+                        double4 q[NL_UNROLL];
+#pragma unroll
+                        for (int u = 0; u < NL_UNROLL; u++) q[u] = pos4[min(k + u, e2 - 1)];
+#pragma unroll
+                        for (int u = 0; u < NL_UNROLL; u++) {
+                            const int j = k + u;
+                            if (j >= e2 || j == g) continue;
+                            // cell_list+forces.c:120-130, same operation order
+                            double dx = minimg_md(pi.x - q[u].x);
+                            double dy = minimg_md(pi.y - q[u].y);
+                            double dz = minimg_md(pi.z - q[u].z);
+                            double dr2 = (dx * dx + dy * dy + dz * dz) * L * L;
+                            cref += dr2 < K_CELL_CUT_SQ;
+                            if (dr2 < (q[u].w == 0.0 ? cut_a : cut_b)) {
+                                if (count < cap) nl[nl_at(R, cap, count, i)] = j;
+                                count++;
                             }
+                        }
+                    } else {
+                        float4 q[NL_UNROLL];
+#pragma unroll
+                        for (int u = 0; u < NL_UNROLL; u++) q[u] = pos4f[min(k + u, e2 - 1)];
+#pragma unroll
+                        for (int u = 0; u < NL_UNROLL; u++) {
+                            const int j = k + u;
+                            if (j >= e2 || j == g) continue;
+                            float dx = pf.x - q[u].x, dy = pf.y - q[u].y, dz = pf.z - q[u].z;
+                            dx = dx > Lh ? dx - Lf : (dx < -Lh ? dx + Lf : dx);
+                            dy = dy > Lh ? dy - Lf : (dy < -Lh ? dy + Lf : dy);
+                            dz = dz > Lh ? dz - Lf : (dz < -Lh ? dz + Lf : dz);
+                            const float r2 = dx * dx + dy * dy + dz * dz;
+                            if (r2 < (q[u].w == 0.0f ? cfa : cfb)) {
+                                if (count < cap) nl[nl_at(R, cap, count, i)] = j;
+                                count++;
+                            }
+                        }
                     }
                 }
-        if (act) {
-            if (count > cap) atomicOr(&chain[r].err, 1);
-            nn[g] = min(count, cap);
+            }
+    if (count > cap) atomicOr(&chain[r].err, 1);
+    nn[g] = min(count, cap);
+    if (EXACT) nn_ref[g] = cref;
+}
+
+// Dynamics lists, one wave per cell: the particles of the 27 neighbour cells
+// (contiguous slot ranges) are staged in LDS in stencil order; then for each
+// home particle the 64 lanes test 64 candidates at once and the hits are
+// appended in candidate order by ballot + popcount, so every list is in the
+// same canonical order as a serial scan.  FP32 screen with the margin
+// described at k_nlist.
+#define NLC_CAP 2304
+__global__ void __launch_bounds__(64) k_nlist_cell(const RepC *__restrict__ reps, ChainS *chain,
+                                                   const int *__restrict__ cblk_rep, const int *__restrict__ cblk_cell,
+                                                   const float4 *__restrict__ pos4f, const int *__restrict__ cnt,
+                                                   const int *__restrict__ start, int *nn, int *nl, int cap) {
+    __shared__ float4 s_c[NLC_CAP];
+    __shared__ int s_pre[28], s_st[27];
+    const int r = cblk_rep[blockIdx.x], c = cblk_cell[blockIdx.x];
+    const RepC R = reps[r];
+    if (c >= R.ncell || !need_rebuild(R, chain[r])) return;
+    const int lane = threadIdx.x, nc = R.nc;
+    const int cx = c % nc, cy = (c / nc) % nc, cz = c / (nc * nc);
+    int ax[3], ay[3], az[3];
+    const int nx = stencil_axis(cx, nc, ax), ny = stencil_axis(cy, nc, ay), nz = stencil_axis(cz, nc, az);
+    const int ns = nx * ny * nz;
+    // lane q < ns reads stencil cell q (stencil order: x fastest, then y, z)
+    int my_n = 0, my_s = 0, is_home = 0;
+    if (lane < ns) {
+        const int cc = ax[lane % nx] + ay[(lane / nx) % ny] * nc + az[lane / (nx * ny)] * nc * nc;
+        my_n = cnt[R.off_cell + cc];
+        my_s = start[R.off_cell + cc];
+        is_home = cc == c;
+    }
+    int pre = my_n;  // inclusive scan over the stencil cells
+    for (int o = 1; o < 32; o <<= 1) {
+        const int v = __shfl_up(pre, o, 64);
+        if (lane >= o) pre += v;
+    }
+    if (lane < ns) { s_pre[lane + 1] = pre; s_st[lane] = my_s; }
+    if (lane == 0) s_pre[0] = 0;
+    const unsigned long long hm = __ballot(is_home);
+    const int hl = __ffsll((long long)hm) - 1;  // stencil index of the home cell
+    const int total = __shfl(pre, ns - 1, 64);
+    const int hn = __shfl(my_n, hl, 64), hs = __shfl(my_s, hl, 64), hoff = __shfl(pre - my_n, hl, 64);
+    if (total > NLC_CAP) {
+        if (lane == 0) atomicOr(&chain[r].err, 1);
+        return;
+    }
+    __syncthreads();
+    for (int k = lane; k < total; k += 64) {  // independent loads
+        int q = 0;
+        while (s_pre[q + 1] <= k) q++;
+        s_c[k] = pos4f[R.off + s_st[q] + (k - s_pre[q])];
+    }
+    __syncthreads();
+    const float Lf = R.Lf, Lh = R.Lhf;
+    const float cut_0 = (float)list_cut2(0) + 1e-3f, cut_1 = (float)list_cut2(1) + 1e-3f,
+                cut_2 = (float)list_cut2(2) + 1e-3f;
+    const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    int mine = 0;  // list length of home particle `lane` (and lane+64, ... kept below)
+    for (int h = 0; h < hn; h++) {
+        const float4 pi = s_c[hoff + h];
+        const int ti = (int)pi.w, kh = hoff + h;
+        int base = 0;
+        for (int k0 = 0; k0 < total; k0 += 64) {
+            const int k = k0 + lane;
+            bool hit = false;
+            if (k < total && k != kh) {
+                const float4 q = s_c[k];
+                float dx = pi.x - q.x, dy = pi.y - q.y, dz = pi.z - q.z;
+                dx = dx > Lh ? dx - Lf : (dx < -Lh ? dx + Lf : dx);
+                dy = dy > Lh ? dy - Lf : (dy < -Lh ? dy + Lf : dy);
+                dz = dz > Lh ? dz - Lf : (dz < -Lh ? dz + Lf : dz);
+                const int ab = ti + (int)q.w;
+                hit = (dx * dx + dy * dy + dz * dz) < (ab == 0 ? cut_0 : (ab == 1 ? cut_1 : cut_2));
+            }
+            const unsigned long long mask = __ballot(hit);
+            if (hit) {
+                const int pos = base + __popcll(mask & below);
+                if (pos < cap) {
+                    int q = 0;
+                    while (s_pre[q + 1] <= k) q++;
+                    nl[nl_at(R, cap, pos, hs + h)] = R.off + s_st[q] + (k - s_pre[q]);
+                }
+            }
+            base += __popcll(mask);
+        }
+        if ((h & 63) == lane) mine = base;
+        if ((h & 63) == 63 || h == hn - 1) {  // flush 64 finished counts
+            const int h0 = h & ~63;
+            if (lane <= h - h0) {
+                if (mine > cap) atomicOr(&chain[r].err, 1);
+                nn[R.off + hs + h0 + lane] = min(mine, cap);
+            }
         }
     }
+}
+
+// ---- in-situ overlap Q(t) for every replica (device-resident accumulators)
+// Origins every origin_every steps are kept as snapshots of x_u in reference
+// order (ring of n_origin per replica); at a sample the overlap of every
+// resident origin whose lag (step - origin step) is a multiple of
+// sample_every below n_lags*sample_every is counted with the arithmetic of
+// Analysis/overlap.c:61-84 (unwrapped difference, one minimum-image
+// correction, L*L*((dx*dx+dy*dy)+dz*dz) < a*a).  Counts are integers, so the
+// per-lag sums do not depend on the order of the threads.
+struct OvCfg {
+    int sample_every, origin_every, n_origin, n_lags;
+    double a2;
+};
+
+__device__ __forceinline__ double minimg_ov(double d) {
+    if (d >= 0.5)
+        d -= 1.0;
+    else if (d < -0.5)
+        d += 1.0;
+    return d;
+}
+
+// grid (particle blocks of all replicas, n_origin)
+__global__ void __launch_bounds__(BLK) k_ov_count(const RepC *__restrict__ reps, const ChainS *__restrict__ chain,
+                                                  const int *__restrict__ blk_rep, const int *__restrict__ blk_first,
+                                                  OvCfg cfg, const int *__restrict__ id, const double *__restrict__ xu,
+                                                  const double *__restrict__ yu, const double *__restrict__ zu,
+                                                  double *ox, double *oy, double *oz, const long long *ostep,
+                                                  unsigned int *ocnt) {
+    const int b = blockIdx.x, r = blk_rep[b], o = blockIdx.y;
+    const RepC R = reps[r];
+    const long long step = chain[r].step;
+    const int i = blk_first[b] + threadIdx.x;
+    const size_t base = ((size_t)R.off * cfg.n_origin + (size_t)o * R.cap_n);  // snapshot o of replica r
+    const long long s0 = ostep[(size_t)r * cfg.n_origin + o];
+    int hit = 0;
+    if (i < R.n && s0 >= 0 && step > s0 && (step - s0) % cfg.sample_every == 0 &&
+        (step - s0) / cfg.sample_every <= cfg.n_lags) {
+        const int g = R.off + i, k = id[g];
+        double dx = xu[g] - ox[base + k], dy = yu[g] - oy[base + k], dz = zu[g] - oz[base + k];
+        dz = minimg_ov(dz);
+        dy = minimg_ov(dy);
+        dx = minimg_ov(dx);
+        hit = R.L * R.L * (dx * dx + dy * dy + dz * dz) < cfg.a2;
+    }
+    const int w = ac4_wave_sum_i(hit);
+    if ((threadIdx.x & 63) == 0 && w) atomicAdd(&ocnt[(size_t)r * cfg.n_origin + o], (unsigned int)w);
+}
+
+// one thread per (replica, origin): fold the count into the lag slot, then
+// store a new origin when the step is a multiple of origin_every
+__global__ void k_ov_fold(const RepC *__restrict__ reps, const ChainS *__restrict__ chain, OvCfg cfg, int nrep,
+                          long long *ostep, unsigned int *ocnt, double *sq, double *sq2, int *scnt) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nrep * cfg.n_origin) return;
+    const int r = t / cfg.n_origin;
+    const RepC R = reps[r];
+    const long long step = chain[r].step, s0 = ostep[t];
+    if (R.n > 0 && s0 >= 0 && step > s0 && (step - s0) % cfg.sample_every == 0) {
+        const long long lag = (step - s0) / cfg.sample_every;
+        if (lag <= cfg.n_lags) {
+            const double q = (double)ocnt[t] / (double)R.n;  // overlap.c:82
+            const size_t k = (size_t)r * (cfg.n_lags + 1) + lag;
+            sq[k] += q;        // overlap.c:89-94 (origin order: one origin per lag per step)
+            sq2[k] += q * q;
+            scnt[k] += 1;
+        }
+    }
+    ocnt[t] = 0u;
+}
+
+__global__ void __launch_bounds__(BLK) k_ov_origin(const RepC *__restrict__ reps, const ChainS *__restrict__ chain,
+                                                   const int *__restrict__ blk_rep, const int *__restrict__ blk_first,
+                                                   OvCfg cfg, const int *__restrict__ id, const double *__restrict__ xu,
+                                                   const double *__restrict__ yu, const double *__restrict__ zu,
+                                                   double *ox, double *oy, double *oz, long long *ostep) {
+    const int b = blockIdx.x, r = blk_rep[b];
+    const RepC R = reps[r];
+    const long long step = chain[r].step;
+    if (R.n == 0 || step % cfg.origin_every != 0) return;
+    const int o = (int)((step / cfg.origin_every) % cfg.n_origin);
+    const int i = blk_first[b] + threadIdx.x;
+    const size_t base = ((size_t)R.off * cfg.n_origin + (size_t)o * R.cap_n);
+    if (i < R.n) {
+        const int g = R.off + i, k = id[g];
+        ox[base + k] = xu[g];
+        oy[base + k] = yu[g];
+        oz[base + k] = zu[g];
+    }
+    if (i == 0) ostep[(size_t)r * cfg.n_origin + o] = step;
 }
 
 // Reference-order export of x_u and type for in-situ analysis.
@@ -506,30 +755,58 @@ struct PairAcc<true> {  // FP32 pair forces, FP64 minimum image, Kahan sums per 
     }
 };
 
+// Pair forces: a team of tpp lanes per particle (tpp is a function of the
+// replica's own N, so results do not depend on the batch) walks the particle's
+// list with stride tpp, 8 independent index/position loads in flight per lane;
+// the team's partial forces are combined by a fixed butterfly.
 template <bool FP32, bool KICK>
 __global__ void __launch_bounds__(BLK) k_force(const RepC *__restrict__ reps, ChainS *chain,
-                                               const int *__restrict__ blk_rep, const int *__restrict__ blk_first,
-                                               const int *__restrict__ blk_idx, const double4 *__restrict__ pos4,
-                                               const int *__restrict__ nn, const int *__restrict__ nl, int stride,
-                                               const int *__restrict__ act, const int8_t *__restrict__ kx,
-                                               const int8_t *__restrict__ ky, const int8_t *__restrict__ kz,
-                                               double *fx, double *fy, double *fz, double *px, double *py, double *pz,
-                                               double *partial) {
+                                               const int *__restrict__ fblk_rep, const int *__restrict__ fblk_first,
+                                               const double4 *__restrict__ pos4, const int *__restrict__ nn,
+                                               const int *__restrict__ nl, int cap, const int *__restrict__ act,
+                                               const int8_t *__restrict__ kx, const int8_t *__restrict__ ky,
+                                               const int8_t *__restrict__ kz, double *fx, double *fy, double *fz,
+                                               double *px, double *py, double *pz, double *partial) {
     __shared__ double s_red[4][BLK / 64];
     __shared__ int s_last;
-    const int b = blockIdx.x, r = blk_rep[b];
+    const int b = blockIdx.x, r = fblk_rep[b];
     const RepC R = reps[r];
-    const int i = blk_first[b] + threadIdx.x;
+    const int tpp = R.tpp, t = threadIdx.x & (tpp - 1);
+    const int i = fblk_first[b] + threadIdx.x / tpp;
     double v_p2 = 0.0, v_pe = 0.0, v_vir = 0.0, v_px = 0.0;
-    if (i < R.n) {
-        const int g = R.off + i;
+    double Fx = 0.0, Fy = 0.0, Fz = 0.0;
+    const bool valid = i < R.n;
+    const int g = R.off + (valid ? i : 0);
+    if (valid) {
         const double4 pi = pos4[g];
         const int ti = (int)pi.w;
         PairAcc<FP32> acc;
         const int m = nn[g];
-        for (int k = 0; k < m; k++) acc.add(pi.x, pi.y, pi.z, ti, pos4[nl[(size_t)k * stride + g]], R.L);
-        double Fx, Fy, Fz;
+        // entry t + c*tpp of particle i: chunk c of the replica's region
+        const int *lp = nl + nl_at(R, cap, t, i);
+        const size_t cstep = (size_t)R.n << R.ltpp;
+        int k = t;
+        for (; k + 7 * tpp < m; k += 8 * tpp, lp += 8 * cstep) {
+            int jj[8];
+            double4 q[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) jj[u] = lp[u * cstep];
+#pragma unroll
+            for (int u = 0; u < 8; u++) q[u] = pos4[jj[u]];
+#pragma unroll
+            for (int u = 0; u < 8; u++) acc.add(pi.x, pi.y, pi.z, ti, q[u], R.L);
+        }
+        for (; k < m; k += tpp, lp += cstep) acc.add(pi.x, pi.y, pi.z, ti, pos4[*lp], R.L);
         acc.result(Fx, Fy, Fz);
+        v_pe = acc.pe;
+        v_vir = acc.vir;
+    }
+    for (int o = 1; o < tpp; o <<= 1) {  // team butterfly (teams are aligned lane groups)
+        Fx += __shfl_xor(Fx, o, 64);
+        Fy += __shfl_xor(Fy, o, 64);
+        Fz += __shfl_xor(Fz, o, 64);
+    }
+    if (valid && t == 0) {
         const int a = act[g];
         if (a != -1) {  // cell_list+forces.c:207-213
             Fx += R.f0 * (double)kx[R.off_act + a];
@@ -537,8 +814,6 @@ __global__ void __launch_bounds__(BLK) k_force(const RepC *__restrict__ reps, Ch
             Fz += R.f0 * (double)kz[R.off_act + a];
         }
         fx[g] = Fx; fy[g] = Fy; fz[g] = Fz;
-        v_pe = acc.pe;
-        v_vir = acc.vir;
         if (KICK) {  // U2 after the forces, time_evolve.c:516
             double vx = px[g] + Fx * K_HALF_DT, vy = py[g] + Fy * K_HALF_DT, vz = pz[g] + Fz * K_HALF_DT;
             px[g] = vx; py[g] = vy; pz[g] = vz;
@@ -560,18 +835,27 @@ __global__ void __launch_bounds__(BLK) k_force(const RepC *__restrict__ reps, Ch
         }
         __threadfence();
         unsigned int prev = atomicAdd(&chain[r].done_blocks, 1u);
-        s_last = (prev == (unsigned int)(R.nblk - 1));
+        s_last = (prev == (unsigned int)(R.fnblk - 1));
     }
     __syncthreads();
-    if (!s_last || threadIdx.x != 0) return;
-    // last block of replica r: fixed-order reduction + chain end
+    if (!s_last || threadIdx.x >= 64) return;
+    // last block of replica r: the first wave sums the partials in a fixed
+    // order (lane t takes blocks t, t+64, ...; then a fixed butterfly)
     __threadfence();
-    ChainS &c = chain[r];
-    double S = 0.0, pe = 0.0, vir = 0.0, spx = 0.0;
-    for (int k = 0; k < R.nblk; k++) {
-        const double *pp = partial + 4 * (size_t)(R.blk_off + k);
-        S += pp[0]; pe += pp[1]; vir += pp[2]; spx += pp[3];
+    double a4[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int k = threadIdx.x; k < R.fnblk; k += 64) {  // volatile: written by other blocks
+        const volatile double *pp = partial + 4 * (size_t)(R.fblk_off + k);
+        a4[0] += pp[0]; a4[1] += pp[1]; a4[2] += pp[2]; a4[3] += pp[3];
     }
+    for (int q = 0; q < 4; q++)
+        for (int o = 1; o < 64; o <<= 1) a4[q] += __shfl_xor(a4[q], o, 64);
+    if (threadIdx.x != 0) return;
+    ChainS &c = chain[r];
+    if (R.n == 0) {  // idle slot
+        c.done_blocks = 0u;
+        return;
+    }
+    double S = a4[0], pe = a4[1], vir = a4[2], spx = a4[3];
     pe = 0.5 * pe;                  // every pair seen from both sides
     vir = (0.5 * vir) / (3.0e0);   // cell_list+forces.c:220
     c.pe = pe;
@@ -622,20 +906,33 @@ struct ac4_md {
     std::vector<RepC> reps_h;
     RepC *reps = nullptr;
     ChainS *chain = nullptr;
-    int *blk_rep = nullptr, *blk_first = nullptr, *blk_idx = nullptr, *cblk_rep = nullptr, *cblk_cell = nullptr;
+    int *blk_rep = nullptr, *blk_first = nullptr, *blk_idx = nullptr, *fblk_rep = nullptr, *fblk_first = nullptr;
+    int *cblk_rep = nullptr, *cblk_cell = nullptr;
+    int fnblk = 0;
     double *partial = nullptr;
     PartArrays P{}, A{};  // primary and permutation scratch
     double *fx = nullptr, *fy = nullptr, *fz = nullptr, *xo = nullptr, *yo = nullptr, *zo = nullptr;
     double4 *pos4 = nullptr;
-    int *rank = nullptr, *nn = nullptr, *nl = nullptr, *cnt = nullptr, *start = nullptr, *members = nullptr;
+    float4 *pos4f = nullptr;
+    int *rank = nullptr, *nn = nullptr, *nn_ref = nullptr, *nl = nullptr, *cnt = nullptr, *start = nullptr,
+        *members = nullptr;
     int8_t *kx = nullptr, *ky = nullptr, *kz = nullptr;
     double *ex = nullptr, *ey = nullptr, *ez = nullptr;  // reference-order export
+    // in-situ overlap accumulators
+    OvCfg ov{0, 0, 0, 0, 0.0};
+    double *ox = nullptr, *oy = nullptr, *oz = nullptr, *ov_q = nullptr, *ov_q2 = nullptr;
+    long long *ov_step = nullptr;
+    unsigned int *ov_cnt = nullptr;
+    int *ov_n = nullptr;
     int *et = nullptr;
     hipStream_t stream = nullptr;
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
     bool have_forces = false;
+    int nlist_mode = 0;
 };
+
+#define TRY(e) do { int rc_ = (e); if (rc_ != AC4_OK) return rc_; } while (0)
 
 template <class T>
 static int dalloc(T **p, size_t n) {
@@ -662,93 +959,186 @@ static void free_parts(PartArrays &a) {
     a = PartArrays{};
 }
 
-extern "C" int ac4_md_create(int n_rep, const ac4_replica_params *p, int precision, ac4_md **out) {
-    AC4_REQUIRE(n_rep >= 1 && p && out, AC4_EARG, "ac4_md_create: bad arguments");
-    AC4_REQUIRE(precision == AC4_FP64 || precision == AC4_FP32, AC4_EARG, "ac4_md_create: precision must be 0 or 1");
+// Tuning knobs read when a handle is created (the bench's autotuner sets
+// them): the largest team of lanes per particle in the force kernel, and the
+// list builder (0: one wave per cell with ballot compaction, 1: one thread
+// per particle).
+static int g_tpp_max = 16, g_nlist_mode = 0;
+
+extern "C" int ac4_set_tuning(int tpp_max, int nlist_mode) {
+    AC4_REQUIRE((tpp_max == 1 || tpp_max == 2 || tpp_max == 4 || tpp_max == 8 || tpp_max == 16) &&
+                    (nlist_mode == 0 || nlist_mode == 1),
+                AC4_EARG, "ac4_set_tuning: tpp_max in {1,2,4,8,16}, nlist_mode in {0,1}");
+    g_tpp_max = tpp_max;
+    g_nlist_mode = nlist_mode;
+    return AC4_OK;
+}
+
+// Job-dependent fields of a slot (the capacity-dependent ones are fixed at
+// creation).  Returns 0 or an error message.
+static const char *configure_rep(RepC &R, const ac4_replica_params &q) {
+    if (q.n < 2 || !(q.density > 0) || !(q.T > 0)) return "bad n/density/T";
+    if (q.n > R.cap_n) return "n exceeds the slot capacity";
+    const int nact = q.n_act < 0 ? (int)((double)q.n * 0.1000) : q.n_act;
+    if (nact > q.n) return "n_act larger than n";
+    const double L = pow((double)q.n / q.density, 0.33333333);  // MDcode/main.c:465
+    if (L <= 2.0 * K_CELL_CUT) return "box too small for the 2.85 list cutoff";
+    R.n = q.n;
+    R.nact = nact;
+    R.L = L;
+    R.invL = 1.0 / R.L;  // :466
+    // MDcode/Parallel.c:72-80 with one process (Lpp = L): the reference grid
+    const double Lpp = R.L / (double)1;
+    R.nc_ref = (int)((double)((int)Lpp) / K_CELL_CUT);
+    if (R.nc_ref < 1) R.nc_ref = 1;
+    R.inv_cell_ref = R.L / (Lpp / (double)R.nc_ref);
+    R.nc = (int)(R.L / K_CELL_CUT);  // list grid: cells at least 2.85 wide
+    R.ncell = R.nc * R.nc * R.nc;
+    if (R.ncell > R.cap_cells) return "density below the slot's cell capacity (0.5)";
+    R.Lf = (float)R.L;
+    R.Lhf = (float)(0.5 * R.L);
+    R.tau = q.tau_per;
+    R.T = q.T;
+    R.f0 = q.f0;
+    R.density = q.density;
+    R.dof = 3.0 * (double)q.n - 3.0;  // global.h:35
+    R.key0 = (uint32_t)q.seed;
+    R.key1 = (uint32_t)(q.seed >> 32);
+    return nullptr;
+}
+
+static void fresh_chain(ChainS &c, const RepC &R) {
+    memset(&c, 0, sizeof(ChainS));
+    const double tau = .20, T = R.T;  // time_evolve.c:477-482
+    for (int k = 0; k < 3; k++) c.q[k] = T * tau * tau;
+    c.q[0] = R.dof * T * tau * tau;
+    c.p_scale = 1.0;
+    c.force_rebuild = R.n > 0 ? 1 : 0;
+}
+
+extern "C" int ac4_md_create_slots(int nslots, const int *cap_n, int precision, ac4_md **out) {
+    AC4_REQUIRE(nslots >= 1 && cap_n && out, AC4_EARG, "ac4_md_create_slots: bad arguments");
+    AC4_REQUIRE(precision == AC4_FP64 || precision == AC4_FP32, AC4_EARG, "precision must be 0 or 1");
     int ndev = 0;
     AC4_REQUIRE(hipGetDeviceCount(&ndev) == hipSuccess && ndev > 0, AC4_EHIP,
                 "ac4_md_create: no HIP device visible (libac4 has no CPU path)");
     ac4_md *md = new ac4_md();
-    md->R = n_rep;
+    md->R = nslots;
     md->prec = precision;
-    int off = 0, off_act = 0, off_cell = 0, blk_off = 0;
-    std::vector<int> brep, bfirst, bidx, crep, ccell;
-    for (int r = 0; r < n_rep; r++) {
+    md->nlist_mode = g_nlist_mode;
+    int off = 0, off_act = 0, off_cell = 0, blk_off = 0, fblk_off = 0;
+    std::vector<int> brep, bfirst, bidx, frep, ffirst, crep, ccell;
+    for (int r = 0; r < nslots; r++) {
         RepC R;
         memset(&R, 0, sizeof R);
-        const ac4_replica_params &q = p[r];
-        if (q.n < 2 || !(q.density > 0) || !(q.T > 0)) { delete md; ac4_set_error("replica %d: bad n/density/T", r); return AC4_EARG; }
-        R.n = q.n;
+        const int cn = cap_n[r];
+        if (cn < 2) { delete md; ac4_set_error("slot %d: capacity %d < 2", r, cn); return AC4_EARG; }
+        R.cap_n = cn;
         R.off = off;
-        R.nact = q.n_act < 0 ? (int)((double)q.n * 0.1000) : q.n_act;
-        if (R.nact > q.n) { delete md; ac4_set_error("replica %d: n_act > n", r); return AC4_EARG; }
         R.off_act = off_act;
-        R.L = pow((double)q.n / q.density, 0.33333333);  // MDcode/main.c:465
-        R.invL = 1.0 / R.L;                              // :466
-        if (R.L <= 2.0 * K_CELL_CUT) { delete md; ac4_set_error("replica %d: box L=%g too small for the 2.85 list cutoff", r, R.L); return AC4_EARG; }
-        // MDcode/Parallel.c:72-80 with one process (Lpp = L): the reference grid
-        const double Lpp = R.L / (double)1;
-        R.nc_ref = (int)((double)((int)Lpp) / K_CELL_CUT);
-        if (R.nc_ref < 1) R.nc_ref = 1;
-        R.inv_cell_ref = R.L / (Lpp / (double)R.nc_ref);
-        // list grid: as many cells as fit with side >= 2.85
-        R.nc = (int)(R.L / K_CELL_CUT);
-        R.ncell = R.nc * R.nc * R.nc;
         R.off_cell = off_cell;
-        R.tau = q.tau_per;
-        R.T = q.T;
-        R.f0 = q.f0;
-        R.density = q.density;
-        R.dof = 3.0 * (double)q.n - 3.0;  // global.h:35
-        R.key0 = (uint32_t)q.seed;
-        R.key1 = (uint32_t)(q.seed >> 32);
-        R.nblk = (q.n + BLK - 1) / BLK;
+        // the densest box a slot can hold sets its cell capacity (rho >= 0.5)
+        const int cap_cells = (int)pow((int)(pow((double)cn / 0.5, 1.0 / 3.0) / K_CELL_CUT) + 1, 3);
+        R.cap_cells = cap_cells;
+        // lanes per particle in the force kernel: a function of the slot
+        // capacity only (power of two, 1..16), so small boxes fill the device
+        R.tpp = 1;
+        R.ltpp = 0;
+        while (R.tpp < g_tpp_max && (double)cn * R.tpp * 2 <= 32768.0) { R.tpp *= 2; R.ltpp++; }
+        R.nblk = (cn + BLK - 1) / BLK;
         R.blk_off = blk_off;
         for (int k = 0; k < R.nblk; k++) { brep.push_back(r); bfirst.push_back(k * BLK); bidx.push_back(k); }
-        for (int c = 0; c < R.ncell; c++) { crep.push_back(r); ccell.push_back(c); }
-        off += q.n;
-        off_act += R.nact;
-        off_cell += R.ncell;
+        const int ppb = BLK / R.tpp;  // particles per force block
+        R.fnblk = (cn + ppb - 1) / ppb;
+        R.fblk_off = fblk_off;
+        for (int k = 0; k < R.fnblk; k++) { frep.push_back(r); ffirst.push_back(k * ppb); }
+        for (int c = 0; c < cap_cells; c++) { crep.push_back(r); ccell.push_back(c); }
+        off += cn;
+        off_act += cn;  // room for n_act up to n
+        off_cell += cap_cells;
         blk_off += R.nblk;
-        md->max_ncell = std::max(md->max_ncell, R.ncell);
-        md->reps_h.push_back(R);
+        fblk_off += R.fnblk;
+        md->max_ncell = std::max(md->max_ncell, cap_cells);
+        md->reps_h.push_back(R);  // idle: n = 0
     }
     md->ntot = off;
     md->nact_tot = off_act;
     md->ncell_tot = off_cell;
     md->nblk = blk_off;
     md->stride = off;
+    md->fnblk = fblk_off;
     int rc = AC4_OK;
 #define ALLOC(ptr, n) if ((rc = dalloc(&md->ptr, (size_t)(n))) != AC4_OK) { ac4_md_destroy(md); return rc; }
-    ALLOC(reps, n_rep) ALLOC(chain, n_rep) ALLOC(blk_rep, md->nblk) ALLOC(blk_first, md->nblk) ALLOC(blk_idx, md->nblk)
-    ALLOC(cblk_rep, off_cell) ALLOC(cblk_cell, off_cell) ALLOC(partial, 4 * (size_t)md->nblk)
+    ALLOC(reps, nslots) ALLOC(chain, nslots) ALLOC(blk_rep, md->nblk) ALLOC(blk_first, md->nblk) ALLOC(blk_idx, md->nblk)
+    ALLOC(fblk_rep, md->fnblk) ALLOC(fblk_first, md->fnblk) ALLOC(cblk_rep, off_cell) ALLOC(cblk_cell, off_cell) ALLOC(partial, 4 * (size_t)md->fnblk) ALLOC(pos4f, off)
     ALLOC(fx, off) ALLOC(fy, off) ALLOC(fz, off) ALLOC(xo, off) ALLOC(yo, off) ALLOC(zo, off) ALLOC(pos4, off)
-    ALLOC(rank, off) ALLOC(nn, off) ALLOC(nl, (size_t)md->cap * off) ALLOC(cnt, off_cell) ALLOC(start, off_cell)
+    ALLOC(rank, off) ALLOC(nn, off) ALLOC(nn_ref, off) ALLOC(nl, (size_t)md->cap * off) ALLOC(cnt, off_cell) ALLOC(start, off_cell)
     ALLOC(members, off) ALLOC(kx, off_act) ALLOC(ky, off_act) ALLOC(kz, off_act)
 #undef ALLOC
     if ((rc = alloc_parts(md->P, off)) != AC4_OK || (rc = alloc_parts(md->A, off)) != AC4_OK) { ac4_md_destroy(md); return rc; }
     if (hipStreamCreate(&md->stream) != hipSuccess) { ac4_md_destroy(md); ac4_set_error("hipStreamCreate failed"); return AC4_EHIP; }
-    AC4_HIP(hipMemcpy(md->reps, md->reps_h.data(), sizeof(RepC) * n_rep, hipMemcpyHostToDevice));
+    AC4_HIP(hipMemcpy(md->reps, md->reps_h.data(), sizeof(RepC) * nslots, hipMemcpyHostToDevice));
     AC4_HIP(hipMemcpy(md->blk_rep, brep.data(), sizeof(int) * md->nblk, hipMemcpyHostToDevice));
     AC4_HIP(hipMemcpy(md->blk_first, bfirst.data(), sizeof(int) * md->nblk, hipMemcpyHostToDevice));
     AC4_HIP(hipMemcpy(md->blk_idx, bidx.data(), sizeof(int) * md->nblk, hipMemcpyHostToDevice));
+    AC4_HIP(hipMemcpy(md->fblk_rep, frep.data(), sizeof(int) * md->fnblk, hipMemcpyHostToDevice));
+    AC4_HIP(hipMemcpy(md->fblk_first, ffirst.data(), sizeof(int) * md->fnblk, hipMemcpyHostToDevice));
     AC4_HIP(hipMemcpy(md->cblk_rep, crep.data(), sizeof(int) * off_cell, hipMemcpyHostToDevice));
     AC4_HIP(hipMemcpy(md->cblk_cell, ccell.data(), sizeof(int) * off_cell, hipMemcpyHostToDevice));
-    std::vector<ChainS> ch(n_rep);
-    for (int r = 0; r < n_rep; r++) {
-        memset(&ch[r], 0, sizeof(ChainS));
-        const double tau = .20, T = md->reps_h[r].T;  // time_evolve.c:477-482
-        for (int k = 0; k < 3; k++) ch[r].q[k] = T * tau * tau;
-        ch[r].q[0] = md->reps_h[r].dof * T * tau * tau;
-        ch[r].p_scale = 1.0;
-        ch[r].force_rebuild = 1;
-    }
-    AC4_HIP(hipMemcpy(md->chain, ch.data(), sizeof(ChainS) * n_rep, hipMemcpyHostToDevice));
+    std::vector<ChainS> ch(nslots);
+    for (int r = 0; r < nslots; r++) fresh_chain(ch[r], md->reps_h[r]);
+    AC4_HIP(hipMemcpy(md->chain, ch.data(), sizeof(ChainS) * nslots, hipMemcpyHostToDevice));
     std::vector<int> neg(off, -1), ids(off);
-    for (int r = 0; r < n_rep; r++)
-        for (int i = 0; i < md->reps_h[r].n; i++) ids[md->reps_h[r].off + i] = i;
+    for (int r = 0; r < nslots; r++)
+        for (int i = 0; i < md->reps_h[r].cap_n; i++) ids[md->reps_h[r].off + i] = i;
     AC4_HIP(hipMemcpy(md->P.act, neg.data(), sizeof(int) * off, hipMemcpyHostToDevice));
     AC4_HIP(hipMemcpy(md->P.id, ids.data(), sizeof(int) * off, hipMemcpyHostToDevice));
+    md->have_forces = true;  // nothing loaded yet; set_state asks for a force evaluation
+    *out = md;
+    return AC4_OK;
+}
+
+extern "C" int ac4_md_assign(ac4_md *md, int slot, const ac4_replica_params *p) {
+    AC4_REQUIRE(md && slot >= 0 && slot < md->R && p, AC4_EARG, "ac4_md_assign: bad arguments");
+    RepC R = md->reps_h[slot];
+    const char *why = configure_rep(R, *p);
+    AC4_REQUIRE(!why, AC4_EARG, "slot %d: %s", slot, why ? why : "");
+    md->reps_h[slot] = R;
+    ChainS c;
+    fresh_chain(c, R);
+    AC4_HIP(hipMemcpyAsync(md->reps + slot, &md->reps_h[slot], sizeof(RepC), hipMemcpyHostToDevice, md->stream));
+    AC4_HIP(hipMemcpyAsync(md->chain + slot, &c, sizeof(ChainS), hipMemcpyHostToDevice, md->stream));
+    AC4_HIP(hipStreamSynchronize(md->stream));
+    return AC4_OK;
+}
+
+extern "C" int ac4_md_release(ac4_md *md, int slot) {
+    AC4_REQUIRE(md && slot >= 0 && slot < md->R, AC4_EARG, "ac4_md_release: bad arguments");
+    RepC &R = md->reps_h[slot];
+    R.n = 0; R.nact = 0; R.ncell = 0; R.tau = 0; R.nc = 0;
+    ChainS c;
+    fresh_chain(c, R);
+    AC4_HIP(hipMemcpyAsync(md->reps + slot, &R, sizeof(RepC), hipMemcpyHostToDevice, md->stream));
+    AC4_HIP(hipMemcpyAsync(md->chain + slot, &c, sizeof(ChainS), hipMemcpyHostToDevice, md->stream));
+    AC4_HIP(hipStreamSynchronize(md->stream));
+    return AC4_OK;
+}
+
+extern "C" int ac4_md_slot_n(const ac4_md *md, int slot) {
+    if (!md || slot < 0 || slot >= md->R) return AC4_EARG;
+    return md->reps_h[slot].n;
+}
+
+extern "C" int ac4_md_create(int n_rep, const ac4_replica_params *p, int precision, ac4_md **out) {
+    AC4_REQUIRE(n_rep >= 1 && p && out, AC4_EARG, "ac4_md_create: bad arguments");
+    std::vector<int> caps(n_rep);
+    for (int r = 0; r < n_rep; r++) caps[r] = p[r].n;
+    ac4_md *md = nullptr;
+    TRY(ac4_md_create_slots(n_rep, caps.data(), precision, &md));
+    for (int r = 0; r < n_rep; r++) {
+        int rc = ac4_md_assign(md, r, &p[r]);
+        if (rc != AC4_OK) { ac4_md_destroy(md); return rc; }
+    }
     *out = md;
     return AC4_OK;
 }
@@ -758,8 +1148,11 @@ extern "C" int ac4_md_destroy(ac4_md *md) {
     if (md->stream) (void)hipStreamSynchronize(md->stream);
     if (md->gexec) (void)hipGraphExecDestroy(md->gexec);
     if (md->graph) (void)hipGraphDestroy(md->graph);
-    void *ptrs[] = {md->reps, md->chain, md->blk_rep, md->blk_first, md->blk_idx, md->cblk_rep, md->cblk_cell,
-                    md->partial, md->fx, md->fy, md->fz, md->xo, md->yo, md->zo, md->pos4, md->rank, md->nn, md->nl,
+    void *ptrs[] = {md->reps, md->chain, md->blk_rep, md->blk_first, md->blk_idx, md->fblk_rep, md->fblk_first,
+                    md->pos4f, md->cblk_rep, md->cblk_cell, md->ox, md->oy, md->oz, md->ov_q, md->ov_q2,
+                    md->ov_step, md->ov_cnt, md->ov_n,
+                    md->partial, md->fx, md->fy, md->fz, md->xo, md->yo, md->zo, md->pos4, md->rank, md->nn,
+                    md->nn_ref, md->nl,
                     md->cnt, md->start, md->members, md->kx, md->ky, md->kz, md->ex, md->ey, md->ez, md->et};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
@@ -774,6 +1167,7 @@ extern "C" int ac4_md_total_particles(const ac4_md *md) { return md ? md->ntot :
 
 static int check_rep(const ac4_md *md, int r) {
     AC4_REQUIRE(md && r >= 0 && r < md->R, AC4_EARG, "bad handle or replica index %d", r);
+    AC4_REQUIRE(md->reps_h[r].n > 0, AC4_ESTATE, "slot %d is idle (no job assigned)", r);
     return AC4_OK;
 }
 
@@ -787,8 +1181,6 @@ static int d2h(T *dst, const T *src, size_t n, hipStream_t s) {
     AC4_HIP(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToHost, s));
     return AC4_OK;
 }
-#define TRY(e) do { int rc_ = (e); if (rc_ != AC4_OK) return rc_; } while (0)
-
 static int get_chain(ac4_md *md, int r, ChainS *c) {
     TRY(d2h(c, md->chain + r, 1, md->stream));
     AC4_HIP(hipStreamSynchronize(md->stream));
@@ -829,10 +1221,32 @@ static int fetch_unpermuted(ac4_md *md, int r, const T *dev, T *out, const std::
     return AC4_OK;
 }
 
+static int set_state_impl(ac4_md *md, int r, const double *x, const double *y, const double *z, const double *xu,
+                          const double *yu, const double *zu, const double *px, const double *py, const double *pz,
+                          const int *type, const int *act_tag, const int *kx, const int *ky, const int *kz,
+                          bool slot_local);
+
 extern "C" int ac4_md_set_state(ac4_md *md, int r, const double *x, const double *y, const double *z,
                                 const double *xu, const double *yu, const double *zu, const double *px,
                                 const double *py, const double *pz, const int *type, const int *act_tag,
                                 const int *kx, const int *ky, const int *kz) {
+    return set_state_impl(md, r, x, y, z, xu, yu, zu, px, py, pz, type, act_tag, kx, ky, kz, false);
+}
+
+// Loads a job into a slot without touching the other slots: its forces start
+// at zero and its list is built inside its first step (used by the server, so
+// admitting a job never changes the running replicas' arithmetic).
+extern "C" int ac4_md_load_slot(ac4_md *md, int r, const double *x, const double *y, const double *z,
+                                const double *xu, const double *yu, const double *zu, const double *px,
+                                const double *py, const double *pz, const int *type, const int *act_tag,
+                                const int *kx, const int *ky, const int *kz) {
+    return set_state_impl(md, r, x, y, z, xu, yu, zu, px, py, pz, type, act_tag, kx, ky, kz, true);
+}
+
+static int set_state_impl(ac4_md *md, int r, const double *x, const double *y, const double *z, const double *xu,
+                          const double *yu, const double *zu, const double *px, const double *py, const double *pz,
+                          const int *type, const int *act_tag, const int *kx, const int *ky, const int *kz,
+                          bool slot_local) {
     TRY(check_rep(md, r));
     AC4_REQUIRE(x && y && z && xu && yu && zu && px && py && pz && type, AC4_EARG, "set_state: NULL array");
     const RepC &R = md->reps_h[r];
@@ -885,9 +1299,27 @@ extern "C" int ac4_md_set_state(ac4_md *md, int r, const double *x, const double
     c.maxbits = 0ull;
     c.step = 0;  // a loaded state starts a new run (MDcode/main.c:238-240: start = 0)
     c.mode = 0;
+    c.err = 0;
     TRY(put_chain(md, r, &c));
-    md->have_forces = false;
+    if (slot_local) {
+        AC4_HIP(hipMemsetAsync(md->fx + g, 0, sizeof(double) * n, s));
+        AC4_HIP(hipMemsetAsync(md->fy + g, 0, sizeof(double) * n, s));
+        AC4_HIP(hipMemsetAsync(md->fz + g, 0, sizeof(double) * n, s));
+        AC4_HIP(hipStreamSynchronize(s));
+    } else {
+        md->have_forces = false;
+    }
     return AC4_OK;
+}
+
+// Velocity-rescale ramp for one slot only, applied by its next nsteps steps.
+extern "C" int ac4_md_relax_slot(ac4_md *md, int r, int nsteps, double f_start, double f_end) {
+    TRY(check_rep(md, r));
+    AC4_REQUIRE(nsteps > 0 && f_start > 0 && f_end > 0, AC4_EARG, "relax_slot: bad arguments");
+    ChainS c;
+    TRY(get_chain(md, r, &c));
+    c.mode = 1; c.ramp_n = nsteps; c.ramp_k = 0; c.ramp0 = f_start; c.ramp1 = f_end;
+    return put_chain(md, r, &c);
 }
 
 extern "C" int ac4_md_get_state(ac4_md *md, int r, double *x, double *y, double *z, double *xu, double *yu,
@@ -959,10 +1391,12 @@ extern "C" int ac4_md_set_rebuild_criterion(ac4_md *md, int criterion) {
     return put_chains(md, c);
 }
 
-static int launch_rebuild(ac4_md *md) {
+// Cell counts are cleared by k_first_half inside a step; outside a step
+// (update_cell_list) with_zero clears them first.
+static int launch_rebuild(ac4_md *md, bool with_zero = false) {
     hipStream_t s = md->stream;
     const int cy = (md->max_ncell + BLK - 1) / BLK, sy = (md->max_ncell + BLK / 64 - 1) / (BLK / 64);
-    hipLaunchKernelGGL(k_cell_zero, dim3(md->R, cy), dim3(BLK), 0, s, md->reps, md->chain, md->cnt);
+    if (with_zero) hipLaunchKernelGGL(k_cell_zero, dim3(md->R, cy), dim3(BLK), 0, s, md->reps, md->chain, md->cnt);
     hipLaunchKernelGGL(k_bin, dim3(md->nblk), dim3(BLK), 0, s, md->reps, md->chain, md->blk_rep, md->blk_first,
                        md->P.x, md->P.y, md->P.z, md->cnt, md->P.cell_of, md->rank, md->P.cell_ref);
     hipLaunchKernelGGL(k_cell_scan, dim3(md->R), dim3(BLK), 0, s, md->reps, md->chain, md->cnt, md->start);
@@ -973,9 +1407,18 @@ static int launch_rebuild(ac4_md *md) {
     hipLaunchKernelGGL(k_gather, dim3(md->nblk), dim3(BLK), 0, s, md->reps, md->chain, md->blk_rep, md->blk_first,
                        md->members, md->P, md->A);
     hipLaunchKernelGGL(k_commit, dim3(md->nblk), dim3(BLK), 0, s, md->reps, md->chain, md->blk_rep, md->blk_first,
-                       md->A, md->P, md->pos4, md->xo, md->yo, md->zo);
-    hipLaunchKernelGGL(k_nlist, dim3(md->ncell_tot), dim3(NL_WAVE), 0, s, md->reps, md->chain, md->cblk_rep,
-                       md->cblk_cell, md->pos4, md->cnt, md->start, md->nn, md->nl, md->stride, md->cap);
+                       md->A, md->P, md->pos4, md->pos4f, md->xo, md->yo, md->zo);
+    if (with_zero)  // update_cell_list: the reference's FP64 test and its 2.85 count
+        hipLaunchKernelGGL(k_nlist<true>, dim3(md->nblk), dim3(BLK), 0, s, md->reps, md->chain, md->blk_rep,
+                           md->blk_first, md->pos4, md->pos4f, md->P.cell_of, md->cnt, md->start, md->nn, md->nn_ref,
+                           md->nl, md->stride, md->cap);
+    else if (md->nlist_mode == 0)
+        hipLaunchKernelGGL(k_nlist_cell, dim3(md->ncell_tot), dim3(64), 0, s, md->reps, md->chain, md->cblk_rep,
+                           md->cblk_cell, md->pos4f, md->cnt, md->start, md->nn, md->nl, md->cap);
+    else
+        hipLaunchKernelGGL(k_nlist<false>, dim3(md->nblk), dim3(BLK), 0, s, md->reps, md->chain, md->blk_rep,
+                           md->blk_first, md->pos4, md->pos4f, md->P.cell_of, md->cnt, md->start, md->nn, md->nn_ref,
+                           md->nl, md->stride, md->cap);
     AC4_HIP(hipGetLastError());
     return AC4_OK;
 }
@@ -984,14 +1427,13 @@ template <bool KICK>
 static int launch_force(ac4_md *md) {
     hipStream_t s = md->stream;
     if (md->prec == AC4_FP32)
-        hipLaunchKernelGGL((k_force<true, KICK>), dim3(md->nblk), dim3(BLK), 0, s, md->reps, md->chain, md->blk_rep,
-                           md->blk_first, md->blk_idx, md->pos4, md->nn, md->nl, md->stride, md->P.act, md->kx,
+        hipLaunchKernelGGL((k_force<true, KICK>), dim3(md->fnblk), dim3(BLK), 0, s, md->reps, md->chain,
+                           md->fblk_rep, md->fblk_first, md->pos4, md->nn, md->nl, md->cap, md->P.act, md->kx,
                            md->ky, md->kz, md->fx, md->fy, md->fz, md->P.px, md->P.py, md->P.pz, md->partial);
     else
-        hipLaunchKernelGGL((k_force<false, KICK>), dim3(md->nblk), dim3(BLK), 0, s, md->reps, md->chain,
-                           md->blk_rep, md->blk_first, md->blk_idx, md->pos4, md->nn, md->nl, md->stride, md->P.act,
-                           md->kx, md->ky, md->kz, md->fx, md->fy, md->fz, md->P.px, md->P.py, md->P.pz,
-                           md->partial);
+        hipLaunchKernelGGL((k_force<false, KICK>), dim3(md->fnblk), dim3(BLK), 0, s, md->reps, md->chain,
+                           md->fblk_rep, md->fblk_first, md->pos4, md->nn, md->nl, md->cap, md->P.act, md->kx,
+                           md->ky, md->kz, md->fx, md->fy, md->fz, md->P.px, md->P.py, md->P.pz, md->partial);
     AC4_HIP(hipGetLastError());
     return AC4_OK;
 }
@@ -1003,7 +1445,8 @@ static void launch_shuffle(ac4_md *md, int force) {
 static void launch_first_half(ac4_md *md) {
     hipLaunchKernelGGL(k_first_half, dim3(md->nblk), dim3(BLK), 0, md->stream, md->reps, md->chain, md->blk_rep,
                        md->blk_first, md->P.x, md->P.y, md->P.z, md->P.xu, md->P.yu, md->P.zu, md->P.px, md->P.py,
-                       md->P.pz, md->fx, md->fy, md->fz, md->xo, md->yo, md->zo, md->P.type, md->pos4, md->chain);
+                       md->P.pz, md->fx, md->fy, md->fz, md->xo, md->yo, md->zo, md->P.type, md->pos4, md->chain,
+                       md->blk_idx, md->cnt);
 }
 
 static int launch_step(ac4_md *md) {
@@ -1029,7 +1472,7 @@ static int force_rebuild_all(ac4_md *md) {
     TRY(get_chains(md, c));
     for (auto &e : c) e.force_rebuild = 1;
     TRY(put_chains(md, c));
-    TRY(launch_rebuild(md));
+    TRY(launch_rebuild(md, true));
     TRY(get_chains(md, c));
     for (auto &e : c) { e.force_rebuild = 0; e.maxbits = 0ull; }
     return put_chains(md, c);
@@ -1042,7 +1485,7 @@ extern "C" int ac4_md_update_cell_list(ac4_md *md, int r_out, int *cell_out, int
         std::vector<int> id;
         TRY(get_ids(md, r_out, id));
         if (cell_out) TRY(fetch_unpermuted(md, r_out, (const int *)md->P.cell_ref, cell_out, id));
-        if (full_count_out) TRY(fetch_unpermuted(md, r_out, (const int *)md->nn, full_count_out, id));
+        if (full_count_out) TRY(fetch_unpermuted(md, r_out, (const int *)md->nn_ref, full_count_out, id));
     }
     return check_err(md);
 }
@@ -1179,6 +1622,76 @@ extern "C" int ac4_philox4x32(const uint32_t *ctr4, const uint32_t *key2, uint32
     AC4_REQUIRE(ctr4 && key2 && out4, AC4_EARG, "NULL argument");
     ac4_philox(ctr4, key2, out4);
     return AC4_OK;
+}
+
+// Reference-order x_u of every active slot into the export buffers and one
+// copy to the host (xyz[3][ntot], slot offsets as ac4_md_slot_offset).
+extern "C" int ac4_md_export_all(ac4_md *md, double *xu, double *yu, double *zu) {
+    AC4_REQUIRE(md && xu && yu && zu, AC4_EARG, "export_all: bad arguments");
+    if (!md->ex) {
+        TRY(dalloc(&md->ex, md->ntot)); TRY(dalloc(&md->ey, md->ntot)); TRY(dalloc(&md->ez, md->ntot));
+        TRY(dalloc(&md->et, md->ntot));
+    }
+    hipLaunchKernelGGL(k_export, dim3(md->nblk), dim3(BLK), 0, md->stream, md->reps, md->blk_rep, md->blk_first,
+                       md->P.id, md->P.xu, md->P.yu, md->P.zu, md->P.type, md->ex, md->ey, md->ez, md->et, -1);
+    AC4_HIP(hipGetLastError());
+    TRY(d2h(xu, (const double *)md->ex, md->ntot, md->stream));
+    TRY(d2h(yu, (const double *)md->ey, md->ntot, md->stream));
+    TRY(d2h(zu, (const double *)md->ez, md->ntot, md->stream));
+    AC4_HIP(hipStreamSynchronize(md->stream));
+    return check_err(md);
+}
+
+extern "C" int ac4_md_overlap_enable(ac4_md *md, int sample_every, int origin_every, int n_origin, int n_lags,
+                                     double a) {
+    AC4_REQUIRE(md && sample_every > 0 && origin_every > 0 && origin_every % sample_every == 0 && n_origin > 0 &&
+                    n_lags > 0 && a > 0,
+                AC4_EARG, "overlap_enable: bad arguments (origin_every must be a multiple of sample_every)");
+    AC4_REQUIRE(!md->ox, AC4_ESTATE, "overlap accumulators already enabled");
+    md->ov = OvCfg{sample_every, origin_every, n_origin, n_lags, a * a};
+    const size_t snap = (size_t)md->ntot * n_origin, acc = (size_t)md->R * (n_lags + 1);
+    TRY(dalloc(&md->ox, snap)); TRY(dalloc(&md->oy, snap)); TRY(dalloc(&md->oz, snap));
+    TRY(dalloc(&md->ov_q, acc)); TRY(dalloc(&md->ov_q2, acc)); TRY(dalloc(&md->ov_n, acc));
+    TRY(dalloc(&md->ov_cnt, (size_t)md->R * n_origin));
+    TRY(dalloc(&md->ov_step, (size_t)md->R * n_origin));
+    std::vector<long long> none((size_t)md->R * n_origin, -1);
+    AC4_HIP(hipMemcpy(md->ov_step, none.data(), sizeof(long long) * none.size(), hipMemcpyHostToDevice));
+    return AC4_OK;
+}
+
+// Counts overlaps for the current step (every replica) and stores a new
+// origin when due.  Call when the step counters are multiples of sample_every.
+extern "C" int ac4_md_overlap_sample(ac4_md *md) {
+    AC4_REQUIRE(md && md->ox, AC4_ESTATE, "overlap accumulators not enabled");
+    hipStream_t s = md->stream;
+    hipLaunchKernelGGL(k_ov_count, dim3(md->nblk, md->ov.n_origin), dim3(BLK), 0, s, md->reps, md->chain,
+                       md->blk_rep, md->blk_first, md->ov, md->P.id, md->P.xu, md->P.yu, md->P.zu, md->ox, md->oy,
+                       md->oz, md->ov_step, md->ov_cnt);
+    const int nt = md->R * md->ov.n_origin;
+    hipLaunchKernelGGL(k_ov_fold, dim3((nt + 255) / 256), dim3(256), 0, s, md->reps, md->chain, md->ov, md->R,
+                       md->ov_step, md->ov_cnt, md->ov_q, md->ov_q2, md->ov_n);
+    hipLaunchKernelGGL(k_ov_origin, dim3(md->nblk), dim3(BLK), 0, s, md->reps, md->chain, md->blk_rep,
+                       md->blk_first, md->ov, md->P.id, md->P.xu, md->P.yu, md->P.zu, md->ox, md->oy, md->oz,
+                       md->ov_step);
+    AC4_HIP(hipGetLastError());
+    return AC4_OK;
+}
+
+// Per-lag sums of replica r: sq[l] = sum_o Q_o(l*sample_every), sq2 likewise
+// for Q^2, cnt[l] = number of origins (l = 0..n_lags).
+extern "C" int ac4_md_overlap_get(ac4_md *md, int r, double *sq, double *sq2, int *cnt) {
+    AC4_REQUIRE(md && md->ox && r >= 0 && r < md->R, AC4_EARG, "overlap_get: bad arguments");
+    const size_t nl = md->ov.n_lags + 1, off = (size_t)r * nl;
+    if (sq) TRY(d2h(sq, (const double *)md->ov_q + off, nl, md->stream));
+    if (sq2) TRY(d2h(sq2, (const double *)md->ov_q2 + off, nl, md->stream));
+    if (cnt) TRY(d2h(cnt, (const int *)md->ov_n + off, nl, md->stream));
+    AC4_HIP(hipStreamSynchronize(md->stream));
+    return AC4_OK;
+}
+
+extern "C" int ac4_md_slot_offset(const ac4_md *md, int slot) {
+    if (!md || slot < 0 || slot >= md->R) return AC4_EARG;
+    return md->reps_h[slot].off;
 }
 
 // Reference-order device copies of x_u and type of replica r for in-situ

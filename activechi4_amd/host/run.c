@@ -229,6 +229,17 @@ int main(int argc, char **argv) {
     printf("\n\n -----------------------------------------------------------\n\n");
     printf("STARTING MOLECULAR DYNAMICS WITH %d SPHERES \n\n", n);
 
+    if (!restart_eq && !restart_run) {
+        /* coldstart(0.1,T) + breakxal(0.1,T) (MDcode/initialise.c:166-201):
+         * momenta rescaled every step along 0.5T -> T, then 1.5T -> T, at
+         * 0.1 temperature units per time unit */
+        int ramp = (int)(0.5 * T / (0.1 * TIME_STEP));
+        if (ramp < 1) ramp = 1;
+        push(md, &S);
+        CK(ac4_md_relax(md, ramp, 0.5, 1.0));
+        CK(ac4_md_relax(md, ramp, 1.5, 1.0));
+        pull(md, &S);
+    }
     if (!restart_run) { /* equilibrate, MDcode/main.c:123-206 */
         fix_drift(&S);
         push(md, &S);

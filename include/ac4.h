@@ -131,6 +131,79 @@ int ac4_md_sync(ac4_md *md);
 /* Raw Philox-4x32-10 blocks (known-answer tests): out4 = philox(ctr4, key2). */
 int ac4_philox4x32(const uint32_t *ctr4, const uint32_t *key2, uint32_t *out4);
 
+/* Tuning knobs for handles created afterwards (set by an autotuner; no
+ * reference counterpart): tpp_max = largest team of lanes per particle in the
+ * force kernel (1,2,4,8,16), nlist_mode = 0 (wave per cell, ballot
+ * compaction) or 1 (thread per particle). */
+int ac4_set_tuning(int tpp_max, int nlist_mode);
+
+/* ------------------------------------------------------ slots (serving) --
+ * A handle can be created with idle slots of fixed capacity; jobs are
+ * assigned to slots between launches and released when done.  No reference
+ * counterpart (the reference runs one job per process). */
+int ac4_md_create_slots(int nslots, const int *cap_n, int precision, ac4_md **out);
+/* Configures slot `slot` for a job of p->n <= capacity particles (state
+ * follows with set_state/load_slot). */
+int ac4_md_assign(ac4_md *md, int slot, const ac4_replica_params *p);
+int ac4_md_release(ac4_md *md, int slot);
+/* Particles of the job in a slot (0 when idle). */
+int ac4_md_slot_n(const ac4_md *md, int slot);
+/* First global index of a slot in the export buffers of ac4_md_export_all. */
+int ac4_md_slot_offset(const ac4_md *md, int slot);
+/* set_state for one slot without touching the others: its forces start at
+ * zero and its list is built inside its first step. */
+int ac4_md_load_slot(ac4_md *md, int r, const double *x, const double *y,
+                     const double *z, const double *xu, const double *yu,
+                     const double *zu, const double *px, const double *py,
+                     const double *pz, const int *type, const int *act_tag,
+                     const int *kx, const int *ky, const int *kz);
+/* ac4_md_relax for one slot, applied during its next nsteps steps. */
+int ac4_md_relax_slot(ac4_md *md, int r, int nsteps, double f_start, double f_end);
+/* x_u of every slot in reference order (arrays of ac4_md_total_particles). */
+int ac4_md_export_all(ac4_md *md, double *xu, double *yu, double *zu);
+
+/* In-situ overlap Q(t) for every replica with device-resident accumulators
+ * (north-star running Q/chi4; the arithmetic of each count is
+ * Analysis/overlap.c:61-84).  Origins every origin_every steps (a ring of
+ * n_origin snapshots per replica), lags l*sample_every for l = 0..n_lags.
+ * Call ac4_md_overlap_sample whenever the step counter is a multiple of
+ * sample_every (e.g. after ac4_md_step_graph(md, sample_every)). */
+int ac4_md_overlap_enable(ac4_md *md, int sample_every, int origin_every, int n_origin,
+                          int n_lags, double a);
+int ac4_md_overlap_sample(ac4_md *md);
+/* sq[l] = sum over origins of Q, sq2[l] of Q^2, cnt[l] origins (n_lags+1). */
+int ac4_md_overlap_get(ac4_md *md, int r, double *sq, double *sq2, int *cnt);
+
+/* Request loop (continuous batching of MD jobs on one GPU). */
+typedef struct {
+    int n;           /* particles (80:20 KA at density 1.2) */
+    double T, f0;    /* temperature, active force */
+    double tau_p;    /* persistence time (time units; tau_per = tau_p/dt) */
+    long steps;      /* production steps */
+    uint64_t seed;   /* initial state + RNG key */
+    double t_arrive; /* arrival time (seconds, caller's clock) */
+} ac4_job;
+typedef struct ac4_server ac4_server;
+/* Slots of the given capacities (the caller sizes them from an HBM budget);
+ * chunk = steps between frames; relax_steps = velocity-rescale ramp before
+ * each job's production steps. */
+int ac4_server_create(int nslots, const int *slot_cap, int precision, int chunk,
+                      int relax_steps, ac4_server **out);
+int ac4_server_destroy(ac4_server *s);
+/* Queues a job; returns its id (>= 0) or an error. */
+int ac4_server_submit(ac4_server *s, const ac4_job *job);
+/* Admits arrived jobs (t_arrive <= now) into free slots, advances all slots
+ * one chunk, copies each running job's frame, retires finished jobs.
+ * Returns the number of jobs that ran in the chunk. */
+int ac4_server_tick(ac4_server *s, double now);
+int ac4_server_pending(const ac4_server *s);
+/* state: 0 queued, 1 running, 2 done. */
+int ac4_server_job(const ac4_server *s, int id, double *t_start, double *t_done,
+                   long *steps_done, int *frames, int *state);
+/* Final unwrapped positions (reference order) of a finished job. */
+int ac4_server_result(const ac4_server *s, int id, double *xu, double *yu, double *zu);
+long long ac4_server_particle_steps(const ac4_server *s);
+
 /* ------------------------------------------------------------ Analysis --
  * Streaming per-frame analysis over all time origins (Analysis/main.c:53-62).
  * The (origin, lag) walk of Analysis/msd_pos.c:44-107 / overlap.c:42-106 is

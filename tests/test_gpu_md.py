@@ -83,6 +83,19 @@ def test_forces_fp64_match_oracle(gpu, n, f0):
     assert pv[1] == pytest.approx(o[1], rel=1e-12)
 
 
+def test_forces_mixed_batch_match_oracle(gpu):
+    # replicas of very different N share one launch (different lanes per
+    # particle and list layouts); every one must still match the oracle
+    states = [lattice_state(n, jitter=0.06, seed=n + 3) for n in (1000, 5000, 1837, 12000)]
+    md = make_md(gpu, states, f0=1.0)
+    pv = md.calculate_forces()
+    for r, (s, L) in enumerate(states):
+        cnt, lst, _ = OL.half_list(s["x"], s["y"], s["z"], L)
+        fx, fy, fz, o = OL.forces_list(s, L, 1.0, cnt, lst)
+        assert _rel_force_err(md.forces(r), (fx, fy, fz)) < 1e-12, r
+        assert pv[r][0] == pytest.approx(o[0], rel=1e-12)
+
+
 def test_forces_fp32_error_bound(gpu):
     errs = []
     for seed in range(4):
@@ -92,6 +105,7 @@ def test_forces_fp32_error_bound(gpu):
         cnt, lst, _ = OL.half_list(s["x"], s["y"], s["z"], L)
         fx, fy, fz, o = OL.forces_list(s, L, 1.5, cnt, lst)
         errs.append(_rel_force_err(md.forces(0), (fx, fy, fz)))
+    print("FP32 force error (max over particles, per configuration):", errs)
     assert max(errs) <= 1e-5, errs
 
 
@@ -179,3 +193,65 @@ def test_stats_consistent_with_state(gpu):
     assert st["ke"] == pytest.approx(ke, rel=1e-12)
     assert st["drift"] == pytest.approx(g["px"].sum(), abs=1e-9)
     assert st["steps"] == 30
+
+
+def _run_server(ac4, caps, jobs, chunk=100):
+    srv = ac4.Server(caps, precision=ac4.FP32, chunk=chunk, relax=200)
+    ids = [srv.submit(*j) for j in jobs]
+    t = 0
+    while srv.pending() and t < 10000:
+        srv.tick(float(t))
+        t += 1
+    return srv, ids
+
+
+def test_server_job_alone_equals_in_mix(gpu):
+    caps = [2048, 2048, 4096, 8192]
+    # (n, T, f0, tau_p, steps, seed, t_arrive in ticks)
+    jobs = [(1500, 0.5, 1.0, 1.0, 600, 11, 0), (3000, 0.6, 0.5, 0.1, 900, 12, 0), (1800, 0.45, 2.0, 10.0, 500, 13, 1),
+            (7000, 0.5, 1.5, 1.0, 400, 14, 2), (1200, 0.8, 0.0, 1.0, 700, 15, 3), (2000, 0.5, 1.0, 0.5, 300, 16, 5)]
+    mix, ids = _run_server(gpu, caps, jobs)
+    st = [mix.job(i) for i in ids]
+    assert all(s["state"] == 2 for s in st)
+    assert st[5]["t_start"] > 0  # the third 2048-class job had to wait for a free slot
+    for k in (0, 2, 3):
+        alone, aid = _run_server(gpu, caps, [jobs[k][:6] + (0,)])
+        a, m = alone.result(aid[0]), mix.result(ids[k])
+        for u in range(3):
+            assert np.array_equal(a[u], m[u]), (k, u)
+        alone.close()
+    assert mix.particle_steps() > 0
+
+
+def test_insitu_overlap_accumulators(gpu):
+    specs = [(1000, 71, 1), (1600, 72, 2)]
+    md = make_relaxed_md(gpu, specs, tau=20, prec=1)
+    se, oe, no, nl = 10, 40, 6, 12
+    md.overlap_enable(se, oe, no, nl, 0.3)
+    frames = {}
+    md.overlap_sample()  # step 0: first origin
+    frames[0] = [md.get_state(r) for r in range(2)]
+    for k in range(1, 13):
+        md.step(se, graph=True)
+        md.overlap_sample()
+        frames[k * se] = [md.get_state(r) for r in range(2)]
+    md.sync()
+    for r, (n, _, _) in enumerate(specs):
+        L = (n / 1.2) ** 0.33333333
+        q, q2, c = md.overlap(r)
+        wq, wq2, wc = np.zeros(nl + 1), np.zeros(nl + 1), np.zeros(nl + 1, dtype=np.int32)
+        for step in sorted(frames):
+            for s0 in sorted(frames):
+                if s0 % oe or s0 >= step or step - s0 > nl * se or (step - s0) // oe >= no + (step - s0) // oe:
+                    continue
+                a, b = frames[step][r], frames[s0][r]
+                d = [a[k] - b[k] for k in ("xu", "yu", "zu")]
+                d = [np.where(v >= 0.5, v - 1.0, np.where(v < -0.5, v + 1.0, v)) for v in d]
+                r2 = L * L * ((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2])
+                qq = float((r2 < 0.3 * 0.3).sum()) / n
+                lag = (step - s0) // se
+                wq[lag] += qq
+                wq2[lag] += qq * qq
+                wc[lag] += 1
+        assert np.array_equal(c, wc)
+        assert np.array_equal(q, wq) and np.array_equal(q2, wq2)
