@@ -90,6 +90,9 @@ _SIGS = {
     "ac4_origin_gap": (ctypes.c_int, [_I, ctypes.c_int]),
     "ac4_chi4_ensemble": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _D, _I, _D]),
     "ac4_set_tuning": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
+    "ac4_chain_init": (None, [ctypes.c_int, ctypes.c_double, ctypes.c_uint64, _D]),
+    "ac4_autotune": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ReplicaParams), ctypes.c_int, ctypes.c_int,
+                                    ctypes.c_int, _I, _D, ctypes.c_int, _I]),
     "ac4_md_create_slots": (ctypes.c_int, [ctypes.c_int, _I, ctypes.c_int, ctypes.POINTER(_VP)]),
     "ac4_md_assign": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.POINTER(ReplicaParams)]),
     "ac4_md_release": (ctypes.c_int, [_VP, ctypes.c_int]),
@@ -246,12 +249,10 @@ class MD:
         _check(lib.ac4_md_relax(self.h, nsteps, f_start, f_end))
 
     def init_chain(self, r, seed=None):
-        """InitialiseNoseHoover: eta = 0, p_eta ~ N(0, T*Q) (MDcode/time_evolve.c:476-493).
-        The draw depends on `seed` only (default: the replica index)."""
+        """InitialiseNoseHoover: eta = 0, p_eta ~ N(0, T*Q) (MDcode/time_evolve.c:476-493)
+        through ac4_chain_init.  The draw depends on `seed` only (default: 7919*r)."""
         p = self.params[r]
-        q = np.array([(3.0 * p.n - 3.0) * p.T * 0.04, p.T * 0.04, p.T * 0.04])
-        rng = np.random.default_rng(7919 * r if seed is None else seed)
-        self.set_chain(r, np.concatenate([np.zeros(3), np.sqrt(p.T * q) * rng.normal(size=3), q]))
+        self.set_chain(r, chain_init(p.n, p.T, 7919 * r if seed is None else seed))
 
     def set_rebuild_criterion(self, crit):
         _check(lib.ac4_md_set_rebuild_criterion(self.h, crit))
@@ -441,10 +442,32 @@ class Server:
         return lib.ac4_server_particle_steps(self.h)
 
 
-def set_tuning(tpp_max=16, nlist_mode=0):
-    """Kernel shape for handles created afterwards (ac4_set_tuning)."""
+def chain_init(n, T, seed):
+    """Nose-Hoover chain {eta[3], p_eta[3], Q[3]} drawn from seed (ac4_chain_init)."""
     load()
-    _check(lib.ac4_set_tuning(tpp_max, nlist_mode))
+    c = np.empty(9)
+    lib.ac4_chain_init(int(n), float(T), int(seed) & (2**64 - 1), _dp(c))
+    return c
+
+
+def autotune(params, precision=FP32, relax=300, steps=200):
+    """Times every kernel shape on this replica set and keeps the fastest for
+    handles created afterwards (ac4_autotune).  Returns (tpp, nlist_mode) and
+    the table of rows (tpp, nlist_mode, particle-steps/s)."""
+    load()
+    arr = (ReplicaParams * len(params))(*params)
+    best = (ctypes.c_int * 2)()
+    table = np.zeros((16, 3))
+    nc = ctypes.c_int(0)
+    _check(lib.ac4_autotune(len(params), arr, precision, relax, steps, best, _dp(table), 16, ctypes.byref(nc)))
+    return (best[0], best[1]), table[: nc.value]
+
+
+def set_tuning(tpp=0, nlist_mode=2):
+    """Kernel shape for handles created afterwards (ac4_set_tuning): force
+    team width tpp (0 = automatic) and dynamics list builder nlist_mode."""
+    load()
+    _check(lib.ac4_set_tuning(tpp, nlist_mode))
 
 
 def schedule(run_length, n_origins=200, factor=1.4):

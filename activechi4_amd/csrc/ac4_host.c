@@ -376,3 +376,23 @@ int ac4_init_lattice(int n, double T, int n_act, uint64_t seed, double *x, doubl
     free(perm);
     return AC4_OK;
 }
+
+/* InitialiseNoseHoover (MDcode/time_evolve.c:476-493): eta = 0, chain masses
+ * Q0 = DOF*T*tau^2, Q1 = Q2 = T*tau^2 (tau = 0.2), p_eta_k ~ N(0, T*Q_k).  The
+ * reference draws p_eta from a time-seeded GSL stream; here the draw is a
+ * xorshift64 stream (Marsaglia 2003) from `seed` through Box-Muller, so a
+ * job's chain is reproducible from its seed. */
+void ac4_chain_init(int n, double T, uint64_t seed, double *c9) {
+    const double tq = .20;
+    for (int k = 0; k < 3; k++) { c9[k] = 0.0; c9[6 + k] = T * tq * tq; }
+    c9[6] = (3.0 * n - 3.0) * T * tq * tq;
+    uint64_t st = seed * 0x9E3779B97F4A7C15ull + 1;
+    for (int k = 0; k < 3; k++) {
+        double u[2];
+        for (int m = 0; m < 2; m++) {
+            st ^= st >> 12; st ^= st << 25; st ^= st >> 27;
+            u[m] = (double)((st * 2685821657736338717ull) >> 11) * (1.0 / 9007199254740992.0);
+        }
+        c9[3 + k] = sqrt(T * c9[6 + k]) * sqrt(-2.0 * log(1.0 - u[0])) * cos(2.0 * 3.14159265358979 * u[1]);
+    }
+}

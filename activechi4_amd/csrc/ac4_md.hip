@@ -59,7 +59,7 @@ struct ChainS {
     unsigned long long maxbits;
     long long step, shuffles;
     int rebuilds, force_rebuild, err, crit;
-    unsigned int done_blocks;
+    unsigned int done_blocks, bin_done;  // last-block counters of k_force and k_bin
     int mode;                 // 0: Nose-Hoover chain, 1: velocity-rescale ramp
     int ramp_n, ramp_k;
     double ramp0, ramp1;      // target T = T*(ramp0 + (ramp1-ramp0)*k/(n-1))
@@ -120,34 +120,67 @@ __device__ __forceinline__ bool need_rebuild(const RepC &R, const ChainS &c) {
     return (c.crit == 1 ? 3.0 : 4.0) * maxD > K_SKIN * K_SKIN;  // :508 (crit 1)
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup w runs on
+// XCD w % 8).  Kernels that gather neighbour positions launch 8*ceil(nb/8)
+// workgroups and map workgroup w to logical block (w % 8)*ceil(nb/8) + w/8, so
+// every XCD works on a contiguous range of blocks (a few whole replicas) and
+// finds their positions in its own L2.  Returns -1 for the padding blocks.
+#define N_XCD 8
+__device__ __forceinline__ int xcd_block(int nb) {
+    const int per = (nb + N_XCD - 1) / N_XCD;
+    const int lb = (int)(blockIdx.x % N_XCD) * per + (int)(blockIdx.x / N_XCD);
+    return lb < nb ? lb : -1;
+}
+static inline int xcd_grid(int nb) { return ((nb + N_XCD - 1) / N_XCD) * N_XCD; }
+
 // ---------------------------------------------------------------- kernels
-#define SHUF_LDS 16384
+// Swap shuffle of the three direction components (one lane each).  The swap
+// partners j_i = (int)(Nact*U_i) do not depend on the array, so for
+// Nact <= SHUF_LDS the 64 lanes draw them all first and the serial part is
+// only the LDS swaps.
+#define SHUF_LDS 4096
 __global__ void __launch_bounds__(64) k_shuffle(const RepC *__restrict__ reps, ChainS *__restrict__ chain,
                                                 int8_t *kx, int8_t *ky, int8_t *kz, int force) {
     __shared__ int8_t s_k[3][SHUF_LDS];
+    __shared__ unsigned short s_j[3][SHUF_LDS];
     const int r = blockIdx.x;
     const RepC R = reps[r];
     ChainS &c = chain[r];
     if (!force && (c.mode != 0 || R.tau <= 0 || (c.step % R.tau) != 0)) return;
     const uint64_t sh = (uint64_t)c.shuffles;
     const bool lds = R.nact <= SHUF_LDS;
-    if (lds)
+    if (lds) {
         for (int i = threadIdx.x; i < R.nact; i += 64) {
             s_k[0][i] = kx[R.off_act + i];
             s_k[1][i] = ky[R.off_act + i];
             s_k[2][i] = kz[R.off_act + i];
         }
+        for (int q = threadIdx.x; q < 3 * R.nact; q += 64) {
+            const int comp = q / R.nact, i = q - comp * R.nact;
+            s_j[comp][i] = (unsigned short)(int)((double)R.nact * ac4_shuffle_uniform(R.key0, R.key1, sh, comp, i));
+        }
+    }
     __syncthreads();
     const int comp = threadIdx.x;
     if (comp < 3 && R.nact > 0) {
-        int8_t *k = lds ? s_k[comp] : (comp == 0 ? kx : (comp == 1 ? ky : kz)) + R.off_act;
         // MDcode/active.c:58-63: for i: j = (int)(Nact*U[0,1)); swap(k[i], k[j])
-        for (int i = 0; i < R.nact; i++) {
-            double u = ac4_shuffle_uniform(R.key0, R.key1, sh, comp, i);
-            int j = (int)((double)R.nact * u);
-            int8_t tmp = k[i];
-            k[i] = k[j];
-            k[j] = tmp;
+        if (lds) {
+            int8_t *k = s_k[comp];
+            const unsigned short *jj = s_j[comp];
+            for (int i = 0; i < R.nact; i++) {
+                const int j = jj[i];
+                const int8_t a = k[i], b = k[j];
+                k[i] = b;
+                k[j] = a;
+            }
+        } else {
+            int8_t *k = (comp == 0 ? kx : (comp == 1 ? ky : kz)) + R.off_act;
+            for (int i = 0; i < R.nact; i++) {
+                const int j = (int)((double)R.nact * ac4_shuffle_uniform(R.key0, R.key1, sh, comp, i));
+                const int8_t a = k[i], b = k[j];
+                k[i] = b;
+                k[j] = a;
+            }
         }
     }
     __syncthreads();
@@ -241,47 +274,80 @@ __global__ void __launch_bounds__(BLK) k_cell_zero(const RepC *__restrict__ reps
 
 __device__ __forceinline__ int clampc(int c, int n) { return min(max(c, 0), n - 1); }
 
-__global__ void __launch_bounds__(BLK) k_bin(const RepC *__restrict__ reps, const ChainS *__restrict__ chain,
+// Binning: the reference cell id (bit-exact, for update_cell_list) and the
+// list-grid cell of every particle, with the slot within its cell taken by
+// one atomic per distinct cell in the wave (particles arrive in cell order,
+// so a wave holds a few runs of equal cells).  The last block of a replica to
+// finish turns the cell counts into start offsets (exclusive scan).
+__global__ void __launch_bounds__(BLK) k_bin(const RepC *__restrict__ reps, ChainS *chain,
                                              const int *__restrict__ blk_rep, const int *__restrict__ blk_first,
                                              const double *__restrict__ x, const double *__restrict__ y,
                                              const double *__restrict__ z, int *cnt, int *cell_of, int *rank,
-                                             int *cell_ref) {
+                                             int *cell_ref, int *start) {
+    __shared__ int s_last, s_w[BLK / 64];
     const int b = blockIdx.x, r = blk_rep[b];
     const RepC R = reps[r];
-    if (!need_rebuild(R, chain[r])) return;
-    const int i = blk_first[b] + threadIdx.x;
-    if (i >= R.n) return;
+    if (!need_rebuild(R, chain[r])) return;  // uniform over the replica's blocks
+    const int i = blk_first[b] + threadIdx.x, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const bool valid = i < R.n;
     const int g = R.off + i;
-    const double sx = x[g], sy = y[g], sz = z[g];
-    // reference cell id, cell_list+forces.c:73-74 (floor in double, then sum)
-    const int rx = clampc((int)floor(sx * R.inv_cell_ref), R.nc_ref), ry = clampc((int)floor(sy * R.inv_cell_ref), R.nc_ref),
-              rz = clampc((int)floor(sz * R.inv_cell_ref), R.nc_ref);
-    cell_ref[g] = rx + ry * R.nc_ref + rz * R.nc_ref * R.nc_ref;
-    const int cx = clampc((int)floor(sx * R.nc), R.nc), cy = clampc((int)floor(sy * R.nc), R.nc),
-              cz = clampc((int)floor(sz * R.nc), R.nc);
-    const int c = cx + cy * R.nc + cz * R.nc * R.nc;
-    cell_of[g] = c;
-    rank[g] = atomicAdd(&cnt[R.off_cell + c], 1);
-}
-
-__global__ void __launch_bounds__(BLK) k_cell_scan(const RepC *__restrict__ reps, const ChainS *__restrict__ chain,
-                                                   const int *__restrict__ cnt, int *start) {
-    const RepC R = reps[blockIdx.x];
-    if (!need_rebuild(R, chain[blockIdx.x])) return;
-    __shared__ int s_part[BLK];
-    const int per = (R.ncell + BLK - 1) / BLK;
-    const int c0 = threadIdx.x * per, c1 = min(c0 + per, R.ncell);
-    int s = 0;
-    for (int c = c0; c < c1; c++) s += cnt[R.off_cell + c];
-    s_part[threadIdx.x] = s;
+    int c = -1;
+    if (valid) {
+        const double sx = x[g], sy = y[g], sz = z[g];
+        // reference cell id, cell_list+forces.c:73-74 (floor in double, then sum)
+        const int rx = clampc((int)floor(sx * R.inv_cell_ref), R.nc_ref),
+                  ry = clampc((int)floor(sy * R.inv_cell_ref), R.nc_ref),
+                  rz = clampc((int)floor(sz * R.inv_cell_ref), R.nc_ref);
+        cell_ref[g] = rx + ry * R.nc_ref + rz * R.nc_ref * R.nc_ref;
+        const int cx = clampc((int)floor(sx * R.nc), R.nc), cy = clampc((int)floor(sy * R.nc), R.nc),
+                  cz = clampc((int)floor(sz * R.nc), R.nc);
+        c = cx + cy * R.nc + cz * R.nc * R.nc;
+        cell_of[g] = c;
+    }
+    const unsigned long long below = (1ull << lane) - 1ull;
+    unsigned long long todo = __ballot(valid);
+    int my_rank = 0;
+    while (todo) {
+        const int leader = __ffsll((long long)todo) - 1;
+        const int cl = __shfl(c, leader, 64);
+        const unsigned long long grp = __ballot(c == cl);
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&cnt[R.off_cell + cl], __popcll(grp));
+        base = __shfl(base, leader, 64);
+        if (c == cl) my_rank = base + __popcll(grp & below);
+        todo &= ~grp;
+    }
+    if (valid) rank[g] = my_rank;
     __syncthreads();
     if (threadIdx.x == 0) {
-        int acc = 0;
-        for (int t = 0; t < BLK; t++) { int v = s_part[t]; s_part[t] = acc; acc += v; }
+        __threadfence();
+        const unsigned int prev = atomicAdd(&chain[r].bin_done, 1u);
+        s_last = prev == (unsigned int)(R.nblk - 1);
     }
     __syncthreads();
-    int acc = s_part[threadIdx.x];
-    for (int c = c0; c < c1; c++) { start[R.off_cell + c] = acc; acc += cnt[R.off_cell + c]; }
+    if (!s_last) return;
+    __threadfence();
+    // exclusive scan of the replica's cell counts (counts written by atomics
+    // of other blocks: read through volatile)
+    const volatile int *vc = cnt + R.off_cell;
+    const int per = (R.ncell + BLK - 1) / BLK;
+    const int c0 = min(threadIdx.x * per, R.ncell), c1 = min(c0 + per, R.ncell);
+    int own = 0;
+    for (int q = c0; q < c1; q++) own += vc[q];
+    int v = own;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(v, o, 64);
+        if (lane >= o) v += t;
+    }
+    if (lane == 63) s_w[w] = v;
+    __syncthreads();
+    int acc = v - own;
+    for (int k = 0; k < w; k++) acc += s_w[k];
+    for (int q = c0; q < c1; q++) {
+        start[R.off_cell + q] = acc;
+        acc += vc[q];
+    }
+    if (threadIdx.x == 0) chain[r].bin_done = 0u;
 }
 
 __global__ void __launch_bounds__(BLK) k_scatter(const RepC *__restrict__ reps, const ChainS *__restrict__ chain,
@@ -487,6 +553,70 @@ __global__ void __launch_bounds__(BLK) k_nlist(const RepC *__restrict__ reps, Ch
     if (count > cap) atomicOr(&chain[r].err, 1);
     nn[g] = min(count, cap);
     if (EXACT) nn_ref[g] = cref;
+}
+
+// Dynamics lists, a team of NLT lanes per particle: the team tests NLT
+// consecutive candidates of a stencil cell at once and appends the hits in
+// candidate order (team ballot + popcount of the lanes below), so the list is
+// in the order of a serial scan (stencil cells z, y, x; slots within a cell)
+// while the chain of dependent loads per particle is NLT times shorter.  The
+// control flow is uniform within a team (one home particle per team), so
+// every lane of a team is active at each ballot.  FP32 screen with the margin
+// described at k_nlist.
+#define NLT 16
+__global__ void __launch_bounds__(BLK) k_nlist_team(const RepC *__restrict__ reps, ChainS *chain,
+                                                    const int *__restrict__ lblk_rep,
+                                                    const int *__restrict__ lblk_first,
+                                                    const float4 *__restrict__ pos4f, const int *__restrict__ cell_of,
+                                                    const int *__restrict__ cnt, const int *__restrict__ start,
+                                                    int *nn, int *nl, int cap) {
+    // plain block order: only the replicas due for a rebuild do work, so
+    // they are spread over every XCD
+    const int b = blockIdx.x, r = lblk_rep[b];
+    const RepC R = reps[r];
+    if (!need_rebuild(R, chain[r])) return;
+    const int i = lblk_first[b] + (int)threadIdx.x / NLT;
+    if (i >= R.n) return;  // whole team
+    const int lane = threadIdx.x & 63, t = lane & (NLT - 1);
+    const unsigned long long tmask = (NLT == 64 ? ~0ull : ((1ull << NLT) - 1ull)) << (lane & ~(NLT - 1));
+    const unsigned long long below = (1ull << lane) - 1ull;
+    const int g = R.off + i, nc = R.nc;
+    const float4 pf = pos4f[g];
+    const int ti = (int)pf.w;
+    const int c = cell_of[g];
+    const int cx = c % nc, cy = (c / nc) % nc, cz = c / (nc * nc);
+    int ax[3], ay[3], az[3];
+    const int nx = stencil_axis(cx, nc, ax), ny = stencil_axis(cy, nc, ay), nz = stencil_axis(cz, nc, az);
+    const float cfa = (float)list_cut2(ti) + 1e-3f, cfb = (float)list_cut2(ti + 1) + 1e-3f, Lf = R.Lf, Lh = R.Lhf;
+    int count = 0;
+    for (int iz = 0; iz < nz; iz++)
+        for (int iy = 0; iy < ny; iy++)
+            for (int ix = 0; ix < nx; ix++) {
+                const int c2 = R.off_cell + ax[ix] + ay[iy] * nc + az[iz] * nc * nc;
+                const int s2 = R.off + start[c2], e2 = s2 + cnt[c2];
+                for (int k0 = s2; k0 < e2; k0 += NLT) {
+                    const int k = k0 + t;
+                    bool hit = false;
+                    if (k < e2 && k != g) {
+                        const float4 q = pos4f[k];
+                        float dx = pf.x - q.x, dy = pf.y - q.y, dz = pf.z - q.z;
+                        dx = dx > Lh ? dx - Lf : (dx < -Lh ? dx + Lf : dx);
+                        dy = dy > Lh ? dy - Lf : (dy < -Lh ? dy + Lf : dy);
+                        dz = dz > Lh ? dz - Lf : (dz < -Lh ? dz + Lf : dz);
+                        hit = dx * dx + dy * dy + dz * dz < (q.w == 0.0f ? cfa : cfb);
+                    }
+                    const unsigned long long m = __ballot(hit) & tmask;
+                    if (hit) {
+                        const int pos = count + __popcll(m & below);
+                        if (pos < cap) nl[nl_at(R, cap, pos, i)] = k;
+                    }
+                    count += __popcll(m);
+                }
+            }
+    if (t == 0) {
+        if (count > cap) atomicOr(&chain[r].err, 1);
+        nn[g] = min(count, cap);
+    }
 }
 
 // Dynamics lists, one wave per cell: the particles of the 27 neighbour cells
@@ -755,8 +885,8 @@ struct PairAcc<true> {  // FP32 pair forces, FP64 minimum image, Kahan sums per 
     }
 };
 
-// Pair forces: a team of tpp lanes per particle (tpp is a function of the
-// replica's own N, so results do not depend on the batch) walks the particle's
+// Pair forces: a team of tpp lanes per particle (tpp is fixed per handle at
+// creation, so a replica's results do not depend on what else is loaded) walks the particle's
 // list with stride tpp, 8 independent index/position loads in flight per lane;
 // the team's partial forces are combined by a fixed butterfly.
 template <bool FP32, bool KICK>
@@ -766,10 +896,12 @@ __global__ void __launch_bounds__(BLK) k_force(const RepC *__restrict__ reps, Ch
                                                const int *__restrict__ nl, int cap, const int *__restrict__ act,
                                                const int8_t *__restrict__ kx, const int8_t *__restrict__ ky,
                                                const int8_t *__restrict__ kz, double *fx, double *fy, double *fz,
-                                               double *px, double *py, double *pz, double *partial) {
+                                               double *px, double *py, double *pz, double *partial, int nb) {
     __shared__ double s_red[4][BLK / 64];
     __shared__ int s_last;
-    const int b = blockIdx.x, r = fblk_rep[b];
+    const int b = xcd_block(nb);
+    if (b < 0) return;
+    const int r = fblk_rep[b];
     const RepC R = reps[r];
     const int tpp = R.tpp, t = threadIdx.x & (tpp - 1);
     const int i = fblk_first[b] + threadIdx.x / tpp;
@@ -908,7 +1040,8 @@ struct ac4_md {
     ChainS *chain = nullptr;
     int *blk_rep = nullptr, *blk_first = nullptr, *blk_idx = nullptr, *fblk_rep = nullptr, *fblk_first = nullptr;
     int *cblk_rep = nullptr, *cblk_cell = nullptr;
-    int fnblk = 0;
+    int fnblk = 0, lnblk = 0;
+    int *lblk_rep = nullptr, *lblk_first = nullptr;  // list-build block table (NLT lanes per particle)
     double *partial = nullptr;
     PartArrays P{}, A{};  // primary and permutation scratch
     double *fx = nullptr, *fy = nullptr, *fz = nullptr, *xo = nullptr, *yo = nullptr, *zo = nullptr;
@@ -925,7 +1058,8 @@ struct ac4_md {
     unsigned int *ov_cnt = nullptr;
     int *ov_n = nullptr;
     int *et = nullptr;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr, side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
     bool have_forces = false;
@@ -959,17 +1093,21 @@ static void free_parts(PartArrays &a) {
     a = PartArrays{};
 }
 
-// Tuning knobs read when a handle is created (the bench's autotuner sets
-// them): the largest team of lanes per particle in the force kernel, and the
-// list builder (0: one wave per cell with ballot compaction, 1: one thread
-// per particle).
-static int g_tpp_max = 16, g_nlist_mode = 0;
+// Kernel shape of handles created afterwards.  tpp: lanes per particle in
+// the force kernel, 1..16, or 0 = automatic (the widest team with at most
+// TPP_LANES lanes over the handle's total capacity, so small handles still
+// fill the device and large ones keep one thread per particle).  The team
+// width fixes the order of each particle's force sum, so results are bitwise
+// reproducible for a given handle shape.  nlist_mode: dynamics list builder,
+// 0 = one wave per cell, 1 = one thread per particle, 2 = NLT-lane teams.
+#define TPP_LANES 65536.0
+static int g_tpp = 0, g_nlist_mode = 2;
 
-extern "C" int ac4_set_tuning(int tpp_max, int nlist_mode) {
-    AC4_REQUIRE((tpp_max == 1 || tpp_max == 2 || tpp_max == 4 || tpp_max == 8 || tpp_max == 16) &&
-                    (nlist_mode == 0 || nlist_mode == 1),
-                AC4_EARG, "ac4_set_tuning: tpp_max in {1,2,4,8,16}, nlist_mode in {0,1}");
-    g_tpp_max = tpp_max;
+extern "C" int ac4_set_tuning(int tpp, int nlist_mode) {
+    AC4_REQUIRE((tpp == 0 || tpp == 1 || tpp == 2 || tpp == 4 || tpp == 8 || tpp == 16) &&
+                    (nlist_mode >= 0 && nlist_mode <= 2),
+                AC4_EARG, "ac4_set_tuning: tpp in {0,1,2,4,8,16}, nlist_mode in {0,1,2}");
+    g_tpp = tpp;
     g_nlist_mode = nlist_mode;
     return AC4_OK;
 }
@@ -1027,7 +1165,15 @@ extern "C" int ac4_md_create_slots(int nslots, const int *cap_n, int precision, 
     md->prec = precision;
     md->nlist_mode = g_nlist_mode;
     int off = 0, off_act = 0, off_cell = 0, blk_off = 0, fblk_off = 0;
-    std::vector<int> brep, bfirst, bidx, frep, ffirst, crep, ccell;
+    std::vector<int> brep, bfirst, bidx, frep, ffirst, crep, ccell, lrep, lfirst;
+    double cap_tot = 0.0;
+    for (int r = 0; r < nslots; r++) cap_tot += cap_n[r] > 0 ? cap_n[r] : 0;
+    int tpp = g_tpp, ltpp = 0;
+    if (tpp == 0) {
+        tpp = 1;
+        while (tpp < 16 && cap_tot * tpp * 2 <= TPP_LANES) tpp *= 2;
+    }
+    while ((1 << ltpp) < tpp) ltpp++;
     for (int r = 0; r < nslots; r++) {
         RepC R;
         memset(&R, 0, sizeof R);
@@ -1040,11 +1186,8 @@ extern "C" int ac4_md_create_slots(int nslots, const int *cap_n, int precision, 
         // the densest box a slot can hold sets its cell capacity (rho >= 0.5)
         const int cap_cells = (int)pow((int)(pow((double)cn / 0.5, 1.0 / 3.0) / K_CELL_CUT) + 1, 3);
         R.cap_cells = cap_cells;
-        // lanes per particle in the force kernel: a function of the slot
-        // capacity only (power of two, 1..16), so small boxes fill the device
-        R.tpp = 1;
-        R.ltpp = 0;
-        while (R.tpp < g_tpp_max && (double)cn * R.tpp * 2 <= 32768.0) { R.tpp *= 2; R.ltpp++; }
+        R.tpp = tpp;
+        R.ltpp = ltpp;
         R.nblk = (cn + BLK - 1) / BLK;
         R.blk_off = blk_off;
         for (int k = 0; k < R.nblk; k++) { brep.push_back(r); bfirst.push_back(k * BLK); bidx.push_back(k); }
@@ -1052,6 +1195,7 @@ extern "C" int ac4_md_create_slots(int nslots, const int *cap_n, int precision, 
         R.fnblk = (cn + ppb - 1) / ppb;
         R.fblk_off = fblk_off;
         for (int k = 0; k < R.fnblk; k++) { frep.push_back(r); ffirst.push_back(k * ppb); }
+        for (int k = 0; k < (cn + BLK / NLT - 1) / (BLK / NLT); k++) { lrep.push_back(r); lfirst.push_back(k * (BLK / NLT)); }
         for (int c = 0; c < cap_cells; c++) { crep.push_back(r); ccell.push_back(c); }
         off += cn;
         off_act += cn;  // room for n_act up to n
@@ -1067,22 +1211,31 @@ extern "C" int ac4_md_create_slots(int nslots, const int *cap_n, int precision, 
     md->nblk = blk_off;
     md->stride = off;
     md->fnblk = fblk_off;
+    md->lnblk = (int)lrep.size();
     int rc = AC4_OK;
 #define ALLOC(ptr, n) if ((rc = dalloc(&md->ptr, (size_t)(n))) != AC4_OK) { ac4_md_destroy(md); return rc; }
     ALLOC(reps, nslots) ALLOC(chain, nslots) ALLOC(blk_rep, md->nblk) ALLOC(blk_first, md->nblk) ALLOC(blk_idx, md->nblk)
-    ALLOC(fblk_rep, md->fnblk) ALLOC(fblk_first, md->fnblk) ALLOC(cblk_rep, off_cell) ALLOC(cblk_cell, off_cell) ALLOC(partial, 4 * (size_t)md->fnblk) ALLOC(pos4f, off)
+    ALLOC(fblk_rep, md->fnblk) ALLOC(fblk_first, md->fnblk) ALLOC(lblk_rep, md->lnblk) ALLOC(lblk_first, md->lnblk) ALLOC(cblk_rep, off_cell) ALLOC(cblk_cell, off_cell) ALLOC(partial, 4 * (size_t)md->fnblk) ALLOC(pos4f, off)
     ALLOC(fx, off) ALLOC(fy, off) ALLOC(fz, off) ALLOC(xo, off) ALLOC(yo, off) ALLOC(zo, off) ALLOC(pos4, off)
     ALLOC(rank, off) ALLOC(nn, off) ALLOC(nn_ref, off) ALLOC(nl, (size_t)md->cap * off) ALLOC(cnt, off_cell) ALLOC(start, off_cell)
     ALLOC(members, off) ALLOC(kx, off_act) ALLOC(ky, off_act) ALLOC(kz, off_act)
 #undef ALLOC
     if ((rc = alloc_parts(md->P, off)) != AC4_OK || (rc = alloc_parts(md->A, off)) != AC4_OK) { ac4_md_destroy(md); return rc; }
-    if (hipStreamCreate(&md->stream) != hipSuccess) { ac4_md_destroy(md); ac4_set_error("hipStreamCreate failed"); return AC4_EHIP; }
+    if (hipStreamCreate(&md->stream) != hipSuccess || hipStreamCreate(&md->side) != hipSuccess ||
+        hipEventCreateWithFlags(&md->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&md->ev_join, hipEventDisableTiming) != hipSuccess) {
+        ac4_md_destroy(md);
+        ac4_set_error("hipStreamCreate/hipEventCreate failed");
+        return AC4_EHIP;
+    }
     AC4_HIP(hipMemcpy(md->reps, md->reps_h.data(), sizeof(RepC) * nslots, hipMemcpyHostToDevice));
     AC4_HIP(hipMemcpy(md->blk_rep, brep.data(), sizeof(int) * md->nblk, hipMemcpyHostToDevice));
     AC4_HIP(hipMemcpy(md->blk_first, bfirst.data(), sizeof(int) * md->nblk, hipMemcpyHostToDevice));
     AC4_HIP(hipMemcpy(md->blk_idx, bidx.data(), sizeof(int) * md->nblk, hipMemcpyHostToDevice));
     AC4_HIP(hipMemcpy(md->fblk_rep, frep.data(), sizeof(int) * md->fnblk, hipMemcpyHostToDevice));
     AC4_HIP(hipMemcpy(md->fblk_first, ffirst.data(), sizeof(int) * md->fnblk, hipMemcpyHostToDevice));
+    AC4_HIP(hipMemcpy(md->lblk_rep, lrep.data(), sizeof(int) * md->lnblk, hipMemcpyHostToDevice));
+    AC4_HIP(hipMemcpy(md->lblk_first, lfirst.data(), sizeof(int) * md->lnblk, hipMemcpyHostToDevice));
     AC4_HIP(hipMemcpy(md->cblk_rep, crep.data(), sizeof(int) * off_cell, hipMemcpyHostToDevice));
     AC4_HIP(hipMemcpy(md->cblk_cell, ccell.data(), sizeof(int) * off_cell, hipMemcpyHostToDevice));
     std::vector<ChainS> ch(nslots);
@@ -1149,6 +1302,7 @@ extern "C" int ac4_md_destroy(ac4_md *md) {
     if (md->gexec) (void)hipGraphExecDestroy(md->gexec);
     if (md->graph) (void)hipGraphDestroy(md->graph);
     void *ptrs[] = {md->reps, md->chain, md->blk_rep, md->blk_first, md->blk_idx, md->fblk_rep, md->fblk_first,
+                    md->lblk_rep, md->lblk_first,
                     md->pos4f, md->cblk_rep, md->cblk_cell, md->ox, md->oy, md->oz, md->ov_q, md->ov_q2,
                     md->ov_step, md->ov_cnt, md->ov_n,
                     md->partial, md->fx, md->fy, md->fz, md->xo, md->yo, md->zo, md->pos4, md->rank, md->nn,
@@ -1159,6 +1313,9 @@ extern "C" int ac4_md_destroy(ac4_md *md) {
     free_parts(md->P);
     free_parts(md->A);
     if (md->stream) (void)hipStreamDestroy(md->stream);
+    if (md->side) (void)hipStreamDestroy(md->side);
+    if (md->ev_fork) (void)hipEventDestroy(md->ev_fork);
+    if (md->ev_join) (void)hipEventDestroy(md->ev_join);
     delete md;
     return AC4_OK;
 }
@@ -1398,8 +1555,7 @@ static int launch_rebuild(ac4_md *md, bool with_zero = false) {
     const int cy = (md->max_ncell + BLK - 1) / BLK, sy = (md->max_ncell + BLK / 64 - 1) / (BLK / 64);
     if (with_zero) hipLaunchKernelGGL(k_cell_zero, dim3(md->R, cy), dim3(BLK), 0, s, md->reps, md->chain, md->cnt);
     hipLaunchKernelGGL(k_bin, dim3(md->nblk), dim3(BLK), 0, s, md->reps, md->chain, md->blk_rep, md->blk_first,
-                       md->P.x, md->P.y, md->P.z, md->cnt, md->P.cell_of, md->rank, md->P.cell_ref);
-    hipLaunchKernelGGL(k_cell_scan, dim3(md->R), dim3(BLK), 0, s, md->reps, md->chain, md->cnt, md->start);
+                       md->P.x, md->P.y, md->P.z, md->cnt, md->P.cell_of, md->rank, md->P.cell_ref, md->start);
     hipLaunchKernelGGL(k_scatter, dim3(md->nblk), dim3(BLK), 0, s, md->reps, md->chain, md->blk_rep, md->blk_first,
                        md->P.cell_of, md->rank, md->start, md->members);
     hipLaunchKernelGGL(k_cell_sort, dim3(md->R, sy), dim3(BLK), 0, s, md->reps, md->chain, md->cnt, md->start,
@@ -1412,6 +1568,10 @@ static int launch_rebuild(ac4_md *md, bool with_zero = false) {
         hipLaunchKernelGGL(k_nlist<true>, dim3(md->nblk), dim3(BLK), 0, s, md->reps, md->chain, md->blk_rep,
                            md->blk_first, md->pos4, md->pos4f, md->P.cell_of, md->cnt, md->start, md->nn, md->nn_ref,
                            md->nl, md->stride, md->cap);
+    else if (md->nlist_mode == 2)
+        hipLaunchKernelGGL(k_nlist_team, dim3(md->lnblk), dim3(BLK), 0, s, md->reps, md->chain,
+                           md->lblk_rep, md->lblk_first, md->pos4f, md->P.cell_of, md->cnt, md->start, md->nn,
+                           md->nl, md->cap);
     else if (md->nlist_mode == 0)
         hipLaunchKernelGGL(k_nlist_cell, dim3(md->ncell_tot), dim3(64), 0, s, md->reps, md->chain, md->cblk_rep,
                            md->cblk_cell, md->pos4f, md->cnt, md->start, md->nn, md->nl, md->cap);
@@ -1427,20 +1587,21 @@ template <bool KICK>
 static int launch_force(ac4_md *md) {
     hipStream_t s = md->stream;
     if (md->prec == AC4_FP32)
-        hipLaunchKernelGGL((k_force<true, KICK>), dim3(md->fnblk), dim3(BLK), 0, s, md->reps, md->chain,
+        hipLaunchKernelGGL((k_force<true, KICK>), dim3(xcd_grid(md->fnblk)), dim3(BLK), 0, s, md->reps, md->chain,
                            md->fblk_rep, md->fblk_first, md->pos4, md->nn, md->nl, md->cap, md->P.act, md->kx,
-                           md->ky, md->kz, md->fx, md->fy, md->fz, md->P.px, md->P.py, md->P.pz, md->partial);
+                           md->ky, md->kz, md->fx, md->fy, md->fz, md->P.px, md->P.py, md->P.pz, md->partial,
+                           md->fnblk);
     else
-        hipLaunchKernelGGL((k_force<false, KICK>), dim3(md->fnblk), dim3(BLK), 0, s, md->reps, md->chain,
+        hipLaunchKernelGGL((k_force<false, KICK>), dim3(xcd_grid(md->fnblk)), dim3(BLK), 0, s, md->reps, md->chain,
                            md->fblk_rep, md->fblk_first, md->pos4, md->nn, md->nl, md->cap, md->P.act, md->kx,
-                           md->ky, md->kz, md->fx, md->fy, md->fz, md->P.px, md->P.py, md->P.pz, md->partial);
+                           md->ky, md->kz, md->fx, md->fy, md->fz, md->P.px, md->P.py, md->P.pz, md->partial,
+                           md->fnblk);
     AC4_HIP(hipGetLastError());
     return AC4_OK;
 }
 
-static void launch_shuffle(ac4_md *md, int force) {
-    hipLaunchKernelGGL(k_shuffle, dim3(md->R), dim3(64), 0, md->stream, md->reps, md->chain, md->kx, md->ky, md->kz,
-                       force);
+static void launch_shuffle(ac4_md *md, int force, hipStream_t s) {
+    hipLaunchKernelGGL(k_shuffle, dim3(md->R), dim3(64), 0, s, md->reps, md->chain, md->kx, md->ky, md->kz, force);
 }
 static void launch_first_half(ac4_md *md) {
     hipLaunchKernelGGL(k_first_half, dim3(md->nblk), dim3(BLK), 0, md->stream, md->reps, md->chain, md->blk_rep,
@@ -1449,11 +1610,22 @@ static void launch_first_half(ac4_md *md) {
                        md->blk_idx, md->cnt);
 }
 
-static int launch_step(ac4_md *md) {
-    launch_shuffle(md, 0);
+// One MD step.  With fork (graph capture), the shuffle, which only the force
+// kernel depends on, runs on a second branch beside the half kick, drift and
+// list rebuild.
+static int launch_step(ac4_md *md, bool fork = false) {
+    if (fork) {
+        AC4_HIP(hipEventRecord(md->ev_fork, md->stream));
+        AC4_HIP(hipStreamWaitEvent(md->side, md->ev_fork, 0));
+        launch_shuffle(md, 0, md->side);
+        AC4_HIP(hipEventRecord(md->ev_join, md->side));
+    } else {
+        launch_shuffle(md, 0, md->stream);
+    }
     launch_first_half(md);
     AC4_HIP(hipGetLastError());
     TRY(launch_rebuild(md));
+    if (fork) AC4_HIP(hipStreamWaitEvent(md->stream, md->ev_join, 0));
     return launch_force<true>(md);
 }
 
@@ -1517,7 +1689,7 @@ extern "C" int ac4_md_step_graph(ac4_md *md, int nsteps) {
     if (!md->have_forces) TRY(ac4_md_calculate_forces(md, nullptr));
     if (!md->gexec) {
         AC4_HIP(hipStreamBeginCapture(md->stream, hipStreamCaptureModeThreadLocal));
-        int rc = launch_step(md);
+        int rc = launch_step(md, true);
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(md->stream, &g);
         if (rc != AC4_OK) return rc;
@@ -1551,7 +1723,7 @@ extern "C" int ac4_md_kernel_times(ac4_md *md, int nsteps, double *ms4) {
     double acc[4] = {0, 0, 0, 0};
     for (int k = 0; k < nsteps; k++) {
         AC4_HIP(hipEventRecord(ev[0], s));
-        launch_shuffle(md, 0);
+        launch_shuffle(md, 0, s);
         AC4_HIP(hipEventRecord(ev[1], s));
         launch_first_half(md);
         AC4_HIP(hipEventRecord(ev[2], s));
@@ -1571,7 +1743,7 @@ extern "C" int ac4_md_kernel_times(ac4_md *md, int nsteps, double *ms4) {
 
 extern "C" int ac4_md_shuffle(ac4_md *md) {
     AC4_REQUIRE(md, AC4_EARG, "NULL handle");
-    launch_shuffle(md, 1);
+    launch_shuffle(md, 1, md->stream);
     AC4_HIP(hipGetLastError());
     AC4_HIP(hipStreamSynchronize(md->stream));
     return AC4_OK;

@@ -113,20 +113,10 @@ static int admit(ac4_server *s, int j, int slot, double now) {
     free(buf);
     free(ib);
     if (rc != AC4_OK) return rc;
-    /* p_eta for the Nose-Hoover phase (MDcode/time_evolve.c:476-493), drawn
-     * from the job seed; the chain starts after the relaxation ramp */
-    double c[9], tq = .20;
-    for (int k = 0; k < 3; k++) { c[k] = 0.0; c[6 + k] = q->T * tq * tq; }
-    c[6] = (3.0 * n - 3.0) * q->T * tq * tq;
-    uint64_t st = q->seed * 0x9E3779B97F4A7C15ull + 1;
-    for (int k = 0; k < 3; k++) {
-        double u[2];
-        for (int m = 0; m < 2; m++) {
-            st ^= st >> 12; st ^= st << 25; st ^= st >> 27;
-            u[m] = ((st * 2685821657736338717ull) >> 11) * (1.0 / 9007199254740992.0);
-        }
-        c[3 + k] = sqrt(q->T * c[6 + k]) * sqrt(-2.0 * log(1.0 - u[0])) * cos(2.0 * 3.14159265358979 * u[1]);
-    }
+    /* chain for the Nose-Hoover phase, drawn from the job seed; it starts
+     * after the relaxation ramp */
+    double c[9];
+    ac4_chain_init(n, q->T, q->seed, c);
     if ((rc = ac4_md_set_chain(s->md, slot, c)) != AC4_OK) return rc;
     if ((rc = ac4_md_relax_slot(s->md, slot, s->relax, 1.5, 1.0)) != AC4_OK) return rc;
     r->state = 1;
@@ -139,9 +129,10 @@ static int admit(ac4_server *s, int j, int slot, double now) {
 int ac4_server_tick(ac4_server *s, double now) {
     if (!s) return AC4_EARG;
     /* admission: queued jobs in arrival order, each into a free slot of its
-     * class (the smallest slot capacity that holds it).  The class fixes the
-     * kernels' lanes per particle, so a job's arithmetic depends on its N and
-     * the server's capacities only, never on which other jobs are running. */
+     * class (the smallest slot capacity that holds it).  The kernel shape is
+     * fixed when the server is created and every kernel sums in a fixed
+     * order, so a job's arithmetic depends on its own parameters and the
+     * server's configuration only, never on which other jobs are running. */
     for (int j = 0; j < s->njobs; j++) {
         if (s->jobs[j].state != 0 || s->jobs[j].job.t_arrive > now) continue;
         int cls = 0;

@@ -1,0 +1,124 @@
+/*
+ * ac4_tune.c -- autotuner for the batched MD step.  No reference counterpart
+ * (the reference has one fixed CPU code path).  The replica set is brought to
+ * temperature once; every candidate kernel shape then starts from that same
+ * state, replays `steps` graph steps, and the fastest shape is kept for the
+ * handles created afterwards (ac4_set_tuning).  Candidates: force-kernel lanes
+ * per particle tpp in {1, 2, 4, 8, 16} (so 256/tpp particles per workgroup)
+ * crossed with the thread-per-particle and 16-lane-team list builders.
+ */
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+#include "ac4.h"
+#include "ac4_host.h"
+
+static double now_s(void) {
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+typedef struct {
+    int n, nact;
+    double *d;   /* x y z xu yu zu px py pz (9n) */
+    int *i;      /* type act_tag (2n), kx ky kz (3(nact+1)) */
+    double c9[9];
+} rep_state;
+
+static int load_all(ac4_md *md, int n_rep, rep_state *st) {
+    for (int r = 0; r < n_rep; r++) {
+        const int n = st[r].n, na = st[r].nact + 1;
+        double *d = st[r].d;
+        int *q = st[r].i;
+        int rc = ac4_md_set_state(md, r, d, d + n, d + 2 * n, d + 3 * n, d + 4 * n, d + 5 * n, d + 6 * n,
+                                  d + 7 * n, d + 8 * n, q, q + n, q + 2 * n, q + 2 * n + na, q + 2 * n + 2 * na);
+        if (rc == AC4_OK) rc = ac4_md_set_chain(md, r, st[r].c9);
+        if (rc != AC4_OK) return rc;
+    }
+    return AC4_OK;
+}
+
+static void free_states(rep_state *st, int n_rep) {
+    if (!st) return;
+    for (int r = 0; r < n_rep; r++) { free(st[r].d); free(st[r].i); }
+    free(st);
+}
+
+int ac4_autotune(int n_rep, const ac4_replica_params *p, int precision, int relax_steps, int steps, int *best2,
+                 double *table, int table_cap, int *n_cand) {
+    if (n_rep < 1 || !p || relax_steps < 1 || steps < 1) {
+        ac4_set_error("ac4_autotune: bad arguments");
+        return AC4_EARG;
+    }
+    static const int tpps[5] = {1, 2, 4, 8, 16}, modes[2] = {1, 2};
+    rep_state *st = calloc((size_t)n_rep, sizeof *st);
+    ac4_md *md = NULL;
+    int rc = ac4_set_tuning(0, 2);
+    if (rc == AC4_OK) rc = ac4_md_create(n_rep, p, precision, &md);
+    long long nsum = 0;
+    for (int r = 0; r < n_rep && rc == AC4_OK; r++) {
+        const int n = p[r].n, nact = p[r].n_act >= 0 ? p[r].n_act : (int)((double)n * 0.1000);
+        st[r].n = n;
+        st[r].nact = nact;
+        nsum += n;
+        st[r].d = malloc(sizeof(double) * 9 * (size_t)n);
+        st[r].i = malloc(sizeof(int) * (2 * (size_t)n + 3 * (size_t)(nact + 1)));
+        double *d = st[r].d;
+        int *q = st[r].i, na = nact + 1;
+        rc = ac4_init_lattice(n, p[r].T, nact, p[r].seed, d, d + n, d + 2 * n, d + 3 * n, d + 4 * n, d + 5 * n,
+                              d + 6 * n, d + 7 * n, d + 8 * n, q, q + n, q + 2 * n, q + 2 * n + na,
+                              q + 2 * n + 2 * na);
+        ac4_chain_init(n, p[r].T, p[r].seed, st[r].c9);
+    }
+    if (rc == AC4_OK) rc = load_all(md, n_rep, st);
+    if (rc == AC4_OK) rc = ac4_md_relax(md, relax_steps, 1.5, 1.0);
+    for (int r = 0; r < n_rep && rc == AC4_OK; r++) {
+        const int n = st[r].n, na = st[r].nact + 1;
+        double *d = st[r].d;
+        int *q = st[r].i;
+        rc = ac4_md_get_state(md, r, d, d + n, d + 2 * n, d + 3 * n, d + 4 * n, d + 5 * n, d + 6 * n, d + 7 * n,
+                              d + 8 * n, q + 2 * n, q + 2 * n + na, q + 2 * n + 2 * na);
+    }
+    ac4_md_destroy(md);
+    md = NULL;
+    if (rc != AC4_OK) {
+        free_states(st, n_rep);
+        ac4_set_tuning(0, 2);
+        return rc;
+    }
+    double best = -1.0;
+    int bt = 0, bm = 2, nc = 0;
+    for (int a = 0; a < 5; a++)
+        for (int b = 0; b < 2; b++) {
+            double rate = 0.0;
+            int ok = ac4_set_tuning(tpps[a], modes[b]) == AC4_OK && ac4_md_create(n_rep, p, precision, &md) == AC4_OK;
+            if (ok) ok = load_all(md, n_rep, st) == AC4_OK && ac4_md_step_graph(md, 20) == AC4_OK &&
+                         ac4_md_sync(md) == AC4_OK;
+            if (ok) {
+                const double t0 = now_s();
+                ok = ac4_md_step_graph(md, steps) == AC4_OK && ac4_md_sync(md) == AC4_OK;
+                const double t1 = now_s();
+                if (ok && t1 > t0) rate = (double)nsum * steps / (t1 - t0);
+            }
+            if (md) ac4_md_destroy(md);
+            md = NULL;
+            if (table && nc < table_cap) {
+                table[3 * nc] = tpps[a];
+                table[3 * nc + 1] = modes[b];
+                table[3 * nc + 2] = rate;
+            }
+            nc++;
+            if (rate > best) { best = rate; bt = tpps[a]; bm = modes[b]; }
+        }
+    free_states(st, n_rep);
+    if (n_cand) *n_cand = nc;
+    if (best <= 0.0) {
+        ac4_set_tuning(0, 2);
+        ac4_set_error("ac4_autotune: no candidate ran");
+        return AC4_EHIP;
+    }
+    if (best2) { best2[0] = bt; best2[1] = bm; }
+    return ac4_set_tuning(bt, bm);
+}

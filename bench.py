@@ -5,20 +5,30 @@ on): 64 replicas of the 80:20 Kob-Andersen mixture at density 1.2 and T=0.5,
 N_r drawn log-uniformly in [1000, 5000] from a seed, active force grid
 f0 in {0, 0.5, 1, 1.5, 2} x tau_p in {0.1, 1, 10} (tau_per 20/200/2000 steps),
 pair forces in FP32 with FP64 positions/momenta/thermostat, all replicas packed
-into one launch sequence per step and replayed as a hipGraph.  A "step" is one
-MD step of every replica; value = sum_r N_r x steps / seconds.
+into one launch sequence per step and replayed as a hipGraph, kernel shape
+chosen by the autotuner (ac4_autotune), 5e4 steps with the overlap Q(t)
+(a = 0.3) of every replica accumulated on the device every 100 steps, chi4(t)
+from those accumulators.  A "step" is one MD step of every replica;
+value = sum_r N_r x steps / seconds of the timed region.
 
-Beside it: the plain single run (one replica, FP64 everywhere, eager launches)
-on the same kind of system, the per-kernel times from HIP events on the
-library's stream, the roofline of the dominant kernel and the oracle (the CPU
-restatement) timed on a bounded sample on one host core.
+Beside it:
+- the plain single run: the same 64 replicas run one after another, FP64
+  everywhere, eager launches (one launch sequence per step);
+- a serving stream: jobs with N logU[500, 20000], T U[0.45, 0.8], f0 U[0, 2],
+  tau_p logU[0.1, 10], steps U[1e4, 1e5] arriving as a seeded Poisson process,
+  frames every 100 steps, p50/p99 time to completion, against the same jobs
+  run one after another with the plain kernel;
+- per-kernel times (HIP events on the library's stream), the roofline of the
+  force kernel, and the oracle (the CPU restatement) on a bounded sample on
+  one host core.
 
-usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--replicas R]
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--replicas R] [--serve-jobs J]
 """
 import argparse
 import json
 import math
 import os
+import re
 import sys
 import time
 
@@ -28,11 +38,14 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# compulsory bytes per particle of the fused force kernel (k_force): own
-# position 24 + type 4 + momentum read 24 + force write 24 + momentum write 24
-FORCE_BYTES_PER_PARTICLE = 100.0
-# whole step (k_first_half + k_force), FP64 positions/momenta, once each
-STEP_BYTES_PER_PARTICLE = 100.0 + 172.0
+# compulsory HBM bytes per particle of the fused force kernel (k_force, FP32
+# pair math, half kick fused): own position 32 (double4 incl. type) + list
+# length 4 + active index 4 + momentum read 24 + force write 24 + momentum
+# write 24 = 112 B.  Neighbour positions are gathered from cache; the list
+# entries (4 B each) are the kernel's other stream and show in the measured
+# traffic.
+FORCE_BYTES_PER_PARTICLE = 112.0
+PMC_FILE = os.path.join(ROOT, "profiles", "k_force_batched64_pmc.txt")
 
 
 def ensemble_params(ac4, R, seed=20261202):
@@ -54,6 +67,138 @@ def load_states(ac4, md, ps):
                      s["type"], s["act_tag"], s["kx"], s["ky"], s["kz"])
 
 
+def start(ac4, ps, precision, relax):
+    """Lattice starts brought to temperature (coldstart/breakxal ramp), then
+    the Nose-Hoover chains drawn from each replica's seed."""
+    md = ac4.MD(ps, precision=precision)
+    load_states(ac4, md, ps)
+    md.relax(relax, 1.5, 1.0)
+    for r, p in enumerate(ps):
+        md.init_chain(r, seed=p.seed)
+    return md
+
+
+def log(msg):
+    print("bench: " + msg, file=sys.stderr, flush=True)
+
+
+def chi4_report(md, ps, sample_every):
+    """chi4(t) = N (<Q^2> - <Q>^2) from the device accumulators (lags >= 1;
+    Q(0) = 1): per replica over its time origins (Analysis/overlap.c:116-118
+    times N), averaged over replicas; and for every (f0, tau_p) point of the
+    grid pooled over its replicas and their origins (ensemble fluctuation)."""
+    acc = [md.overlap(r) for r in range(len(ps))]
+    lags = np.arange(1, len(acc[0][0])) * sample_every * 0.005
+    per = []
+    for r, (q, q2, c) in enumerate(acc):
+        mq, mq2 = q[1:] / c[1:], q2[1:] / c[1:]
+        per.append(ps[r].n * (mq2 - mq * mq))
+    mean_per = np.mean(per, axis=0)
+    groups = {}
+    for r, p in enumerate(ps):
+        groups.setdefault((p.f0, p.tau_per), []).append(r)
+    by_param = []
+    for (f0, tau), rs in sorted(groups.items()):
+        q = sum(acc[r][0][1:] for r in rs)
+        q2 = sum(acc[r][1][1:] for r in rs)
+        c = sum(acc[r][2][1:] for r in rs)
+        nbar = float(np.mean([ps[r].n for r in rs]))
+        chi = nbar * (q2 / c - (q / c) ** 2)
+        k = int(np.argmax(chi))
+        by_param.append({"f0": f0, "tau_p": tau * 0.005, "replicas": len(rs), "chi4_peak": float(chi[k]),
+                         "peak_time": float(lags[k]), "Q_at_peak": float(q[k] / c[k])})
+    qall = sum(a[0][1:] for a in acc) / sum(a[2][1:] for a in acc)
+    k = int(np.argmax(mean_per))
+    pick = [0, len(lags) // 4, len(lags) // 2, len(lags) - 1]
+    return {"a": 0.3, "lag_times": [float(lags[i]) for i in pick], "Q_mean_at_those_lags": [float(qall[i]) for i in pick],
+            "chi4_origins_peak": float(mean_per[k]), "chi4_origins_peak_time": float(lags[k]),
+            "chi4_ensemble_by_param": by_param, "max_lag_time": float(lags[-1])}
+
+
+def plain_sequential(ac4, ps, relax, steps):
+    """The same replicas one after another: FP64, eager launches."""
+    ac4.set_tuning(0, 2)
+    tot_ps, tot_t = 0.0, 0.0
+    for p in ps:
+        md = start(ac4, [p], ac4.FP64, relax)
+        md.step(50)
+        md.sync()
+        t0 = time.perf_counter()
+        md.step(steps)
+        md.sync()
+        tot_t += time.perf_counter() - t0
+        tot_ps += p.n * steps
+        md.close()
+    return tot_ps / tot_t, tot_t / steps * 1e3
+
+
+def plain_rate(ac4, n, T, f0, tau, seed, relax=150, steps=200):
+    ac4.set_tuning(0, 2)
+    md = start(ac4, [ac4.ReplicaParams(n, 1.2, T, f0, -1, tau, seed)], ac4.FP64, relax)
+    md.step(30)
+    md.sync()
+    t0 = time.perf_counter()
+    md.step(steps)
+    md.sync()
+    dt = time.perf_counter() - t0
+    md.close()
+    return n * steps / dt
+
+
+def serving(ac4, njobs, rate_per_s, seed=4242):
+    """Seeded Poisson stream of MD jobs through the continuous-batching server."""
+    rng = np.random.default_rng(seed)
+    jobs = []
+    t = 0.0
+    for k in range(njobs):
+        t += rng.exponential(1.0 / rate_per_s)
+        n = int(math.exp(rng.uniform(math.log(500), math.log(20000))))
+        jobs.append(dict(n=n, T=float(rng.uniform(0.45, 0.8)), f0=float(rng.uniform(0.0, 2.0)),
+                         tau_p=float(math.exp(rng.uniform(math.log(0.1), math.log(10.0)))),
+                         steps=int(rng.integers(10000, 100001)), seed=1000 + k, t=t))
+    # slot classes sized from an HBM budget: a class's slot count follows the
+    # share of jobs it takes under the N distribution (about 1.2 KB per
+    # particle of capacity incl. the 192-entry neighbour list)
+    caps = [1024] * 8 + [2048] * 10 + [4096] * 10 + [8192] * 8 + [20480] * 4
+    ac4.set_tuning(0, 2)
+    srv = ac4.Server(caps, precision=ac4.FP32, chunk=100, relax=300)
+    ids = [srv.submit(j["n"], j["T"], j["f0"], j["tau_p"], j["steps"], j["seed"], j["t"]) for j in jobs]
+    t0 = time.perf_counter()
+    while srv.pending():
+        if srv.tick(time.perf_counter() - t0) == 0:
+            time.sleep(0.001)  # nothing admitted yet: wait for the next arrival
+    wall = time.perf_counter() - t0
+    lat = []
+    for j, i in zip(jobs, ids):
+        st = srv.job(i)
+        assert st["state"] == 2
+        lat.append(st["t_done"] - j["t"])
+    srv.close()
+    work = sum(j["n"] * j["steps"] for j in jobs)
+    serve_rate = work / (wall - jobs[0]["t"])
+    # the same jobs run one after another with the plain kernel (rate measured
+    # per job on a short run, extrapolated to the job's steps)
+    t_plain = [j["n"] * j["steps"] / plain_rate(ac4, j["n"], j["T"], j["f0"],
+                                                 max(1, int(round(j["tau_p"] / 0.005))), j["seed"])
+               for j in jobs]
+    t_seq = sum(t_plain)
+    lat = np.array(lat)
+    # the same arrivals served first-come first-served, one job at a time,
+    # with the plain kernel
+    free, seq_lat = 0.0, []
+    for j, tj in zip(jobs, t_plain):
+        free = max(free, j["t"]) + tj
+        seq_lat.append(free - j["t"])
+    return {"jobs": njobs, "arrival_rate_per_s": rate_per_s, "slots": len(caps),
+            "slot_capacity_particles": int(sum(caps)), "p50_latency_s": float(np.percentile(lat, 50)),
+            "p99_latency_s": float(np.percentile(lat, 99)), "value": serve_rate, "unit": "particle-steps/s",
+            "plain_sequential_value": work / t_seq, "plain_sequential_makespan_s": t_seq,
+            "plain_sequential_p50_latency_s": float(np.percentile(seq_lat, 50)),
+            "plain_sequential_p99_latency_s": float(np.percentile(seq_lat, 99)),
+            "speedup_vs_plain_sequential": serve_rate * t_seq / work, "wall_s": wall,
+            "production_particle_steps": work}
+
+
 def cpu_baseline(ac4, n=2485, steps=1500):
     """Oracle (single-threaded C restatement of the reference step) on one core,
     from a state brought to temperature on the GPU (the oracle has no ramp)."""
@@ -61,10 +206,8 @@ def cpu_baseline(ac4, n=2485, steps=1500):
     import oracle_lib as OL
     lib = OL.load()
     p = ac4.ReplicaParams(n, 1.2, 0.5, 1.0, -1, 0, 99)
-    md = ac4.MD([p], precision=ac4.FP64)
-    load_states(ac4, md, [p])
-    md.relax(300, 1.5, 1.0)
-    md.init_chain(0)
+    ac4.set_tuning(0, 2)
+    md = start(ac4, [p], ac4.FP64, 300)
     g = md.get_state(0)
     chain = md.get_chain(0)
     md.close()
@@ -79,14 +222,27 @@ def cpu_baseline(ac4, n=2485, steps=1500):
                       % (n, steps, dt)}
 
 
+def pmc_traffic():
+    """HBM bytes per k_force launch from the committed rocprofv3 PMC summary."""
+    try:
+        txt = open(PMC_FILE).read()
+    except OSError:
+        return None
+    m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", txt, re.M)
+    return float(m.group(1)) if m else None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--steps", type=int, default=50000)
     ap.add_argument("--warmup", type=int, default=300)
     ap.add_argument("--replicas", type=int, default=64)
-    ap.add_argument("--plain-steps", type=int, default=400)
+    ap.add_argument("--plain-steps", type=int, default=300)
     ap.add_argument("--relax", type=int, default=300)
+    ap.add_argument("--serve-jobs", type=int, default=24)
+    ap.add_argument("--serve-rate", type=float, default=2.0, help="job arrivals per second")
+    ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -98,40 +254,48 @@ def main():
 
     ps = ensemble_params(ac4, a.replicas)
     ntot = sum(p.n for p in ps)
-    md = ac4.MD(ps, precision=ac4.FP32)
-    load_states(ac4, md, ps)
-    md.relax(a.relax, 1.5, 1.0)  # lattice start -> temperature (coldstart/breakxal)
-    for r in range(a.replicas):
-        md.init_chain(r, seed=r)
-    print("bench: %d replicas, %d particles; relaxed %d steps, warmup %d steps" % (a.replicas, ntot, a.relax,
-                                                                                a.warmup), file=sys.stderr,
-          flush=True)
-    md.step(a.warmup, graph=True)
+    tune = None
+    if not a.no_tune:
+        best, table = ac4.autotune(ps, ac4.FP32, relax=a.relax, steps=200)
+        tune = {"tpp": best[0], "nlist_mode": best[1],
+                "candidates": [[int(t), int(m), float(v)] for t, m, v in table]}
+        log("autotune: tpp=%d nlist_mode=%d" % best)
+    md = start(ac4, ps, ac4.FP32, a.relax)
+    sample_every = 100
+    md.overlap_enable(sample_every, 1000, 21, 200, 0.3)
+    log("%d replicas, %d particles; relaxed %d steps, warmup %d steps" % (a.replicas, ntot, a.relax, a.warmup))
+    md.step(a.warmup, graph=True)  # chain steps before the first origin
+    md.sync()
+    # step counters restart at the origin of the accumulators: the first
+    # sample is taken at the first multiple of sample_every
+    done = a.warmup
+    if done % sample_every:
+        md.step(sample_every - done % sample_every, graph=True)
     md.sync()
     t0 = time.perf_counter()
-    md.step(a.steps, graph=True)
+    left = a.steps
+    md.overlap_sample()
+    while left > 0:
+        k = min(sample_every, left)
+        md.step(k, graph=True)
+        left -= k
+        if k == sample_every:
+            md.overlap_sample()
     md.sync()
     t_batched = time.perf_counter() - t0
     value = ntot * a.steps / t_batched
-    print("bench: batched %.3f s for %d steps" % (t_batched, a.steps), file=sys.stderr, flush=True)
+    log("batched %.3f s for %d steps" % (t_batched, a.steps))
+    chi4 = chi4_report(md, ps, sample_every)
     kt = md.kernel_times(200)
     rebuilds = sum(md.rebuilds(r) for r in range(a.replicas))
     md.close()
 
-    # plain single run: one replica, FP64, eager launches
-    p0 = ac4.ReplicaParams(ps[0].n, 1.2, 0.5, 1.0, -1, 200, 7)
-    plain = ac4.MD([p0], precision=ac4.FP64)
-    load_states(ac4, plain, [p0])
-    plain.relax(a.relax, 1.5, 1.0)
-    plain.init_chain(0)
-    plain.step(100)
-    plain.sync()
-    t0 = time.perf_counter()
-    plain.step(a.plain_steps)
-    plain.sync()
-    t_plain = time.perf_counter() - t0
-    plain_value = p0.n * a.plain_steps / t_plain
-    plain.close()
+    plain_value, plain_ms = plain_sequential(ac4, ps, a.relax, a.plain_steps)
+    log("plain sequential %.3e particle-steps/s" % plain_value)
+    serve = None
+    if a.serve_jobs > 0:
+        serve = serving(ac4, a.serve_jobs, a.serve_rate)
+        log("serving p50 %.2f s p99 %.2f s" % (serve["p50_latency_s"], serve["p99_latency_s"]))
 
     t_force = kt["force"] * 1e-3
     achieved = FORCE_BYTES_PER_PARTICLE * ntot / t_force / 1e9
@@ -147,19 +311,24 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32",
-        "data": "synthetic (seeded lattice starts)",
+        "data": "synthetic (seeded lattice starts, relaxed on the GPU)",
         "config": {"workload": "batched ensemble: %d replicas KA 80:20 rho=1.2 T=0.5, N logU[1000,5000] "
-                               "(sum %d), f0 x tau_p grid, FP32 pair forces + FP64 state, hipGraph"
-                               % (a.replicas, ntot),
+                               "(sum %d), f0 x tau_p grid, FP32 pair forces + FP64 state, hipGraph, "
+                               "in-situ Q(t) a=0.3 every 100 steps" % (a.replicas, ntot),
                    "replicas": a.replicas, "particles": ntot, "forces": "fp32", "positions": "fp64"},
-        "plain_single_run": {"value": plain_value, "n": p0.n, "dtype": "fp64", "steps": a.plain_steps,
-                             "ms_per_step": 1e3 * t_plain / a.plain_steps},
+        "plain_single_run": {"value": plain_value, "replicas_run_sequentially": a.replicas, "dtype": "fp64",
+                             "launch": "eager", "steps_each": a.plain_steps,
+                             "ms_per_step_all_replicas": plain_ms},
         "speedup_vs_plain": value / plain_value,
+        "autotune": tune,
+        "chi4": chi4,
+        "serving": serve,
         "kernel_ms_per_step": kt,
         "rebuilds_total": rebuilds,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_force<fp32,kick>", "bytes_per_particle": FORCE_BYTES_PER_PARTICLE},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": pmc_traffic(),
+                     "kernel": "k_force<fp32,kick>", "bytes_per_particle": FORCE_BYTES_PER_PARTICLE,
+                     "particles_per_launch": ntot},
     }
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(ac4)

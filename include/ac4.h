@@ -131,11 +131,15 @@ int ac4_md_sync(ac4_md *md);
 /* Raw Philox-4x32-10 blocks (known-answer tests): out4 = philox(ctr4, key2). */
 int ac4_philox4x32(const uint32_t *ctr4, const uint32_t *key2, uint32_t *out4);
 
-/* Tuning knobs for handles created afterwards (set by an autotuner; no
- * reference counterpart): tpp_max = largest team of lanes per particle in the
- * force kernel (1,2,4,8,16), nlist_mode = 0 (wave per cell, ballot
- * compaction) or 1 (thread per particle). */
-int ac4_set_tuning(int tpp_max, int nlist_mode);
+/* Kernel shape of handles created afterwards (set by the autotuner; no
+ * reference counterpart).  tpp = lanes per particle in the force kernel
+ * (1,2,4,8,16), or 0 = automatic from the handle's total capacity (default).
+ * The team width fixes the summation order of each particle's force, so a
+ * replica's trajectory is bitwise reproducible for a given handle shape.
+ * nlist_mode = dynamics list builder: 0 wave per cell, 1 thread per particle,
+ * 2 teams of 16 lanes per particle (default).  All builders produce the same
+ * lists in the same order. */
+int ac4_set_tuning(int tpp, int nlist_mode);
 
 /* ------------------------------------------------------ slots (serving) --
  * A handle can be created with idle slots of fixed capacity; jobs are
@@ -163,7 +167,7 @@ int ac4_md_relax_slot(ac4_md *md, int r, int nsteps, double f_start, double f_en
 int ac4_md_export_all(ac4_md *md, double *xu, double *yu, double *zu);
 
 /* In-situ overlap Q(t) for every replica with device-resident accumulators
- * (north-star running Q/chi4; the arithmetic of each count is
+ * (running Q/chi4 accumulators; the arithmetic of each count is
  * Analysis/overlap.c:61-84).  Origins every origin_every steps (a ring of
  * n_origin snapshots per replica), lags l*sample_every for l = 0..n_lags.
  * Call ac4_md_overlap_sample whenever the step counter is a multiple of
@@ -278,8 +282,18 @@ int ac4_init_lattice(int n, double T, int n_act, uint64_t seed, double *x, doubl
                      int *ky, int *kz);
 /* read_all_data.c:36-48 */
 int ac4_origin_gap(const int *fidx, int nframes);
-/* Ensemble chi4 over replicas (no reference counterpart, north-star
- * capability): q[r*nor+o]; out2 = {pooled Nbar*(<Q^2>-<Q>^2), mean_r N_r*Var_o Q_r}. */
+/* InitialiseNoseHoover (MDcode/time_evolve.c:476-493) with a seeded draw:
+ * chain9 = {eta[3] = 0, p_eta[3] ~ N(0, T*Q), Q[3]}. */
+void ac4_chain_init(int n, double T, uint64_t seed, double *chain9);
+/* Autotuner (no reference counterpart): from one relaxed start of the given
+ * replica set, times `steps` graph-replayed steps for every kernel shape
+ * (force lanes per particle 1..16, i.e. 256..16 particles per workgroup;
+ * neighbour-list builder) and keeps the fastest with ac4_set_tuning.
+ * table (rows of {tpp, nlist_mode, particle-steps/s}) may be NULL. */
+int ac4_autotune(int n_rep, const ac4_replica_params *p, int precision, int relax_steps, int steps,
+                 int *best2, double *table, int table_cap, int *n_cand);
+/* Ensemble chi4 over replicas (no reference counterpart; chi4 as an
+ * ensemble fluctuation over independent replicas): q[r*nor+o]; out2 = {pooled Nbar*(<Q^2>-<Q>^2), mean_r N_r*Var_o Q_r}. */
 int ac4_chi4_ensemble(int nrep, int nor, const double *q, const int *n_r,
                       double *out2);
 

@@ -1,6 +1,8 @@
 """GPU parity of the MD hot path (libac4.so through the C ABI) against the
 oracle: bit-exact cells and neighbour counts, FP64 forces within 1e-12,
 trajectories, shuffle rule, batch independence, graph replay."""
+import contextlib
+
 import numpy as np
 import pytest
 
@@ -19,6 +21,16 @@ def make_md(ac4, states, f0=1.5, T=0.6, tau=0, prec=0, seeds=None):
         md.set_state(k, s["x"], s["y"], s["z"], s["xu"], s["yu"], s["zu"], s["px"], s["py"], s["pz"], s["type"],
                      s["act_tag"], s["kx"], s["ky"], s["kz"])
     return md
+
+
+@contextlib.contextmanager
+def tuning(ac4, tpp, nlist_mode=2):
+    """Kernel shape for handles created inside the block (default restored)."""
+    ac4.set_tuning(tpp, nlist_mode)
+    try:
+        yield
+    finally:
+        ac4.set_tuning(0, 2)
 
 
 _RELAXED = {}
@@ -153,12 +165,14 @@ def test_shuffle_follows_reference_swap_rule(gpu):
     assert not np.array_equal(after["kx"], before["kx"])
 
 
-@pytest.mark.parametrize("prec", [0, 1])
-def test_replica_alone_equals_in_batch(gpu, prec):
-    alone = make_relaxed_md(gpu, [(1000, 31, 77)], tau=20, prec=prec)
+@pytest.mark.parametrize("prec,tpp", [(0, 4), (1, 1), (1, 8)])
+def test_replica_alone_equals_in_batch(gpu, prec, tpp):
+    # the force team width fixes the summation order: pinned for both handles
+    with tuning(gpu, tpp):
+        alone = make_relaxed_md(gpu, [(1000, 31, 77)], tau=20, prec=prec)
+        mixed = make_relaxed_md(gpu, [(1500, 32, 5), (1000, 31, 77), (2600, 33, 9)], tau=20, prec=prec)
     alone.step(150)
     alone.sync()
-    mixed = make_relaxed_md(gpu, [(1500, 32, 5), (1000, 31, 77), (2600, 33, 9)], tau=20, prec=prec)
     mixed.step(150)
     mixed.sync()
     a, m = alone.get_state(0), mixed.get_state(1)
@@ -166,6 +180,27 @@ def test_replica_alone_equals_in_batch(gpu, prec):
         assert np.array_equal(a[k], m[k]), k
     assert alone.stats(0)["Heff"] == mixed.stats(1)["Heff"]
     assert alone.rebuilds(0) == mixed.rebuilds(1) > 0
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_list_builders_give_identical_trajectories(gpu, prec):
+    """Wave-per-cell, thread-per-particle and team list builders write the
+    same neighbours in the same order, so the dynamics are bitwise equal."""
+    spec = [(1300, 61, 4), (2400, 62, 5)]
+    out = []
+    for mode in (0, 1, 2):
+        with tuning(gpu, 0, mode):
+            md = make_relaxed_md(gpu, spec, tau=30, prec=prec)
+        md.step(200, graph=True)
+        md.sync()
+        out.append([md.get_state(r) for r in range(2)] + [md.rebuilds(0), md.rebuilds(1)])
+        md.close()
+    assert out[0][2] > 0 and out[0][3] > 0
+    for o in out[1:]:
+        assert o[2:] == out[0][2:]
+        for r in range(2):
+            for k in ("xu", "yu", "zu", "px"):
+                assert np.array_equal(o[r][k], out[0][r][k]), (r, k)
 
 
 def test_graph_replay_equals_eager(gpu):

@@ -24,7 +24,7 @@ def lattice_state(n, T=0.6, seed=3, jitter=0.05):
 
 
 def test_box_lengths(oracle):
-    # App. B of the survey / closed form pow(N/rho, 0.33333333) vs pow(N/1.2, 1/3)
+    # closed form pow(N/rho, 0.33333333) (MD) vs pow(N/1.2, 1/3) (Analysis)
     assert oracle.or_md_box_length(1000, 1.2) == pytest.approx(9.4103600778, abs=1e-9)
     assert oracle.or_ana_box_length(1000) == pytest.approx(9.4103602888, abs=1e-9)
     assert oracle.or_md_box_length(10000, 1.2) == pytest.approx(20.2740060418, abs=1e-9)
@@ -235,7 +235,7 @@ def test_schedule(oracle, run, expect):
         assert sched.tolist() == u[:-1] and int(raw[len(sched)]) == u[-1]
     else:
         assert sched.tolist() == u
-    if expect:  # frames written by the run loop (survey App. B counts)
+    if expect:  # frames written by the run loop (schedule restatement counts)
         assert len(u) == expect
 
 
