@@ -457,13 +457,13 @@ def autotune(params, precision=FP32, relax=300, steps=200):
     load()
     arr = (ReplicaParams * len(params))(*params)
     best = (ctypes.c_int * 2)()
-    table = np.zeros((16, 3))
+    table = np.zeros((32, 3))
     nc = ctypes.c_int(0)
-    _check(lib.ac4_autotune(len(params), arr, precision, relax, steps, best, _dp(table), 16, ctypes.byref(nc)))
+    _check(lib.ac4_autotune(len(params), arr, precision, relax, steps, best, _dp(table), 32, ctypes.byref(nc)))
     return (best[0], best[1]), table[: nc.value]
 
 
-def set_tuning(tpp=0, nlist_mode=2):
+def set_tuning(tpp=0, nlist_mode=4):
     """Kernel shape for handles created afterwards (ac4_set_tuning): force
     team width tpp (0 = automatic) and dynamics list builder nlist_mode."""
     load()

@@ -27,9 +27,11 @@
 #include <stdlib.h>
 #include <vector>
 #include <algorithm>
+#include <type_traits>
 #include "ac4_internal.h"
 
 #define BLK 256
+#define LPB 16  // particles per list-build workgroup
 
 // MDcode/global.h:24-45 (Sigma = 1.00)
 #define K_LCUTSQR (6.25 * 1.00 * 1.00)
@@ -46,11 +48,25 @@ struct RepC {
     int nc, ncell, off_cell, tau;      // list grid: nc = floor(L/2.85) cells per axis (27-cell stencil)
     int nblk, blk_off, nc_ref, tpp;    // nc_ref: the reference's grid (Parallel.c:72-80)
     int fnblk, fblk_off, ltpp, cap_n;  // force-kernel block table (tpp = 1<<ltpp lanes per particle)
-    int cap_cells, pad2;
+    int cap_cells, lblk_off;           // lblk: list-build blocks of LPB particles
+    int lnblk, pad2;
     double L, invL, inv_cell_ref, T, f0, dof, density;
-    float Lf, Lhf;
     uint32_t key0, key1;
+    double qscale;  // L / 2^32: real length of one unit of a fixed-point coordinate
+    float qs2, pad3; // qscale^2 in FP32
 };
+
+// Fixed-point copy of a wrapped position for the FP32 force path: each box
+// fraction s in [0,1) as the 32-bit integer floor(s * 2^32) (quantum L/2^32,
+// below 1e-8 sigma for L < 40), type in w.  The 32-bit difference of two
+// such coordinates, read as a signed integer, is the minimum-image separation
+// exactly (cell_list+forces.c:92-97 up to the tie at exactly half a box), and
+// a neighbour costs one 16-byte load instead of two.
+__host__ __device__ __forceinline__ uint4 qpos(double sx, double sy, double sz, int t) {
+    return make_uint4((unsigned int)(unsigned long long)(sx * 4294967296.0),
+                      (unsigned int)(unsigned long long)(sy * 4294967296.0),
+                      (unsigned int)(unsigned long long)(sz * 4294967296.0), (unsigned int)t);
+}
 
 struct ChainS {
     double eta[3], peta[3], q[3];
@@ -119,6 +135,19 @@ __device__ __forceinline__ bool need_rebuild(const RepC &R, const ChainS &c) {
     maxD *= R.L * R.L;  // time_evolve.c:553
     return (c.crit == 1 ? 3.0 : 4.0) * maxD > K_SKIN * K_SKIN;  // :508 (crit 1)
 }
+
+// Rebuild work of one step: the blocks of the replicas due for a rebuild,
+// appended by the last k_bin block of each such replica, so the kernels after
+// k_bin run a fixed grid over this list instead of one workgroup per block of
+// every replica (most of which would have nothing to do).  n[0]: particle
+// blocks of BLK (global block ids); n[1]: list blocks of LPB particles
+// (global ids) at item + off_l; n[2]: cell blocks (replica << 16 | block of
+// BLK/64 cells) at item + off_c.  The counters are cleared by k_first_half
+// (and k_cell_zero for a forced rebuild).
+struct WorkList {
+    int *item, *n;
+    int off_l, off_c;
+};
 
 // Workgroups are dispatched round-robin over the 8 XCDs (workgroup w runs on
 // XCD w % 8).  Kernels that gather neighbour positions launch 8*ceil(nb/8)
@@ -200,10 +229,11 @@ __global__ void __launch_bounds__(BLK) k_first_half(const RepC *__restrict__ rep
                                                     const double *__restrict__ fy, const double *__restrict__ fz,
                                                     const double *__restrict__ xo, const double *__restrict__ yo,
                                                     const double *__restrict__ zo, const int *__restrict__ type,
-                                                    double4 *pos4, ChainS *chain_w, const int *__restrict__ blk_idx,
-                                                    int *cnt) {
+                                                    double4 *pos4, uint4 *posq, int q32, ChainS *chain_w,
+                                                    const int *__restrict__ blk_idx, int *cnt, int *wl_n) {
     __shared__ double s_scale[2];
     __shared__ double s_max[BLK / 64];
+    if (blockIdx.x == 0 && threadIdx.x < 3) wl_n[threadIdx.x] = 0;  // rebuild work of this step
     const int b = blockIdx.x, r = blk_rep[b];
     const RepC R = reps[r];
     if (threadIdx.x == 0) {
@@ -245,7 +275,10 @@ __global__ void __launch_bounds__(BLK) k_first_half(const RepC *__restrict__ rep
         while (sz < 0.0) sz += 1.0;
         x[g] = sx; y[g] = sy; z[g] = sz;
         xu[g] = ux; yu[g] = uy; zu[g] = uz;
-        pos4[g] = make_double4(sx, sy, sz, (double)type[g]);
+        if (q32)
+            posq[g] = qpos(sx, sy, sz, type[g]);
+        else
+            pos4[g] = make_double4(sx, sy, sz, (double)type[g]);
         double ex = ux - xo[g], ey = uy - yo[g], ez = uz - zo[g];  // :547-551
         d2 = ex * ex + ey * ey + ez * ez;
     }
@@ -265,7 +298,8 @@ __global__ void __launch_bounds__(BLK) k_first_half(const RepC *__restrict__ rep
 // reference's own grid (MDcode/Parallel.c:72-80) is evaluated beside it for
 // the cell ids update_cell_list reports.
 __global__ void __launch_bounds__(BLK) k_cell_zero(const RepC *__restrict__ reps, const ChainS *__restrict__ chain,
-                                                   int *cnt) {
+                                                   int *cnt, int *wl_n) {
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < 3) wl_n[threadIdx.x] = 0;
     const RepC R = reps[blockIdx.x];
     const int c = blockIdx.y * BLK + threadIdx.x;
     if (c >= R.ncell || !need_rebuild(R, chain[blockIdx.x])) return;
@@ -283,8 +317,8 @@ __global__ void __launch_bounds__(BLK) k_bin(const RepC *__restrict__ reps, Chai
                                              const int *__restrict__ blk_rep, const int *__restrict__ blk_first,
                                              const double *__restrict__ x, const double *__restrict__ y,
                                              const double *__restrict__ z, int *cnt, int *cell_of, int *rank,
-                                             int *cell_ref, int *start) {
-    __shared__ int s_last, s_w[BLK / 64];
+                                             int *cell_ref, int *start, WorkList wl) {
+    __shared__ int s_last, s_w[BLK / 64], s_base[3];
     const int b = blockIdx.x, r = blk_rep[b];
     const RepC R = reps[r];
     if (!need_rebuild(R, chain[r])) return;  // uniform over the replica's blocks
@@ -347,20 +381,34 @@ __global__ void __launch_bounds__(BLK) k_bin(const RepC *__restrict__ reps, Chai
         start[R.off_cell + q] = acc;
         acc += vc[q];
     }
-    if (threadIdx.x == 0) chain[r].bin_done = 0u;
+    // this replica's share of the step's rebuild work
+    const int nb = (R.n + BLK - 1) / BLK, nl = (R.n + LPB - 1) / LPB, ncb = (R.ncell + BLK / 64 - 1) / (BLK / 64);
+    if (threadIdx.x == 0) {
+        s_base[0] = atomicAdd(&wl.n[0], nb);
+        s_base[1] = atomicAdd(&wl.n[1], nl);
+        s_base[2] = atomicAdd(&wl.n[2], ncb);
+        chain[r].bin_done = 0u;
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < nb; q += BLK) wl.item[s_base[0] + q] = R.blk_off + q;
+    for (int q = threadIdx.x; q < nl; q += BLK) wl.item[wl.off_l + s_base[1] + q] = R.lblk_off + q;
+    for (int q = threadIdx.x; q < ncb; q += BLK) wl.item[wl.off_c + s_base[2] + q] = (r << 16) | q;
 }
 
-__global__ void __launch_bounds__(BLK) k_scatter(const RepC *__restrict__ reps, const ChainS *__restrict__ chain,
-                                                 const int *__restrict__ blk_rep, const int *__restrict__ blk_first,
+__global__ void __launch_bounds__(BLK) k_scatter(const RepC *__restrict__ reps, const int *__restrict__ blk_rep,
+                                                 const int *__restrict__ blk_first, WorkList wl,
                                                  const int *__restrict__ cell_of, const int *__restrict__ rank,
                                                  const int *__restrict__ start, int *members) {
-    const int b = blockIdx.x, r = blk_rep[b];
-    const RepC R = reps[r];
-    if (!need_rebuild(R, chain[r])) return;
-    const int i = blk_first[b] + threadIdx.x;
-    if (i >= R.n) return;
-    const int g = R.off + i;
-    members[R.off + start[R.off_cell + cell_of[g]] + rank[g]] = i;
+    const int nit = wl.n[0];
+    for (int it = blockIdx.x; it < nit; it += gridDim.x) {
+        const int b = wl.item[it], r = blk_rep[b];
+        const RepC R = reps[r];
+        const int i = blk_first[b] + threadIdx.x;
+        if (i < R.n) {
+            const int g = R.off + i;
+            members[R.off + start[R.off_cell + cell_of[g]] + rank[g]] = i;
+        }
+    }
 }
 
 // Members of a cell in increasing original index (the binning order of
@@ -368,32 +416,36 @@ __global__ void __launch_bounds__(BLK) k_scatter(const RepC *__restrict__ reps, 
 // depend on launch timing or on which replicas share the launch.  One wave per
 // cell: keys staged in LDS, each lane ranks its entries by counting.
 #define SORT_CAP 256
-__global__ void __launch_bounds__(BLK) k_cell_sort(const RepC *__restrict__ reps, const ChainS *__restrict__ chain,
+__global__ void __launch_bounds__(BLK) k_cell_sort(const RepC *__restrict__ reps, WorkList wl,
                                                    const int *__restrict__ cnt, const int *__restrict__ start,
                                                    const int *__restrict__ id, int *members) {
     __shared__ int s_m[BLK / 64][SORT_CAP];
     __shared__ int s_k[BLK / 64][SORT_CAP];
-    const RepC R = reps[blockIdx.x];
-    if (!need_rebuild(R, chain[blockIdx.x])) return;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int c = (blockIdx.y * (BLK / 64)) + w;
-    if (c >= R.ncell) return;
-    int *m = members + R.off + start[R.off_cell + c];
-    const int n = cnt[R.off_cell + c];
-    if (n <= SORT_CAP) {
-        for (int k = lane; k < n; k += 64) { s_m[w][k] = m[k]; s_k[w][k] = id[R.off + m[k]]; }
-        __syncwarp();
-        for (int k = lane; k < n; k += 64) {
-            const int v = s_k[w][k];
-            int rank = 0;
-            for (int q = 0; q < n; q++) rank += s_k[w][q] < v;
-            m[rank] = s_m[w][k];
-        }
-    } else if (lane == 0) {  // crowded cell: serial insertion sort on the key
-        for (int a = 1; a < n; a++) {
-            int v = m[a], kv = id[R.off + v], k = a - 1;
-            while (k >= 0 && id[R.off + m[k]] > kv) { m[k + 1] = m[k]; k--; }
-            m[k + 1] = v;
+    const int nit = wl.n[2];
+    for (int it = blockIdx.x; it < nit; it += gridDim.x) {
+        const int item = wl.item[wl.off_c + it], r = item >> 16;
+        const RepC R = reps[r];
+        const int c = (item & 0xffff) * (BLK / 64) + w;  // one cell per wave
+        if (c >= R.ncell) continue;
+        int *m = members + R.off + start[R.off_cell + c];
+        const int n = cnt[R.off_cell + c];
+        if (n <= SORT_CAP) {
+            for (int k = lane; k < n; k += 64) { s_m[w][k] = m[k]; s_k[w][k] = id[R.off + m[k]]; }
+            __syncwarp();
+            for (int k = lane; k < n; k += 64) {
+                const int v = s_k[w][k];
+                int rank = 0;
+                for (int q = 0; q < n; q++) rank += s_k[w][q] < v;
+                m[rank] = s_m[w][k];
+            }
+            __syncwarp();
+        } else if (lane == 0) {  // crowded cell: serial insertion sort on the key
+            for (int a = 1; a < n; a++) {
+                int v = m[a], kv = id[R.off + v], k = a - 1;
+                while (k >= 0 && id[R.off + m[k]] > kv) { m[k + 1] = m[k]; k--; }
+                m[k + 1] = v;
+            }
         }
     }
 }
@@ -404,44 +456,48 @@ struct PartArrays {  // particle arrays that move with a particle
 };
 
 // new slot s <- old slot members[s]
-__global__ void __launch_bounds__(BLK) k_gather(const RepC *__restrict__ reps, const ChainS *__restrict__ chain,
-                                                const int *__restrict__ blk_rep, const int *__restrict__ blk_first,
+__global__ void __launch_bounds__(BLK) k_gather(const RepC *__restrict__ reps, const int *__restrict__ blk_rep,
+                                                const int *__restrict__ blk_first, WorkList wl,
                                                 const int *__restrict__ members, PartArrays src, PartArrays dst) {
-    const int b = blockIdx.x, r = blk_rep[b];
-    const RepC R = reps[r];
-    if (!need_rebuild(R, chain[r])) return;
-    const int i = blk_first[b] + threadIdx.x;
-    if (i >= R.n) return;
-    const int g = R.off + i, o = R.off + members[g];
-    dst.x[g] = src.x[o]; dst.y[g] = src.y[o]; dst.z[g] = src.z[o];
-    dst.xu[g] = src.xu[o]; dst.yu[g] = src.yu[o]; dst.zu[g] = src.zu[o];
-    dst.px[g] = src.px[o]; dst.py[g] = src.py[o]; dst.pz[g] = src.pz[o];
-    dst.type[g] = src.type[o]; dst.act[g] = src.act[o]; dst.id[g] = src.id[o];
-    dst.cell_ref[g] = src.cell_ref[o]; dst.cell_of[g] = src.cell_of[o];
+    const int nit = wl.n[0];
+    for (int it = blockIdx.x; it < nit; it += gridDim.x) {
+        const int b = wl.item[it], r = blk_rep[b];
+        const RepC R = reps[r];
+        const int i = blk_first[b] + threadIdx.x;
+        if (i >= R.n) continue;
+        const int g = R.off + i, o = R.off + members[g];
+        dst.x[g] = src.x[o]; dst.y[g] = src.y[o]; dst.z[g] = src.z[o];
+        dst.xu[g] = src.xu[o]; dst.yu[g] = src.yu[o]; dst.zu[g] = src.zu[o];
+        dst.px[g] = src.px[o]; dst.py[g] = src.py[o]; dst.pz[g] = src.pz[o];
+        dst.type[g] = src.type[o]; dst.act[g] = src.act[o]; dst.id[g] = src.id[o];
+        dst.cell_ref[g] = src.cell_ref[o]; dst.cell_of[g] = src.cell_of[o];
+    }
 }
 
-__global__ void __launch_bounds__(BLK) k_commit(const RepC *__restrict__ reps, const ChainS *__restrict__ chain,
-                                                const int *__restrict__ blk_rep, const int *__restrict__ blk_first,
-                                                PartArrays src, PartArrays dst, double4 *pos4, float4 *pos4f,
+__global__ void __launch_bounds__(BLK) k_commit(const RepC *__restrict__ reps, const int *__restrict__ blk_rep,
+                                                const int *__restrict__ blk_first, WorkList wl,
+                                                PartArrays src, PartArrays dst, double4 *pos4, uint4 *posq,
                                                 double *xo, double *yo, double *zo) {
-    const int b = blockIdx.x, r = blk_rep[b];
-    const RepC R = reps[r];
-    if (!need_rebuild(R, chain[r])) return;
-    const int i = blk_first[b] + threadIdx.x;
-    if (i >= R.n) return;
-    const int g = R.off + i;
-    const double sx = src.x[g], sy = src.y[g], sz = src.z[g], ux = src.xu[g], uy = src.yu[g], uz = src.zu[g];
-    const int t = src.type[g];
-    dst.x[g] = sx; dst.y[g] = sy; dst.z[g] = sz;
-    dst.xu[g] = ux; dst.yu[g] = uy; dst.zu[g] = uz;
-    dst.px[g] = src.px[g]; dst.py[g] = src.py[g]; dst.pz[g] = src.pz[g];
-    dst.type[g] = t; dst.act[g] = src.act[g]; dst.id[g] = src.id[g];
-    dst.cell_ref[g] = src.cell_ref[g]; dst.cell_of[g] = src.cell_of[g];
-    pos4[g] = make_double4(sx, sy, sz, (double)t);
-    pos4f[g] = make_float4((float)(sx * R.L), (float)(sy * R.L), (float)(sz * R.L), (float)t);
-    xo[g] = ux;  // update_cell_list, cell_list+forces.c:141-146
-    yo[g] = uy;
-    zo[g] = uz;
+    const int nit = wl.n[0];
+    for (int it = blockIdx.x; it < nit; it += gridDim.x) {
+        const int b = wl.item[it], r = blk_rep[b];
+        const RepC R = reps[r];
+        const int i = blk_first[b] + threadIdx.x;
+        if (i >= R.n) continue;
+        const int g = R.off + i;
+        const double sx = src.x[g], sy = src.y[g], sz = src.z[g], ux = src.xu[g], uy = src.yu[g], uz = src.zu[g];
+        const int t = src.type[g];
+        dst.x[g] = sx; dst.y[g] = sy; dst.z[g] = sz;
+        dst.xu[g] = ux; dst.yu[g] = uy; dst.zu[g] = uz;
+        dst.px[g] = src.px[g]; dst.py[g] = src.py[g]; dst.pz[g] = src.pz[g];
+        dst.type[g] = t; dst.act[g] = src.act[g]; dst.id[g] = src.id[g];
+        dst.cell_ref[g] = src.cell_ref[g]; dst.cell_of[g] = src.cell_of[g];
+        pos4[g] = make_double4(sx, sy, sz, (double)t);
+        posq[g] = qpos(sx, sy, sz, t);
+        xo[g] = ux;  // update_cell_list, cell_list+forces.c:141-146
+        yo[g] = uy;
+        zo[g] = uz;
+    }
 }
 
 // Distinct neighbour-cell coordinates along one axis (cells are at least one
@@ -471,6 +527,16 @@ __device__ __forceinline__ size_t nl_at(const RepC &R, int cap, int k, int i) {
     return (size_t)R.off * cap + (((size_t)(k >> R.ltpp) * R.n + i) << R.ltpp) + (k & (R.tpp - 1));
 }
 
+// Dynamics-list screen shared by every builder (so all of them keep exactly
+// the same pairs): exact minimum image from the fixed-point positions, r^2 in
+// FP32 against the list radius plus a margin of 1e-3 sigma^2, far above the
+// FP32 error of r^2 (below 1e-5 for the list radii), so no pair within its
+// list radius is lost.
+__device__ __forceinline__ bool qscreen(const uint4 &pi, const uint4 &q, float qs2, float cfa, float cfb) {
+    const float dx = (float)(int)(pi.x - q.x), dy = (float)(int)(pi.y - q.y), dz = (float)(int)(pi.z - q.z);
+    return (dx * dx + dy * dy + dz * dz) * qs2 < (q.w == 0u ? cfa : cfb);
+}
+
 #define NL_UNROLL 8
 // Thread per particle (slots are in cell order, so the lanes of a wave scan
 // the same few cells and the position loads are shared in L1); candidate
@@ -483,7 +549,7 @@ __device__ __forceinline__ size_t nl_at(const RepC &R, int cap, int k, int i) {
 template <bool EXACT>
 __global__ void __launch_bounds__(BLK) k_nlist(const RepC *__restrict__ reps, ChainS *chain,
                                                const int *__restrict__ blk_rep, const int *__restrict__ blk_first,
-                                               const double4 *__restrict__ pos4, const float4 *__restrict__ pos4f,
+                                               const double4 *__restrict__ pos4, const uint4 *__restrict__ posq,
                                                const int *__restrict__ cell_of, const int *__restrict__ cnt,
                                                const int *__restrict__ start, int *nn, int *nn_ref, int *nl,
                                                int stride, int cap) {
@@ -494,7 +560,7 @@ __global__ void __launch_bounds__(BLK) k_nlist(const RepC *__restrict__ reps, Ch
     if (i >= R.n) return;
     const int g = R.off + i, nc = R.nc;
     const double4 pi = pos4[g];
-    const float4 pf = pos4f[g];
+    const uint4 pq = posq[g];
     const int ti = (int)pi.w;
     const int c = cell_of[g];
     const int cx = c % nc, cy = (c / nc) % nc, cz = c / (nc * nc);
@@ -502,7 +568,7 @@ __global__ void __launch_bounds__(BLK) k_nlist(const RepC *__restrict__ reps, Ch
     const int nx = stencil_axis(cx, nc, ax), ny = stencil_axis(cy, nc, ay), nz = stencil_axis(cz, nc, az);
     const double L = R.L;
     const double cut_a = list_cut2(ti), cut_b = list_cut2(ti + 1);
-    const float cfa = (float)cut_a + 1e-3f, cfb = (float)cut_b + 1e-3f, Lf = R.Lf, Lh = R.Lhf;
+    const float cfa = (float)cut_a + 1e-3f, cfb = (float)cut_b + 1e-3f;
     int count = 0, cref = 0;
     for (int iz = 0; iz < nz; iz++)
         for (int iy = 0; iy < ny; iy++)
@@ -530,19 +596,14 @@ __global__ void __launch_bounds__(BLK) k_nlist(const RepC *__restrict__ reps, Ch
                             }
                         }
                     } else {
-                        float4 q[NL_UNROLL];
+                        uint4 q[NL_UNROLL];
 #pragma unroll
-                        for (int u = 0; u < NL_UNROLL; u++) q[u] = pos4f[min(k + u, e2 - 1)];
+                        for (int u = 0; u < NL_UNROLL; u++) q[u] = posq[min(k + u, e2 - 1)];
 #pragma unroll
                         for (int u = 0; u < NL_UNROLL; u++) {
                             const int j = k + u;
                             if (j >= e2 || j == g) continue;
-                            float dx = pf.x - q[u].x, dy = pf.y - q[u].y, dz = pf.z - q[u].z;
-                            dx = dx > Lh ? dx - Lf : (dx < -Lh ? dx + Lf : dx);
-                            dy = dy > Lh ? dy - Lf : (dy < -Lh ? dy + Lf : dy);
-                            dz = dz > Lh ? dz - Lf : (dz < -Lh ? dz + Lf : dz);
-                            const float r2 = dx * dx + dy * dy + dz * dz;
-                            if (r2 < (q[u].w == 0.0f ? cfa : cfb)) {
+                            if (qscreen(pq, q[u], R.qs2, cfa, cfb)) {
                                 if (count < cap) nl[nl_at(R, cap, count, i)] = j;
                                 count++;
                             }
@@ -555,67 +616,88 @@ __global__ void __launch_bounds__(BLK) k_nlist(const RepC *__restrict__ reps, Ch
     if (EXACT) nn_ref[g] = cref;
 }
 
-// Dynamics lists, a team of NLT lanes per particle: the team tests NLT
-// consecutive candidates of a stencil cell at once and appends the hits in
-// candidate order (team ballot + popcount of the lanes below), so the list is
-// in the order of a serial scan (stencil cells z, y, x; slots within a cell)
-// while the chain of dependent loads per particle is NLT times shorter.  The
+// Dynamics lists, a team of NLT lanes per particle (16, 32 or 64; LPB = 16
+// particles per workgroup): the team tests NLT consecutive candidates at once
+// and appends the hits in candidate order (team ballot + popcount of the
+// lanes below), so the list is in the order of a serial scan (stencil cells
+// z, y, x; slots within a cell) while the chain of dependent loads per
+// particle is NLT times shorter.  Only the replicas due for a rebuild do work
+// in a step, so wide teams are what keeps enough waves in flight.  The
 // control flow is uniform within a team (one home particle per team), so
 // every lane of a team is active at each ballot.  FP32 screen with the margin
 // described at k_nlist.
-#define NLT 16
-__global__ void __launch_bounds__(BLK) k_nlist_team(const RepC *__restrict__ reps, ChainS *chain,
-                                                    const int *__restrict__ lblk_rep,
-                                                    const int *__restrict__ lblk_first,
-                                                    const float4 *__restrict__ pos4f, const int *__restrict__ cell_of,
-                                                    const int *__restrict__ cnt, const int *__restrict__ start,
-                                                    int *nn, int *nl, int cap) {
-    // plain block order: only the replicas due for a rebuild do work, so
-    // they are spread over every XCD
-    const int b = blockIdx.x, r = lblk_rep[b];
-    const RepC R = reps[r];
-    if (!need_rebuild(R, chain[r])) return;
-    const int i = lblk_first[b] + (int)threadIdx.x / NLT;
-    if (i >= R.n) return;  // whole team
+template <int NLT>
+__device__ __forceinline__ void nlist_team_one(const RepC &R, int r, int i, ChainS *chain,
+                                               const uint4 *__restrict__ posq, const int *__restrict__ cell_of,
+                                               const int *__restrict__ cnt, const int *__restrict__ start, int *nn,
+                                               int *nl, int cap) {
     const int lane = threadIdx.x & 63, t = lane & (NLT - 1);
     const unsigned long long tmask = (NLT == 64 ? ~0ull : ((1ull << NLT) - 1ull)) << (lane & ~(NLT - 1));
     const unsigned long long below = (1ull << lane) - 1ull;
     const int g = R.off + i, nc = R.nc;
-    const float4 pf = pos4f[g];
-    const int ti = (int)pf.w;
+    const uint4 pi = posq[g];
+    const int ti = (int)pi.w;
     const int c = cell_of[g];
     const int cx = c % nc, cy = (c / nc) % nc, cz = c / (nc * nc);
     int ax[3], ay[3], az[3];
     const int nx = stencil_axis(cx, nc, ax), ny = stencil_axis(cy, nc, ay), nz = stencil_axis(cz, nc, az);
-    const float cfa = (float)list_cut2(ti) + 1e-3f, cfb = (float)list_cut2(ti + 1) + 1e-3f, Lf = R.Lf, Lh = R.Lhf;
+    const float cfa = (float)list_cut2(ti) + 1e-3f, cfb = (float)list_cut2(ti + 1) + 1e-3f, qs2 = R.qs2;
     int count = 0;
+    auto inlist = [&](const uint4 &q) -> bool { return qscreen(pi, q, qs2, cfa, cfb); };
+    auto emit = [&](bool hit, int k) {
+        const unsigned long long m = __ballot(hit) & tmask;
+        if (hit) {
+            const int pos = count + __popcll(m & below);
+            if (pos < cap) nl[nl_at(R, cap, pos, i)] = k;
+        }
+        count += __popcll(m);
+    };
     for (int iz = 0; iz < nz; iz++)
-        for (int iy = 0; iy < ny; iy++)
-            for (int ix = 0; ix < nx; ix++) {
-                const int c2 = R.off_cell + ax[ix] + ay[iy] * nc + az[iz] * nc * nc;
-                const int s2 = R.off + start[c2], e2 = s2 + cnt[c2];
-                for (int k0 = s2; k0 < e2; k0 += NLT) {
-                    const int k = k0 + t;
-                    bool hit = false;
-                    if (k < e2 && k != g) {
-                        const float4 q = pos4f[k];
-                        float dx = pf.x - q.x, dy = pf.y - q.y, dz = pf.z - q.z;
-                        dx = dx > Lh ? dx - Lf : (dx < -Lh ? dx + Lf : dx);
-                        dy = dy > Lh ? dy - Lf : (dy < -Lh ? dy + Lf : dy);
-                        dz = dz > Lh ? dz - Lf : (dz < -Lh ? dz + Lf : dz);
-                        hit = dx * dx + dy * dy + dz * dz < (q.w == 0.0f ? cfa : cfb);
-                    }
-                    const unsigned long long m = __ballot(hit) & tmask;
-                    if (hit) {
-                        const int pos = count + __popcll(m & below);
-                        if (pos < cap) nl[nl_at(R, cap, pos, i)] = k;
-                    }
-                    count += __popcll(m);
+        for (int iy = 0; iy < ny; iy++) {
+            // cells of one x-row that follow each other in cell order are one
+            // contiguous slot range (two ranges where the row wraps), taken in
+            // stencil order
+            const int row = R.off_cell + ay[iy] * nc + az[iz] * nc * nc;
+            int ix = 0;
+            while (ix < nx) {
+                const int cf = ax[ix];
+                int cl = cf;
+                while (ix + 1 < nx && ax[ix + 1] == cl + 1) { ix++; cl++; }
+                ix++;
+                const int s2 = R.off + start[row + cf], e2 = R.off + start[row + cl] + cnt[row + cl];
+                // two candidates per lane, both loaded before either is used
+                // (indices clamped into [s2, e2), validity applied after)
+                for (int k0 = s2; k0 < e2; k0 += 2 * NLT) {
+                    const int ka = k0 + t, kb = k0 + NLT + t;
+                    const uint4 qa = posq[min(ka, e2 - 1)], qb = posq[min(kb, e2 - 1)];
+                    const bool ha = inlist(qa) && ka < e2 && ka != g, hb = inlist(qb) && kb < e2 && kb != g;
+                    emit(ha, ka);
+                    emit(hb, kb);
                 }
             }
+        }
     if (t == 0) {
         if (count > cap) atomicOr(&chain[r].err, 1);
         nn[g] = min(count, cap);
+    }
+}
+
+// Fixed grid over the step's list blocks (LPB particles each, one team per
+// particle); plain block order spreads the few rebuilding replicas over all
+// XCDs.
+template <int NLT>
+__global__ void __launch_bounds__(NLT * LPB) k_nlist_team(const RepC *__restrict__ reps, ChainS *chain,
+                                                          const int *__restrict__ lblk_rep,
+                                                          const int *__restrict__ lblk_first, WorkList wl,
+                                                          const uint4 *__restrict__ posq,
+                                                          const int *__restrict__ cell_of, const int *__restrict__ cnt,
+                                                          const int *__restrict__ start, int *nn, int *nl, int cap) {
+    const int nit = wl.n[1];
+    for (int it = blockIdx.x; it < nit; it += gridDim.x) {
+        const int b = wl.item[wl.off_l + it], r = lblk_rep[b];
+        const RepC R = reps[r];
+        const int i = lblk_first[b] + (int)threadIdx.x / NLT;
+        if (i < R.n) nlist_team_one<NLT>(R, r, i, chain, posq, cell_of, cnt, start, nn, nl, cap);  // whole team
     }
 }
 
@@ -628,9 +710,9 @@ __global__ void __launch_bounds__(BLK) k_nlist_team(const RepC *__restrict__ rep
 #define NLC_CAP 2304
 __global__ void __launch_bounds__(64) k_nlist_cell(const RepC *__restrict__ reps, ChainS *chain,
                                                    const int *__restrict__ cblk_rep, const int *__restrict__ cblk_cell,
-                                                   const float4 *__restrict__ pos4f, const int *__restrict__ cnt,
+                                                   const uint4 *__restrict__ posq, const int *__restrict__ cnt,
                                                    const int *__restrict__ start, int *nn, int *nl, int cap) {
-    __shared__ float4 s_c[NLC_CAP];
+    __shared__ uint4 s_c[NLC_CAP];
     __shared__ int s_pre[28], s_st[27];
     const int r = cblk_rep[blockIdx.x], c = cblk_cell[blockIdx.x];
     const RepC R = reps[r];
@@ -667,30 +749,20 @@ __global__ void __launch_bounds__(64) k_nlist_cell(const RepC *__restrict__ reps
     for (int k = lane; k < total; k += 64) {  // independent loads
         int q = 0;
         while (s_pre[q + 1] <= k) q++;
-        s_c[k] = pos4f[R.off + s_st[q] + (k - s_pre[q])];
+        s_c[k] = posq[R.off + s_st[q] + (k - s_pre[q])];
     }
     __syncthreads();
-    const float Lf = R.Lf, Lh = R.Lhf;
-    const float cut_0 = (float)list_cut2(0) + 1e-3f, cut_1 = (float)list_cut2(1) + 1e-3f,
-                cut_2 = (float)list_cut2(2) + 1e-3f;
     const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     int mine = 0;  // list length of home particle `lane` (and lane+64, ... kept below)
     for (int h = 0; h < hn; h++) {
-        const float4 pi = s_c[hoff + h];
+        const uint4 pi = s_c[hoff + h];
         const int ti = (int)pi.w, kh = hoff + h;
+        const float cfa = (float)list_cut2(ti) + 1e-3f, cfb = (float)list_cut2(ti + 1) + 1e-3f;
         int base = 0;
         for (int k0 = 0; k0 < total; k0 += 64) {
             const int k = k0 + lane;
             bool hit = false;
-            if (k < total && k != kh) {
-                const float4 q = s_c[k];
-                float dx = pi.x - q.x, dy = pi.y - q.y, dz = pi.z - q.z;
-                dx = dx > Lh ? dx - Lf : (dx < -Lh ? dx + Lf : dx);
-                dy = dy > Lh ? dy - Lf : (dy < -Lh ? dy + Lf : dy);
-                dz = dz > Lh ? dz - Lf : (dz < -Lh ? dz + Lf : dz);
-                const int ab = ti + (int)q.w;
-                hit = (dx * dx + dy * dy + dz * dz) < (ab == 0 ? cut_0 : (ab == 1 ? cut_1 : cut_2));
-            }
+            if (k < total && k != kh) hit = qscreen(pi, s_c[k], R.qs2, cfa, cfb);
             const unsigned long long mask = __ballot(hit);
             if (hit) {
                 const int pos = base + __popcll(mask & below);
@@ -826,10 +898,12 @@ struct PairAcc;
 template <>
 struct PairAcc<false> {  // FP64: the reference's per-pair arithmetic
     double fx = 0, fy = 0, fz = 0, pe = 0, vir = 0;
-    __device__ __forceinline__ void add(double xi, double yi, double zi, int ti, const double4 pj, double L) {
-        double dx = minimg_md(xi - pj.x);
-        double dy = minimg_md(yi - pj.y);
-        double dz = minimg_md(zi - pj.z);
+    __device__ __forceinline__ explicit PairAcc(int) {}
+    __device__ __forceinline__ void add(const double4 &pi, int ti, const double4 &pj, const RepC &R) {
+        const double L = R.L;
+        double dx = minimg_md(pi.x - pj.x);
+        double dy = minimg_md(pi.y - pj.y);
+        double dz = minimg_md(pi.z - pj.z);
         double dr2 = (dx * dx + dy * dy + dz * dz) * L * L;  // cell_list+forces.c:183
         const int tj = (int)pj.w;
         double eps = eps_ab(ti, tj), sig = sig_ab(ti, tj);
@@ -845,43 +919,67 @@ struct PairAcc<false> {  // FP64: the reference's per-pair arithmetic
             vir += grad * dr2;
         }
     }
-    __device__ __forceinline__ void result(double &ox, double &oy, double &oz) const { ox = fx; oy = fy; oz = fz; }
+    __device__ __forceinline__ void result(double &ox, double &oy, double &oz, const RepC &) const {
+        ox = fx; oy = fy; oz = fz;
+    }
 };
 
 template <>
-struct PairAcc<true> {  // FP32 pair forces, FP64 minimum image, Kahan sums per particle
+struct PairAcc<true> {
+    // FP32 pair forces from fixed-point positions (qpos): exact minimum
+    // image in integers, separations in units of L/2^32 rounded once to FP32,
+    // per-particle Kahan sums in those units, scaled to real units at the end.
+    // The pair constants of the two possible partner species are set once per
+    // particle; one hardware reciprocal of r^2 (v_rcp_f32, within 1 ulp)
+    // replaces the two divisions, since sigma^2/r^2 = (sigma/r)^2 and
+    // 1/r^2 appears in the force factor.
     float fx = 0, fy = 0, fz = 0, cx = 0, cy = 0, cz = 0;
     double pe = 0, vir = 0;
+    float isg[2], sg2[2], e4[2], e8[2], rc2[2];
+    __device__ __forceinline__ explicit PairAcc(int ti) {
+        // (ti, tj) -> ab = ti + tj: sigma {1, 0.8, 0.88}, epsilon {1, 1.5, 0.5}
+        // (cell_list+forces.c:19-20); cutoff (r/sigma)^2 < 6.25 (:189)
+        for (int tj = 0; tj < 2; tj++) {
+            const int ab = ti + tj;
+            const double sig = ab == 0 ? 1.00 : (ab == 1 ? 0.80 : 0.88);
+            const double eps = ab == 0 ? 1.00 : (ab == 1 ? 1.50 : 0.50);
+            isg[tj] = (float)(1.0 / (sig * sig));
+            sg2[tj] = (float)(sig * sig);
+            e4[tj] = (float)(4.0 * eps);
+            e8[tj] = (float)(8.0 * eps);
+            rc2[tj] = (float)(K_LCUTSQR * sig * sig);
+        }
+    }
     __device__ __forceinline__ static void kahan(float &s, float &c, float v) {
         float y = v - c;
         float t = s + y;
         c = (t - s) - y;
         s = t;
     }
-    __device__ __forceinline__ void add(double xi, double yi, double zi, int ti, const double4 pj, double L) {
-        const float dx = (float)(minimg_md(xi - pj.x) * L);
-        const float dy = (float)(minimg_md(yi - pj.y) * L);
-        const float dz = (float)(minimg_md(zi - pj.z) * L);
-        const int tj = (int)pj.w, ab = ti + tj;
-        const float isig2 = ab == 0 ? 1.0f : (ab == 1 ? (float)(1.0 / (0.80 * 0.80)) : (float)(1.0 / (0.88 * 0.88)));
-        const float eps = ab == 0 ? 1.0f : (ab == 1 ? 1.5f : 0.5f);
-        const float r2 = dx * dx + dy * dy + dz * dz;
-        const float rs2 = r2 * isig2;
-        if (rs2 < 6.25f) {
-            const float s2 = 1.0f / rs2;
+    __device__ __forceinline__ void add(const uint4 &pi, int, const uint4 &pj, const RepC &R) {
+        const float dx = (float)(int)(pi.x - pj.x);
+        const float dy = (float)(int)(pi.y - pj.y);
+        const float dz = (float)(int)(pi.z - pj.z);
+        const bool b = pj.w != 0u;
+        const float r2 = (dx * dx + dy * dy + dz * dz) * R.qs2;  // real units
+        if (r2 < (b ? rc2[1] : rc2[0])) {
+            const float isig2 = b ? isg[1] : isg[0], sig2 = b ? sg2[1] : sg2[0];
+            const float inv = __builtin_amdgcn_rcpf(r2);
+            const float s2 = sig2 * inv;  // (sigma/r)^2
             const float s6 = s2 * s2 * s2;
-            pe += (double)(4.0f * eps * ((s6 * (s6 - 1.0f)) + (float)K_C0 + (float)K_C2 * rs2));
-            const float grad = 8.0f * eps * (6.0f * s6 * (s6 - 0.5f) / r2 - (float)K_C2 * isig2);
+            const float rs2 = r2 * isig2;
+            pe += (double)((b ? e4[1] : e4[0]) * ((s6 * (s6 - 1.0f)) + (float)K_C0 + (float)K_C2 * rs2));
+            const float grad = (b ? e8[1] : e8[0]) * (6.0f * s6 * (s6 - 0.5f) * inv - (float)K_C2 * isig2);
             kahan(fx, cx, grad * dx);
             kahan(fy, cy, grad * dy);
             kahan(fz, cz, grad * dz);
             vir += (double)(grad * r2);
         }
     }
-    __device__ __forceinline__ void result(double &ox, double &oy, double &oz) const {
-        ox = (double)fx - (double)cx;
-        oy = (double)fy - (double)cy;
-        oz = (double)fz - (double)cz;
+    __device__ __forceinline__ void result(double &ox, double &oy, double &oz, const RepC &R) const {
+        ox = ((double)fx - (double)cx) * R.qscale;
+        oy = ((double)fy - (double)cy) * R.qscale;
+        oz = ((double)fz - (double)cz) * R.qscale;
     }
 };
 
@@ -892,7 +990,8 @@ struct PairAcc<true> {  // FP32 pair forces, FP64 minimum image, Kahan sums per 
 template <bool FP32, bool KICK>
 __global__ void __launch_bounds__(BLK) k_force(const RepC *__restrict__ reps, ChainS *chain,
                                                const int *__restrict__ fblk_rep, const int *__restrict__ fblk_first,
-                                               const double4 *__restrict__ pos4, const int *__restrict__ nn,
+                                               const double4 *__restrict__ pos4, const uint4 *__restrict__ posq,
+                                               const int *__restrict__ nn,
                                                const int *__restrict__ nl, int cap, const int *__restrict__ act,
                                                const int8_t *__restrict__ kx, const int8_t *__restrict__ ky,
                                                const int8_t *__restrict__ kz, double *fx, double *fy, double *fz,
@@ -909,27 +1008,40 @@ __global__ void __launch_bounds__(BLK) k_force(const RepC *__restrict__ reps, Ch
     double Fx = 0.0, Fy = 0.0, Fz = 0.0;
     const bool valid = i < R.n;
     const int g = R.off + (valid ? i : 0);
+    // FP64: the wrapped box fractions; FP32: their fixed-point copy
+    using PosT = typename std::conditional<FP32, uint4, double4>::type;
+    const PosT *__restrict__ pos = (const PosT *)(FP32 ? (const void *)posq : (const void *)pos4);
     if (valid) {
-        const double4 pi = pos4[g];
+        const PosT pi = pos[g];
         const int ti = (int)pi.w;
-        PairAcc<FP32> acc;
+        PairAcc<FP32> acc(ti);
         const int m = nn[g];
         // entry t + c*tpp of particle i: chunk c of the replica's region
         const int *lp = nl + nl_at(R, cap, t, i);
         const size_t cstep = (size_t)R.n << R.ltpp;
         int k = t;
-        for (; k + 7 * tpp < m; k += 8 * tpp, lp += 8 * cstep) {
-            int jj[8];
-            double4 q[8];
+        // groups of 8 entries; the indices of the next group are loaded
+        // while the positions of the current one are in flight
+        int jn[8];
+        if (k + 7 * tpp < m) {
 #pragma unroll
-            for (int u = 0; u < 8; u++) jj[u] = lp[u * cstep];
-#pragma unroll
-            for (int u = 0; u < 8; u++) q[u] = pos4[jj[u]];
-#pragma unroll
-            for (int u = 0; u < 8; u++) acc.add(pi.x, pi.y, pi.z, ti, q[u], R.L);
+            for (int u = 0; u < 8; u++) jn[u] = lp[u * cstep];
         }
-        for (; k < m; k += tpp, lp += cstep) acc.add(pi.x, pi.y, pi.z, ti, pos4[*lp], R.L);
-        acc.result(Fx, Fy, Fz);
+        while (k + 7 * tpp < m) {
+            PosT q[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) q[u] = pos[jn[u]];
+            k += 8 * tpp;
+            lp += 8 * cstep;
+            if (k + 7 * tpp < m) {
+#pragma unroll
+                for (int u = 0; u < 8; u++) jn[u] = lp[u * cstep];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u++) acc.add(pi, ti, q[u], R);
+        }
+        for (; k < m; k += tpp, lp += cstep) acc.add(pi, ti, pos[*lp], R);
+        acc.result(Fx, Fy, Fz, R);
         v_pe = acc.pe;
         v_vir = acc.vir;
     }
@@ -1040,13 +1152,14 @@ struct ac4_md {
     ChainS *chain = nullptr;
     int *blk_rep = nullptr, *blk_first = nullptr, *blk_idx = nullptr, *fblk_rep = nullptr, *fblk_first = nullptr;
     int *cblk_rep = nullptr, *cblk_cell = nullptr;
-    int fnblk = 0, lnblk = 0;
-    int *lblk_rep = nullptr, *lblk_first = nullptr;  // list-build block table (NLT lanes per particle)
+    int fnblk = 0, lnblk = 0, ncb = 0;
+    int *wl_item = nullptr, *wl_n = nullptr;  // rebuild work list (WorkList)
+    int *lblk_rep = nullptr, *lblk_first = nullptr;  // list-build block table (LPB particles per block)
     double *partial = nullptr;
     PartArrays P{}, A{};  // primary and permutation scratch
     double *fx = nullptr, *fy = nullptr, *fz = nullptr, *xo = nullptr, *yo = nullptr, *zo = nullptr;
     double4 *pos4 = nullptr;
-    float4 *pos4f = nullptr;
+    uint4 *posq = nullptr;
     int *rank = nullptr, *nn = nullptr, *nn_ref = nullptr, *nl = nullptr, *cnt = nullptr, *start = nullptr,
         *members = nullptr;
     int8_t *kx = nullptr, *ky = nullptr, *kz = nullptr;
@@ -1099,14 +1212,15 @@ static void free_parts(PartArrays &a) {
 // fill the device and large ones keep one thread per particle).  The team
 // width fixes the order of each particle's force sum, so results are bitwise
 // reproducible for a given handle shape.  nlist_mode: dynamics list builder,
-// 0 = one wave per cell, 1 = one thread per particle, 2 = NLT-lane teams.
+// 0 = one wave per cell, 1 = one thread per particle, 2/3/4 = teams of
+// 16/32/64 lanes per particle.
 #define TPP_LANES 65536.0
-static int g_tpp = 0, g_nlist_mode = 2;
+static int g_tpp = 0, g_nlist_mode = 4;
 
 extern "C" int ac4_set_tuning(int tpp, int nlist_mode) {
     AC4_REQUIRE((tpp == 0 || tpp == 1 || tpp == 2 || tpp == 4 || tpp == 8 || tpp == 16) &&
-                    (nlist_mode >= 0 && nlist_mode <= 2),
-                AC4_EARG, "ac4_set_tuning: tpp in {0,1,2,4,8,16}, nlist_mode in {0,1,2}");
+                    (nlist_mode >= 0 && nlist_mode <= 4),
+                AC4_EARG, "ac4_set_tuning: tpp in {0,1,2,4,8,16}, nlist_mode in {0,...,4}");
     g_tpp = tpp;
     g_nlist_mode = nlist_mode;
     return AC4_OK;
@@ -1133,8 +1247,8 @@ static const char *configure_rep(RepC &R, const ac4_replica_params &q) {
     R.nc = (int)(R.L / K_CELL_CUT);  // list grid: cells at least 2.85 wide
     R.ncell = R.nc * R.nc * R.nc;
     if (R.ncell > R.cap_cells) return "density below the slot's cell capacity (0.5)";
-    R.Lf = (float)R.L;
-    R.Lhf = (float)(0.5 * R.L);
+    R.qscale = R.L / 4294967296.0;
+    R.qs2 = (float)(R.qscale * R.qscale);
     R.tau = q.tau_per;
     R.T = q.T;
     R.f0 = q.f0;
@@ -1166,6 +1280,8 @@ extern "C" int ac4_md_create_slots(int nslots, const int *cap_n, int precision, 
     md->nlist_mode = g_nlist_mode;
     int off = 0, off_act = 0, off_cell = 0, blk_off = 0, fblk_off = 0;
     std::vector<int> brep, bfirst, bidx, frep, ffirst, crep, ccell, lrep, lfirst;
+    int ncb = 0;
+    AC4_REQUIRE(nslots < 32768, AC4_EARG, "at most 32767 slots");
     double cap_tot = 0.0;
     for (int r = 0; r < nslots; r++) cap_tot += cap_n[r] > 0 ? cap_n[r] : 0;
     int tpp = g_tpp, ltpp = 0;
@@ -1195,7 +1311,11 @@ extern "C" int ac4_md_create_slots(int nslots, const int *cap_n, int precision, 
         R.fnblk = (cn + ppb - 1) / ppb;
         R.fblk_off = fblk_off;
         for (int k = 0; k < R.fnblk; k++) { frep.push_back(r); ffirst.push_back(k * ppb); }
-        for (int k = 0; k < (cn + BLK / NLT - 1) / (BLK / NLT); k++) { lrep.push_back(r); lfirst.push_back(k * (BLK / NLT)); }
+        R.lblk_off = (int)lrep.size();
+        R.lnblk = (cn + LPB - 1) / LPB;
+        for (int k = 0; k < R.lnblk; k++) { lrep.push_back(r); lfirst.push_back(k * LPB); }
+        if ((cap_cells + BLK / 64 - 1) / (BLK / 64) >= 65536) { delete md; ac4_set_error("slot %d: too many cells", r); return AC4_EARG; }
+        ncb += (cap_cells + BLK / 64 - 1) / (BLK / 64);
         for (int c = 0; c < cap_cells; c++) { crep.push_back(r); ccell.push_back(c); }
         off += cn;
         off_act += cn;  // room for n_act up to n
@@ -1212,11 +1332,13 @@ extern "C" int ac4_md_create_slots(int nslots, const int *cap_n, int precision, 
     md->stride = off;
     md->fnblk = fblk_off;
     md->lnblk = (int)lrep.size();
+    md->ncb = ncb;
     int rc = AC4_OK;
 #define ALLOC(ptr, n) if ((rc = dalloc(&md->ptr, (size_t)(n))) != AC4_OK) { ac4_md_destroy(md); return rc; }
     ALLOC(reps, nslots) ALLOC(chain, nslots) ALLOC(blk_rep, md->nblk) ALLOC(blk_first, md->nblk) ALLOC(blk_idx, md->nblk)
-    ALLOC(fblk_rep, md->fnblk) ALLOC(fblk_first, md->fnblk) ALLOC(lblk_rep, md->lnblk) ALLOC(lblk_first, md->lnblk) ALLOC(cblk_rep, off_cell) ALLOC(cblk_cell, off_cell) ALLOC(partial, 4 * (size_t)md->fnblk) ALLOC(pos4f, off)
-    ALLOC(fx, off) ALLOC(fy, off) ALLOC(fz, off) ALLOC(xo, off) ALLOC(yo, off) ALLOC(zo, off) ALLOC(pos4, off)
+    ALLOC(fblk_rep, md->fnblk) ALLOC(fblk_first, md->fnblk) ALLOC(lblk_rep, md->lnblk) ALLOC(lblk_first, md->lnblk)
+    ALLOC(wl_item, (size_t)md->nblk + md->lnblk + md->ncb) ALLOC(wl_n, 4) ALLOC(cblk_rep, off_cell) ALLOC(cblk_cell, off_cell) ALLOC(partial, 4 * (size_t)md->fnblk)
+    ALLOC(fx, off) ALLOC(fy, off) ALLOC(fz, off) ALLOC(xo, off) ALLOC(yo, off) ALLOC(zo, off) ALLOC(pos4, off) ALLOC(posq, off)
     ALLOC(rank, off) ALLOC(nn, off) ALLOC(nn_ref, off) ALLOC(nl, (size_t)md->cap * off) ALLOC(cnt, off_cell) ALLOC(start, off_cell)
     ALLOC(members, off) ALLOC(kx, off_act) ALLOC(ky, off_act) ALLOC(kz, off_act)
 #undef ALLOC
@@ -1302,10 +1424,10 @@ extern "C" int ac4_md_destroy(ac4_md *md) {
     if (md->gexec) (void)hipGraphExecDestroy(md->gexec);
     if (md->graph) (void)hipGraphDestroy(md->graph);
     void *ptrs[] = {md->reps, md->chain, md->blk_rep, md->blk_first, md->blk_idx, md->fblk_rep, md->fblk_first,
-                    md->lblk_rep, md->lblk_first,
-                    md->pos4f, md->cblk_rep, md->cblk_cell, md->ox, md->oy, md->oz, md->ov_q, md->ov_q2,
+                    md->lblk_rep, md->lblk_first, md->wl_item, md->wl_n,
+                    md->cblk_rep, md->cblk_cell, md->ox, md->oy, md->oz, md->ov_q, md->ov_q2,
                     md->ov_step, md->ov_cnt, md->ov_n,
-                    md->partial, md->fx, md->fy, md->fz, md->xo, md->yo, md->zo, md->pos4, md->rank, md->nn,
+                    md->partial, md->fx, md->fy, md->fz, md->xo, md->yo, md->zo, md->pos4, md->posq, md->rank, md->nn,
                     md->nn_ref, md->nl,
                     md->cnt, md->start, md->members, md->kx, md->ky, md->kz, md->ex, md->ey, md->ez, md->et};
     for (void *q : ptrs)
@@ -1409,6 +1531,7 @@ static int set_state_impl(ac4_md *md, int r, const double *x, const double *y, c
     const RepC &R = md->reps_h[r];
     const int n = R.n, g = R.off;
     std::vector<double4> p4(n);
+    std::vector<uint4> q4(n);
     std::vector<int> ids(n);
     double keblk = 0.0;  // time_evolve.c:586-589 order
     for (int i = 0; i < n; i++) {
@@ -1416,6 +1539,7 @@ static int set_state_impl(ac4_md *md, int r, const double *x, const double *y, c
         AC4_REQUIRE(x[i] >= 0.0 && x[i] < 1.0 && y[i] >= 0.0 && y[i] < 1.0 && z[i] >= 0.0 && z[i] < 1.0, AC4_EARG,
                     "set_state: wrapped coordinate of particle %d outside [0,1)", i);
         p4[i] = make_double4(x[i], y[i], z[i], (double)type[i]);
+        q4[i] = qpos(x[i], y[i], z[i], type[i]);
         keblk += px[i] * px[i] + py[i] * py[i] + pz[i] * pz[i];
         ids[i] = i;
     }
@@ -1428,6 +1552,7 @@ static int set_state_impl(ac4_md *md, int r, const double *x, const double *y, c
     TRY(h2d(md->P.type + g, type, n, s));
     TRY(h2d(md->P.id + g, ids.data(), n, s));
     TRY(h2d(md->pos4 + g, p4.data(), n, s));
+    TRY(h2d(md->posq + g, q4.data(), n, s));
     std::vector<int> at;
     if (act_tag) {
         for (int i = 0; i < n; i++)
@@ -1550,34 +1675,41 @@ extern "C" int ac4_md_set_rebuild_criterion(ac4_md *md, int criterion) {
 
 // Cell counts are cleared by k_first_half inside a step; outside a step
 // (update_cell_list) with_zero clears them first.
+// fixed grid of the work-list kernels: enough workgroups to fill the device
+#define WL_GRID 1024
 static int launch_rebuild(ac4_md *md, bool with_zero = false) {
     hipStream_t s = md->stream;
-    const int cy = (md->max_ncell + BLK - 1) / BLK, sy = (md->max_ncell + BLK / 64 - 1) / (BLK / 64);
-    if (with_zero) hipLaunchKernelGGL(k_cell_zero, dim3(md->R, cy), dim3(BLK), 0, s, md->reps, md->chain, md->cnt);
+    const int cy = (md->max_ncell + BLK - 1) / BLK;
+    const WorkList wl{md->wl_item, md->wl_n, md->nblk, md->nblk + md->lnblk};
+    const int gb = std::min(md->nblk, WL_GRID), gl = std::min(md->lnblk, WL_GRID), gc = std::min(md->ncb, WL_GRID);
+    if (with_zero)
+        hipLaunchKernelGGL(k_cell_zero, dim3(md->R, cy), dim3(BLK), 0, s, md->reps, md->chain, md->cnt, md->wl_n);
     hipLaunchKernelGGL(k_bin, dim3(md->nblk), dim3(BLK), 0, s, md->reps, md->chain, md->blk_rep, md->blk_first,
-                       md->P.x, md->P.y, md->P.z, md->cnt, md->P.cell_of, md->rank, md->P.cell_ref, md->start);
-    hipLaunchKernelGGL(k_scatter, dim3(md->nblk), dim3(BLK), 0, s, md->reps, md->chain, md->blk_rep, md->blk_first,
+                       md->P.x, md->P.y, md->P.z, md->cnt, md->P.cell_of, md->rank, md->P.cell_ref, md->start, wl);
+    hipLaunchKernelGGL(k_scatter, dim3(gb), dim3(BLK), 0, s, md->reps, md->blk_rep, md->blk_first, wl,
                        md->P.cell_of, md->rank, md->start, md->members);
-    hipLaunchKernelGGL(k_cell_sort, dim3(md->R, sy), dim3(BLK), 0, s, md->reps, md->chain, md->cnt, md->start,
-                       md->P.id, md->members);
-    hipLaunchKernelGGL(k_gather, dim3(md->nblk), dim3(BLK), 0, s, md->reps, md->chain, md->blk_rep, md->blk_first,
-                       md->members, md->P, md->A);
-    hipLaunchKernelGGL(k_commit, dim3(md->nblk), dim3(BLK), 0, s, md->reps, md->chain, md->blk_rep, md->blk_first,
-                       md->A, md->P, md->pos4, md->pos4f, md->xo, md->yo, md->zo);
+    hipLaunchKernelGGL(k_cell_sort, dim3(gc), dim3(BLK), 0, s, md->reps, wl, md->cnt, md->start, md->P.id,
+                       md->members);
+    hipLaunchKernelGGL(k_gather, dim3(gb), dim3(BLK), 0, s, md->reps, md->blk_rep, md->blk_first, wl, md->members,
+                       md->P, md->A);
+    hipLaunchKernelGGL(k_commit, dim3(gb), dim3(BLK), 0, s, md->reps, md->blk_rep, md->blk_first, wl, md->A, md->P,
+                       md->pos4, md->posq, md->xo, md->yo, md->zo);
     if (with_zero)  // update_cell_list: the reference's FP64 test and its 2.85 count
         hipLaunchKernelGGL(k_nlist<true>, dim3(md->nblk), dim3(BLK), 0, s, md->reps, md->chain, md->blk_rep,
-                           md->blk_first, md->pos4, md->pos4f, md->P.cell_of, md->cnt, md->start, md->nn, md->nn_ref,
+                           md->blk_first, md->pos4, md->posq, md->P.cell_of, md->cnt, md->start, md->nn, md->nn_ref,
                            md->nl, md->stride, md->cap);
-    else if (md->nlist_mode == 2)
-        hipLaunchKernelGGL(k_nlist_team, dim3(md->lnblk), dim3(BLK), 0, s, md->reps, md->chain,
-                           md->lblk_rep, md->lblk_first, md->pos4f, md->P.cell_of, md->cnt, md->start, md->nn,
-                           md->nl, md->cap);
+    else if (md->nlist_mode >= 2) {
+        const int w = md->nlist_mode == 2 ? 16 : (md->nlist_mode == 3 ? 32 : 64);
+        auto kern = w == 16 ? k_nlist_team<16> : (w == 32 ? k_nlist_team<32> : k_nlist_team<64>);
+        hipLaunchKernelGGL(kern, dim3(gl), dim3(w * LPB), 0, s, md->reps, md->chain, md->lblk_rep, md->lblk_first, wl,
+                           md->posq, md->P.cell_of, md->cnt, md->start, md->nn, md->nl, md->cap);
+    }
     else if (md->nlist_mode == 0)
         hipLaunchKernelGGL(k_nlist_cell, dim3(md->ncell_tot), dim3(64), 0, s, md->reps, md->chain, md->cblk_rep,
-                           md->cblk_cell, md->pos4f, md->cnt, md->start, md->nn, md->nl, md->cap);
+                           md->cblk_cell, md->posq, md->cnt, md->start, md->nn, md->nl, md->cap);
     else
         hipLaunchKernelGGL(k_nlist<false>, dim3(md->nblk), dim3(BLK), 0, s, md->reps, md->chain, md->blk_rep,
-                           md->blk_first, md->pos4, md->pos4f, md->P.cell_of, md->cnt, md->start, md->nn, md->nn_ref,
+                           md->blk_first, md->pos4, md->posq, md->P.cell_of, md->cnt, md->start, md->nn, md->nn_ref,
                            md->nl, md->stride, md->cap);
     AC4_HIP(hipGetLastError());
     return AC4_OK;
@@ -1588,12 +1720,12 @@ static int launch_force(ac4_md *md) {
     hipStream_t s = md->stream;
     if (md->prec == AC4_FP32)
         hipLaunchKernelGGL((k_force<true, KICK>), dim3(xcd_grid(md->fnblk)), dim3(BLK), 0, s, md->reps, md->chain,
-                           md->fblk_rep, md->fblk_first, md->pos4, md->nn, md->nl, md->cap, md->P.act, md->kx,
+                           md->fblk_rep, md->fblk_first, md->pos4, md->posq, md->nn, md->nl, md->cap, md->P.act, md->kx,
                            md->ky, md->kz, md->fx, md->fy, md->fz, md->P.px, md->P.py, md->P.pz, md->partial,
                            md->fnblk);
     else
         hipLaunchKernelGGL((k_force<false, KICK>), dim3(xcd_grid(md->fnblk)), dim3(BLK), 0, s, md->reps, md->chain,
-                           md->fblk_rep, md->fblk_first, md->pos4, md->nn, md->nl, md->cap, md->P.act, md->kx,
+                           md->fblk_rep, md->fblk_first, md->pos4, md->posq, md->nn, md->nl, md->cap, md->P.act, md->kx,
                            md->ky, md->kz, md->fx, md->fy, md->fz, md->P.px, md->P.py, md->P.pz, md->partial,
                            md->fnblk);
     AC4_HIP(hipGetLastError());
@@ -1606,8 +1738,8 @@ static void launch_shuffle(ac4_md *md, int force, hipStream_t s) {
 static void launch_first_half(ac4_md *md) {
     hipLaunchKernelGGL(k_first_half, dim3(md->nblk), dim3(BLK), 0, md->stream, md->reps, md->chain, md->blk_rep,
                        md->blk_first, md->P.x, md->P.y, md->P.z, md->P.xu, md->P.yu, md->P.zu, md->P.px, md->P.py,
-                       md->P.pz, md->fx, md->fy, md->fz, md->xo, md->yo, md->zo, md->P.type, md->pos4, md->chain,
-                       md->blk_idx, md->cnt);
+                       md->P.pz, md->fx, md->fy, md->fz, md->xo, md->yo, md->zo, md->P.type, md->pos4, md->posq,
+                       md->prec == AC4_FP32 ? 1 : 0, md->chain, md->blk_idx, md->cnt, md->wl_n);
 }
 
 // One MD step.  With fork (graph capture), the shuffle, which only the force

@@ -5,7 +5,7 @@
  * state, replays `steps` graph steps, and the fastest shape is kept for the
  * handles created afterwards (ac4_set_tuning).  Candidates: force-kernel lanes
  * per particle tpp in {1, 2, 4, 8, 16} (so 256/tpp particles per workgroup)
- * crossed with the thread-per-particle and 16-lane-team list builders.
+ * crossed with the thread-per-particle and 16/32/64-lane-team list builders.
  */
 #include <stdint.h>
 #include <stdlib.h>
@@ -52,10 +52,10 @@ int ac4_autotune(int n_rep, const ac4_replica_params *p, int precision, int rela
         ac4_set_error("ac4_autotune: bad arguments");
         return AC4_EARG;
     }
-    static const int tpps[5] = {1, 2, 4, 8, 16}, modes[2] = {1, 2};
+    static const int tpps[5] = {1, 2, 4, 8, 16}, modes[4] = {1, 2, 3, 4};
     rep_state *st = calloc((size_t)n_rep, sizeof *st);
     ac4_md *md = NULL;
-    int rc = ac4_set_tuning(0, 2);
+    int rc = ac4_set_tuning(0, 4);
     if (rc == AC4_OK) rc = ac4_md_create(n_rep, p, precision, &md);
     long long nsum = 0;
     for (int r = 0; r < n_rep && rc == AC4_OK; r++) {
@@ -85,13 +85,13 @@ int ac4_autotune(int n_rep, const ac4_replica_params *p, int precision, int rela
     md = NULL;
     if (rc != AC4_OK) {
         free_states(st, n_rep);
-        ac4_set_tuning(0, 2);
+        ac4_set_tuning(0, 4);
         return rc;
     }
     double best = -1.0;
-    int bt = 0, bm = 2, nc = 0;
+    int bt = 0, bm = 4, nc = 0;
     for (int a = 0; a < 5; a++)
-        for (int b = 0; b < 2; b++) {
+        for (int b = 0; b < 4; b++) {
             double rate = 0.0;
             int ok = ac4_set_tuning(tpps[a], modes[b]) == AC4_OK && ac4_md_create(n_rep, p, precision, &md) == AC4_OK;
             if (ok) ok = load_all(md, n_rep, st) == AC4_OK && ac4_md_step_graph(md, 20) == AC4_OK &&
@@ -115,7 +115,7 @@ int ac4_autotune(int n_rep, const ac4_replica_params *p, int precision, int rela
     free_states(st, n_rep);
     if (n_cand) *n_cand = nc;
     if (best <= 0.0) {
-        ac4_set_tuning(0, 2);
+        ac4_set_tuning(0, 4);
         ac4_set_error("ac4_autotune: no candidate ran");
         return AC4_EHIP;
     }

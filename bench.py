@@ -88,7 +88,14 @@ def chi4_report(md, ps, sample_every):
     times N), averaged over replicas; and for every (f0, tau_p) point of the
     grid pooled over its replicas and their origins (ensemble fluctuation)."""
     acc = [md.overlap(r) for r in range(len(ps))]
-    lags = np.arange(1, len(acc[0][0])) * sample_every * 0.005
+    # lags >= 1 that every replica has reached from at least one origin
+    nl = 1
+    while nl < len(acc[0][2]) and all(a[2][nl] > 0 for a in acc):
+        nl += 1
+    acc = [tuple(v[:nl] for v in a) for a in acc]
+    if nl < 3:
+        return None
+    lags = np.arange(1, nl) * sample_every * 0.005
     per = []
     for r, (q, q2, c) in enumerate(acc):
         mq, mq2 = q[1:] / c[1:], q2[1:] / c[1:]
@@ -117,7 +124,7 @@ def chi4_report(md, ps, sample_every):
 
 def plain_sequential(ac4, ps, relax, steps):
     """The same replicas one after another: FP64, eager launches."""
-    ac4.set_tuning(0, 2)
+    ac4.set_tuning(0, 4)
     tot_ps, tot_t = 0.0, 0.0
     for p in ps:
         md = start(ac4, [p], ac4.FP64, relax)
@@ -133,7 +140,7 @@ def plain_sequential(ac4, ps, relax, steps):
 
 
 def plain_rate(ac4, n, T, f0, tau, seed, relax=150, steps=200):
-    ac4.set_tuning(0, 2)
+    ac4.set_tuning(0, 4)
     md = start(ac4, [ac4.ReplicaParams(n, 1.2, T, f0, -1, tau, seed)], ac4.FP64, relax)
     md.step(30)
     md.sync()
@@ -160,7 +167,7 @@ def serving(ac4, njobs, rate_per_s, seed=4242):
     # share of jobs it takes under the N distribution (about 1.2 KB per
     # particle of capacity incl. the 192-entry neighbour list)
     caps = [1024] * 8 + [2048] * 10 + [4096] * 10 + [8192] * 8 + [20480] * 4
-    ac4.set_tuning(0, 2)
+    ac4.set_tuning(0, 4)
     srv = ac4.Server(caps, precision=ac4.FP32, chunk=100, relax=300)
     ids = [srv.submit(j["n"], j["T"], j["f0"], j["tau_p"], j["steps"], j["seed"], j["t"]) for j in jobs]
     t0 = time.perf_counter()
@@ -206,7 +213,7 @@ def cpu_baseline(ac4, n=2485, steps=1500):
     import oracle_lib as OL
     lib = OL.load()
     p = ac4.ReplicaParams(n, 1.2, 0.5, 1.0, -1, 0, 99)
-    ac4.set_tuning(0, 2)
+    ac4.set_tuning(0, 4)
     md = start(ac4, [p], ac4.FP64, 300)
     g = md.get_state(0)
     chain = md.get_chain(0)
@@ -244,6 +251,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=2.0, help="job arrivals per second")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only-batched", action="store_true",
+                    help="batched ensemble only (no plain, serving or CPU legs), e.g. under a profiler")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
@@ -290,6 +299,10 @@ def main():
     rebuilds = sum(md.rebuilds(r) for r in range(a.replicas))
     md.close()
 
+    if a.only_batched:
+        print(json.dumps({"metric": "particle-steps/s (batched only)", "value": value, "unit": "particle-steps/s",
+                          "steps": a.steps, "kernel_ms_per_step": kt}), flush=True)
+        return
     plain_value, plain_ms = plain_sequential(ac4, ps, a.relax, a.plain_steps)
     log("plain sequential %.3e particle-steps/s" % plain_value)
     serve = None

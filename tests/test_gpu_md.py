@@ -24,13 +24,13 @@ def make_md(ac4, states, f0=1.5, T=0.6, tau=0, prec=0, seeds=None):
 
 
 @contextlib.contextmanager
-def tuning(ac4, tpp, nlist_mode=2):
+def tuning(ac4, tpp, nlist_mode=4):
     """Kernel shape for handles created inside the block (default restored)."""
     ac4.set_tuning(tpp, nlist_mode)
     try:
         yield
     finally:
-        ac4.set_tuning(0, 2)
+        ac4.set_tuning(0, 4)
 
 
 _RELAXED = {}
@@ -184,11 +184,12 @@ def test_replica_alone_equals_in_batch(gpu, prec, tpp):
 
 @pytest.mark.parametrize("prec", [0, 1])
 def test_list_builders_give_identical_trajectories(gpu, prec):
-    """Wave-per-cell, thread-per-particle and team list builders write the
-    same neighbours in the same order, so the dynamics are bitwise equal."""
+    """Wave-per-cell, thread-per-particle and 16/32/64-lane team list
+    builders write the same neighbours in the same order, so the dynamics are
+    bitwise equal."""
     spec = [(1300, 61, 4), (2400, 62, 5)]
     out = []
-    for mode in (0, 1, 2):
+    for mode in (0, 1, 2, 3, 4):
         with tuning(gpu, 0, mode):
             md = make_relaxed_md(gpu, spec, tau=30, prec=prec)
         md.step(200, graph=True)
