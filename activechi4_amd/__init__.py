@@ -90,6 +90,7 @@ _SIGS = {
     "ac4_origin_gap": (ctypes.c_int, [_I, ctypes.c_int]),
     "ac4_chi4_ensemble": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _D, _I, _D]),
     "ac4_set_tuning": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
+    "ac4_md_set_nve": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int]),
     "ac4_chain_init": (None, [ctypes.c_int, ctypes.c_double, ctypes.c_uint64, _D]),
     "ac4_autotune": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ReplicaParams), ctypes.c_int, ctypes.c_int,
                                     ctypes.c_int, _I, _D, ctypes.c_int, _I]),
@@ -253,6 +254,10 @@ class MD:
         through ac4_chain_init.  The draw depends on `seed` only (default: 7919*r)."""
         p = self.params[r]
         self.set_chain(r, chain_init(p.n, p.T, 7919 * r if seed is None else seed))
+
+    def set_nve(self, r, on=True):
+        """Plain velocity Verlet for replica r (ac4_md_set_nve); Heff = KE + PE."""
+        _check(lib.ac4_md_set_nve(self.h, r, 1 if on else 0))
 
     def set_rebuild_criterion(self, crit):
         _check(lib.ac4_md_set_rebuild_criterion(self.h, crit))

@@ -76,7 +76,7 @@ struct ChainS {
     long long step, shuffles;
     int rebuilds, force_rebuild, err, crit;
     unsigned int done_blocks, bin_done;  // last-block counters of k_force and k_bin
-    int mode;                 // 0: Nose-Hoover chain, 1: velocity-rescale ramp
+    int mode;                 // 0: Nose-Hoover chain, 1: velocity-rescale ramp, 2: NVE
     int ramp_n, ramp_k;
     double ramp0, ramp1;      // target T = T*(ramp0 + (ramp1-ramp0)*k/(n-1))
 };
@@ -175,7 +175,7 @@ __global__ void __launch_bounds__(64) k_shuffle(const RepC *__restrict__ reps, C
     const int r = blockIdx.x;
     const RepC R = reps[r];
     ChainS &c = chain[r];
-    if (!force && (c.mode != 0 || R.tau <= 0 || (c.step % R.tau) != 0)) return;
+    if (!force && (c.mode == 1 || R.tau <= 0 || (c.step % R.tau) != 0)) return;
     const uint64_t sh = (uint64_t)c.shuffles;
     const bool lds = R.nact <= SHUF_LDS;
     if (lds) {
@@ -1107,7 +1107,22 @@ __global__ void __launch_bounds__(BLK) k_force(const RepC *__restrict__ reps, Ch
     c.done_blocks = 0u;
     if (!KICK) return;
     const bool rebuilt = need_rebuild(R, c);
-    if (c.mode != 0) {
+    if (c.mode == 2) {
+        // NVE: plain velocity Verlet (AdvanceTimeNVE, MDcode/time_evolve.c:39-171,
+        // same U2-U1-F-U2 order as the thermostatted step), Heff = KE + PE
+        c.p_scale = 1.0;
+        c.S_end = S;
+        c.ke = S * 0.5;
+        c.inst_T = 2.0 * c.ke / R.dof;
+        c.drift = spx;
+        c.heff = c.ke + pe;
+        if (rebuilt) c.rebuilds += 1;
+        c.maxbits = 0ull;
+        c.force_rebuild = 0;
+        c.step += 1;
+        return;
+    }
+    if (c.mode == 1) {
         // coldstart/breakxal of MDcode/initialise.c:166-201: momenta rescaled
         // to a ramped target temperature every step, no thermostat chain
         const double f = c.ramp_n > 1 ? c.ramp0 + (c.ramp1 - c.ramp0) * (double)c.ramp_k / (double)(c.ramp_n - 1)
@@ -1592,6 +1607,17 @@ static int set_state_impl(ac4_md *md, int r, const double *x, const double *y, c
         md->have_forces = false;
     }
     return AC4_OK;
+}
+
+// NVE (on != 0) or back to the Nose-Hoover chain (on == 0) for replica r; the
+// chain variables are kept and resume where they were.
+extern "C" int ac4_md_set_nve(ac4_md *md, int r, int on) {
+    TRY(check_rep(md, r));
+    ChainS c;
+    TRY(get_chain(md, r, &c));
+    AC4_REQUIRE(c.mode != 1, AC4_ESTATE, "set_nve: replica %d is in a relaxation ramp", r);
+    c.mode = on ? 2 : 0;
+    return put_chain(md, r, &c);
 }
 
 // Velocity-rescale ramp for one slot only, applied by its next nsteps steps.

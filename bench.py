@@ -229,6 +229,30 @@ def cpu_baseline(ac4, n=2485, steps=1500):
                       % (n, steps, dt)}
 
 
+def pmc_probe(ac4):
+    """For hardware-counter runs (rocprofv3 --pmc): the batched workload with
+    eager launches (every dispatch is counted), then three device-to-device
+    copies of 256 MiB that calibrate FETCH_SIZE/WRITE_SIZE against a known
+    byte count."""
+    import ctypes
+    ps = ensemble_params(ac4, 64)
+    md = start(ac4, ps, ac4.FP32, 100)
+    md.step(40)
+    md.sync()
+    md.close()
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = ctypes.c_size_t(256 << 20)
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), n) == 0 and hip.hipMalloc(ctypes.byref(b), n) == 0
+    assert hip.hipMemset(a, 1, n) == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(b, a, n, 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log("pmc probe done")
+
+
 def pmc_traffic():
     """HBM bytes per k_force launch from the committed rocprofv3 PMC summary."""
     try:
@@ -251,6 +275,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=2.0, help="job arrivals per second")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="short eager run + 256 MiB copies, for rocprofv3 --pmc (prints no bench line)")
     ap.add_argument("--only-batched", action="store_true",
                     help="batched ensemble only (no plain, serving or CPU legs), e.g. under a profiler")
     a = ap.parse_args()
@@ -260,6 +286,9 @@ def main():
         sys.exit(2)
     import activechi4_amd as ac4
     ac4.require_gpu()
+    if a.pmc_probe:
+        pmc_probe(ac4)
+        return
 
     ps = ensemble_params(ac4, a.replicas)
     ntot = sum(p.n for p in ps)

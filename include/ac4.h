@@ -110,6 +110,11 @@ int ac4_md_set_rebuild_criterion(ac4_md *md, int criterion);
  * chain, no direction shuffles, step counter unchanged.  Used to bring a
  * lattice start to temperature before Nose-Hoover dynamics. */
 int ac4_md_relax(ac4_md *md, int nsteps, double f_start, double f_end);
+/* NVE for replica r (on != 0): plain velocity Verlet, the reference's
+ * AdvanceTimeNVE (MDcode/time_evolve.c:39-171) in the step's U2-U1-F-U2 order,
+ * for energy-drift checks; Heff reported as KE + PE.  on == 0 returns to the
+ * Nose-Hoover chain with its variables as they were. */
+int ac4_md_set_nve(ac4_md *md, int r, int on);
 /* ShuffleActDirec() (MDcode/active.c:45-85) now, for every replica, with the
  * device Philox stream; shuffle counter advances by one. */
 int ac4_md_shuffle(ac4_md *md);
