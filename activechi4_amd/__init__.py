@@ -90,6 +90,17 @@ _SIGS = {
     "ac4_origin_gap": (ctypes.c_int, [_I, ctypes.c_int]),
     "ac4_chi4_ensemble": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _D, _I, _D]),
     "ac4_set_tuning": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
+    "ac4_frames_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.POINTER(_VP)]),
+    "ac4_frames_destroy": (ctypes.c_int, [_VP]),
+    "ac4_frames_push": (ctypes.c_int, [_VP, _D, _D, _D, ctypes.c_longlong]),
+    "ac4_frames_push_md": (ctypes.c_int, [_VP, _VP, ctypes.c_int, ctypes.c_longlong]),
+    "ac4_frames_count": (ctypes.c_int, [_VP]),
+    "ac4_frames_find": (ctypes.c_int, [_VP, ctypes.c_longlong]),
+    "ac4_frames_overlap": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_double, _I]),
+    "ac4_shell_directions": (ctypes.c_int, [ctypes.c_int, ctypes.c_uint64, _D]),
+    "ac4_frames_s4_shells": (ctypes.c_int, [_VP, ctypes.c_int, _I, _I, ctypes.c_double, ctypes.c_double,
+                                            ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, _D, _D,
+                                            _D]),
     "ac4_md_set_nve": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int]),
     "ac4_chain_init": (None, [ctypes.c_int, ctypes.c_double, ctypes.c_uint64, _D]),
     "ac4_autotune": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ReplicaParams), ctypes.c_int, ctypes.c_int,
@@ -445,6 +456,64 @@ class Server:
 
     def particle_steps(self):
         return lib.ac4_server_particle_steps(self.h)
+
+
+class Frames:
+    """Device-resident trajectory of one replica (ac4_frames_* of include/ac4.h)."""
+
+    def __init__(self, n, L, cap):
+        load()
+        h = _VP()
+        _check(lib.ac4_frames_create(n, L, cap, ctypes.byref(h)))
+        self.h = h
+        self.n = n
+
+    def close(self):
+        if self.h:
+            lib.ac4_frames_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def push(self, xu, yu, zu, step):
+        a = [_f64(v) for v in (xu, yu, zu)]
+        _check(lib.ac4_frames_push(self.h, *[_dp(v) for v in a], int(step)))
+
+    def push_md(self, md, r, step):
+        _check(lib.ac4_frames_push_md(self.h, md.h, r, int(step)))
+
+    def count(self):
+        return lib.ac4_frames_count(self.h)
+
+    def find(self, step):
+        return lib.ac4_frames_find(self.h, int(step))
+
+    def overlap(self, i0, i1, a=0.3):
+        c = ctypes.c_int()
+        _check(lib.ac4_frames_overlap(self.h, i0, i1, a, ctypes.byref(c)))
+        return c.value
+
+    def s4_shells(self, orig, lagf, q0, dq, nshell, ndir, seed, a=0.3, phases=False):
+        """Returns (s4_ref, s4_dir[, phases[npairs][ndir][nshell][2]])."""
+        o, lf = _i32(orig), _i32(lagf)
+        s4r, s4d = np.empty(nshell), np.empty(nshell)
+        ph = np.empty((len(o), ndir, nshell, 2)) if phases else None
+        _check(lib.ac4_frames_s4_shells(self.h, len(o), _ip(o), _ip(lf), a, q0, dq, nshell, ndir,
+                                        int(seed) & (2**64 - 1), _dp(s4r), _dp(s4d),
+                                        _dp(ph) if phases else None))
+        return (s4r, s4d, ph) if phases else (s4r, s4d)
+
+
+def shell_directions(ndir, seed):
+    """ndir unit vectors from seed (ac4_shell_directions)."""
+    load()
+    n3 = np.empty((ndir, 3))
+    _check(lib.ac4_shell_directions(ndir, int(seed) & (2**64 - 1), _dp(n3)))
+    return n3
 
 
 def chain_init(n, T, seed):

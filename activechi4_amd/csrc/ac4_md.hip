@@ -163,63 +163,47 @@ __device__ __forceinline__ int xcd_block(int nb) {
 static inline int xcd_grid(int nb) { return ((nb + N_XCD - 1) / N_XCD) * N_XCD; }
 
 // ---------------------------------------------------------------- kernels
-// Swap shuffle of the three direction components (one lane each).  The swap
-// partners j_i = (int)(Nact*U_i) do not depend on the array, so for
-// Nact <= SHUF_LDS the 64 lanes draw them all first and the serial part is
-// only the LDS swaps.
-#define SHUF_LDS 4096
-__global__ void __launch_bounds__(64) k_shuffle(const RepC *__restrict__ reps, ChainS *__restrict__ chain,
-                                                int8_t *kx, int8_t *ky, int8_t *kz, int force) {
-    __shared__ int8_t s_k[3][SHUF_LDS];
-    __shared__ unsigned short s_j[3][SHUF_LDS];
-    const int r = blockIdx.x;
+// Swap shuffle of one direction component of one replica per workgroup
+// (grid: replicas x 3).  The swap partners j_i = (int)(Nact*U_i) do not depend
+// on the array, so the 256 lanes draw them SHUF_CH at a time into LDS and one
+// lane applies the swaps in order.  The component lives in dynamic LDS when
+// it fits (kcap bytes, sized from the device's LDS per workgroup), else in
+// global memory.  The shuffle counter is advanced by the force epilogue (or
+// the host for an explicit shuffle), after every component has read it.
+#define SHUF_CH 4096
+__global__ void __launch_bounds__(256) k_shuffle(const RepC *__restrict__ reps, const ChainS *__restrict__ chain,
+                                                 int8_t *kx, int8_t *ky, int8_t *kz, int force, int kcap) {
+    extern __shared__ unsigned int s_shuf[];  // SHUF_CH partners, then kcap bytes of directions
+    unsigned int *s_j = s_shuf;
+    int8_t *s_k = (int8_t *)(s_shuf + SHUF_CH);
+    const int r = blockIdx.x, comp = blockIdx.y;
     const RepC R = reps[r];
-    ChainS &c = chain[r];
+    const ChainS &c = chain[r];
     if (!force && (c.mode == 1 || R.tau <= 0 || (c.step % R.tau) != 0)) return;
     const uint64_t sh = (uint64_t)c.shuffles;
-    const bool lds = R.nact <= SHUF_LDS;
-    if (lds) {
-        for (int i = threadIdx.x; i < R.nact; i += 64) {
-            s_k[0][i] = kx[R.off_act + i];
-            s_k[1][i] = ky[R.off_act + i];
-            s_k[2][i] = kz[R.off_act + i];
-        }
-        for (int q = threadIdx.x; q < 3 * R.nact; q += 64) {
-            const int comp = q / R.nact, i = q - comp * R.nact;
-            s_j[comp][i] = (unsigned short)(int)((double)R.nact * ac4_shuffle_uniform(R.key0, R.key1, sh, comp, i));
-        }
-    }
-    __syncthreads();
-    const int comp = threadIdx.x;
-    if (comp < 3 && R.nact > 0) {
-        // MDcode/active.c:58-63: for i: j = (int)(Nact*U[0,1)); swap(k[i], k[j])
-        if (lds) {
-            int8_t *k = s_k[comp];
-            const unsigned short *jj = s_j[comp];
-            for (int i = 0; i < R.nact; i++) {
-                const int j = jj[i];
-                const int8_t a = k[i], b = k[j];
+    const int na = R.nact;
+    int8_t *kg = (comp == 0 ? kx : (comp == 1 ? ky : kz)) + R.off_act;
+    const bool lds = na <= kcap;
+    int8_t *k = lds ? s_k : kg;
+    if (lds)
+        for (int i = threadIdx.x; i < na; i += 256) s_k[i] = kg[i];
+    for (int c0 = 0; c0 < na; c0 += SHUF_CH) {
+        const int m = min(SHUF_CH, na - c0);
+        __syncthreads();
+        for (int q = threadIdx.x; q < m; q += 256)
+            s_j[q] = (unsigned int)(int)((double)na * ac4_shuffle_uniform(R.key0, R.key1, sh, comp, c0 + q));
+        __syncthreads();
+        if (threadIdx.x == 0)  // MDcode/active.c:58-63: for i: j = (int)(Nact*U[0,1)); swap(k[i], k[j])
+            for (int q = 0; q < m; q++) {
+                const int i = c0 + q, jj = (int)s_j[q];
+                const int8_t a = k[i], b = k[jj];
                 k[i] = b;
-                k[j] = a;
+                k[jj] = a;
             }
-        } else {
-            int8_t *k = (comp == 0 ? kx : (comp == 1 ? ky : kz)) + R.off_act;
-            for (int i = 0; i < R.nact; i++) {
-                const int j = (int)((double)R.nact * ac4_shuffle_uniform(R.key0, R.key1, sh, comp, i));
-                const int8_t a = k[i], b = k[j];
-                k[i] = b;
-                k[j] = a;
-            }
-        }
     }
     __syncthreads();
     if (lds)
-        for (int i = threadIdx.x; i < R.nact; i += 64) {
-            kx[R.off_act + i] = s_k[0][i];
-            ky[R.off_act + i] = s_k[1][i];
-            kz[R.off_act + i] = s_k[2][i];
-        }
-    if (threadIdx.x == 0) c.shuffles += 1;
+        for (int i = threadIdx.x; i < na; i += 256) kg[i] = s_k[i];
 }
 
 __global__ void __launch_bounds__(BLK) k_first_half(const RepC *__restrict__ reps, const ChainS *chain,
@@ -1119,6 +1103,7 @@ __global__ void __launch_bounds__(BLK) k_force(const RepC *__restrict__ reps, Ch
         if (rebuilt) c.rebuilds += 1;
         c.maxbits = 0ull;
         c.force_rebuild = 0;
+        if (R.tau > 0 && c.step % R.tau == 0) c.shuffles += 1;  // k_shuffle ran this step
         c.step += 1;
         return;
     }
@@ -1156,6 +1141,7 @@ __global__ void __launch_bounds__(BLK) k_force(const RepC *__restrict__ reps, Ch
     if (rebuilt) c.rebuilds += 1;
     c.maxbits = 0ull;
     c.force_rebuild = 0;
+    if (R.tau > 0 && c.step % R.tau == 0) c.shuffles += 1;  // k_shuffle ran this step
     c.step += 1;
 }
 
@@ -1168,6 +1154,8 @@ struct ac4_md {
     int *blk_rep = nullptr, *blk_first = nullptr, *blk_idx = nullptr, *fblk_rep = nullptr, *fblk_first = nullptr;
     int *cblk_rep = nullptr, *cblk_cell = nullptr;
     int fnblk = 0, lnblk = 0, ncb = 0;
+    int shuf_kcap = 0;     // largest Nact whose directions k_shuffle keeps in LDS
+    size_t shuf_lds = 0;   // dynamic LDS of k_shuffle
     int *wl_item = nullptr, *wl_n = nullptr;  // rebuild work list (WorkList)
     int *lblk_rep = nullptr, *lblk_first = nullptr;  // list-build block table (LPB particles per block)
     double *partial = nullptr;
@@ -1348,6 +1336,22 @@ extern "C" int ac4_md_create_slots(int nslots, const int *cap_n, int precision, 
     md->fnblk = fblk_off;
     md->lnblk = (int)lrep.size();
     md->ncb = ncb;
+    {  // k_shuffle: the partner chunk plus one component in LDS when it fits
+        int dev = 0, lds = 65536, cap_max = 0;
+        (void)hipGetDevice(&dev);
+        if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) lds = 65536;
+        for (int r = 0; r < nslots; r++) cap_max = std::max(cap_max, cap_n[r]);
+        const int room = lds - (int)(SHUF_CH * sizeof(unsigned int));
+        md->shuf_kcap = std::max(0, std::min(room, (cap_max + 15) & ~15));
+        md->shuf_lds = SHUF_CH * sizeof(unsigned int) + (size_t)md->shuf_kcap;
+        if (md->shuf_lds > 65536 &&
+            hipFuncSetAttribute((const void *)k_shuffle, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)md->shuf_lds) != hipSuccess) {
+            (void)hipGetLastError();
+            md->shuf_kcap = 65536 - (int)(SHUF_CH * sizeof(unsigned int));
+            md->shuf_lds = 65536;
+        }
+    }
     int rc = AC4_OK;
 #define ALLOC(ptr, n) if ((rc = dalloc(&md->ptr, (size_t)(n))) != AC4_OK) { ac4_md_destroy(md); return rc; }
     ALLOC(reps, nslots) ALLOC(chain, nslots) ALLOC(blk_rep, md->nblk) ALLOC(blk_first, md->nblk) ALLOC(blk_idx, md->nblk)
@@ -1759,7 +1763,8 @@ static int launch_force(ac4_md *md) {
 }
 
 static void launch_shuffle(ac4_md *md, int force, hipStream_t s) {
-    hipLaunchKernelGGL(k_shuffle, dim3(md->R), dim3(64), 0, s, md->reps, md->chain, md->kx, md->ky, md->kz, force);
+    hipLaunchKernelGGL(k_shuffle, dim3(md->R, 3), dim3(256), md->shuf_lds, s, md->reps, md->chain, md->kx, md->ky,
+                       md->kz, force, md->shuf_kcap);
 }
 static void launch_first_half(ac4_md *md) {
     hipLaunchKernelGGL(k_first_half, dim3(md->nblk), dim3(BLK), 0, md->stream, md->reps, md->chain, md->blk_rep,
@@ -1903,8 +1908,10 @@ extern "C" int ac4_md_shuffle(ac4_md *md) {
     AC4_REQUIRE(md, AC4_EARG, "NULL handle");
     launch_shuffle(md, 1, md->stream);
     AC4_HIP(hipGetLastError());
-    AC4_HIP(hipStreamSynchronize(md->stream));
-    return AC4_OK;
+    std::vector<ChainS> c;
+    TRY(get_chains(md, c));  // synchronises
+    for (auto &e : c) e.shuffles += 1;
+    return put_chains(md, c);
 }
 
 extern "C" int ac4_md_shuffle_uniforms(ac4_md *md, int r, double *u) {

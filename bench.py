@@ -229,6 +229,75 @@ def cpu_baseline(ac4, n=2485, steps=1500):
                       % (n, steps, dt)}
 
 
+def cfg5(ac4, n=1_000_000, steps=20000, nframes=500, relax=500, ndir=64, seed=5150):
+    """Large single box with S4: KA N=10^6, rho=1.2, T=0.5, c=0.1, f0=1.0,
+    tau_p=1.0, FP32 forces, 2e4 steps with about 500 log-spaced frames (the
+    reference schedule with the smallest origin count that gives that many),
+    every frame kept in HBM.  Q(t) and chi4(t) over the reference's
+    origin/lag walk (a=0.3) give the peak lag tau*; S4(q, tau*) then follows
+    on |q| shells from 0.1 to 7.25 in steps of 2 pi/L with ndir seeded random
+    directions per shell, weighted by the overlap of each origin at tau*."""
+    ac4.set_tuning(0, 4)
+    p = ac4.ReplicaParams(n, 1.2, 0.5, 1.0, -1, 200, seed)
+    t_all = time.perf_counter()
+    md = start(ac4, [p], ac4.FP32, relax)
+    for no in range(5, 200):
+        fidx = np.asarray(ac4.schedule(steps, no)[0], dtype=np.int64)
+        if len(fidx) >= nframes:
+            break
+    L_ana = (n / 1.2) ** (1.0 / 3.0)  # Analysis/main.c:15
+    fr = ac4.Frames(n, L_ana, len(fidx))
+    an = ac4.Analysis(n, L_ana, fidx, obs=ac4.OBS_OVERLAP)
+    log("cfg5: N=%d, %d frames (%d origins), frame store %.1f GB" % (n, len(fidx), no, 24e-9 * n * len(fidx)))
+    md.sync()
+    t0 = time.perf_counter()
+    t = 0
+    for f, fi in enumerate(fidx):
+        if fi > t:
+            md.step(int(fi - t), graph=True)
+            t = int(fi)
+        fr.push_md(md, 0, int(fi))
+        an.push_md(f, md, 0)
+    if t < steps:
+        md.step(steps - t, graph=True)
+    md.sync()
+    t_md = time.perf_counter() - t0
+    log("cfg5: %d steps with frames in %.1f s" % (steps, t_md))
+    an.finalize()
+    ov = an.overlap()  # per slot: sum_o Q, sum_o Q^2, origin counts (Analysis/overlap.c:89-118)
+    cnt = ov["counters"][: ov["noi"]].astype(float)
+    ok = np.arange(len(cnt)) >= 1
+    ok &= cnt > 0
+    qm = np.where(ok, ov["c"][: len(cnt)] / np.maximum(cnt, 1), np.nan)
+    x4 = np.where(ok, ov["x4"][: len(cnt)] / np.maximum(cnt, 1) - qm * qm, np.nan)
+    chi4 = n * x4
+    k = int(np.nanargmax(chi4))
+    tau_star = float(ov["delta_t"][k])
+    lag = int(round(tau_star / 0.005))
+    gap = an.gap
+    pairs = [(fr.find(o), fr.find(o + lag)) for o in range(0, steps + 1, gap)]
+    pairs = [q for q in pairs if q[0] >= 0 and q[1] >= 0]
+    dq = 2.0 * math.pi / L_ana
+    nshell = int((7.25 - 0.1) / dq + 1e-9) + 1
+    t1 = time.perf_counter()
+    s4r, s4d = fr.s4_shells([q[0] for q in pairs], [q[1] for q in pairs], 0.1, dq, nshell, ndir, seed)
+    t_s4 = time.perf_counter() - t1
+    qk = 0.1 + dq * np.arange(nshell)
+    pick = [0, 1, 2, 4, 8, 16, 32, 64, nshell - 1]
+    fr.close()
+    an.close()
+    md.close()
+    return {"metric": "cfg5 large box", "n": n, "steps": steps, "frames": int(len(fidx)), "origins": no,
+            "frame_store_GB": 24e-9 * n * len(fidx), "md_particle_steps_per_s": n * steps / t_md,
+            "md_seconds": t_md, "chi4_peak": float(chi4[k]), "tau_star": tau_star,
+            "Q_at_tau_star": float(qm[k]), "chi4_t": [[float(ov["delta_t"][i]), float(qm[i]), float(chi4[i])]
+                                                      for i in range(len(cnt)) if ok[i]],
+            "pairs_at_tau_star": len(pairs), "shells": nshell,
+            "directions": ndir, "s4_seconds": t_s4,
+            "S4_q": [float(qk[i]) for i in pick], "S4_ref_form": [float(s4r[i]) for i in pick],
+            "S4_dir_avg": [float(s4d[i]) for i in pick], "total_seconds": time.perf_counter() - t_all}
+
+
 def pmc_probe(ac4):
     """For hardware-counter runs (rocprofv3 --pmc): the batched workload with
     eager launches (every dispatch is counted), then three device-to-device
@@ -277,6 +346,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="short eager run + 256 MiB copies, for rocprofv3 --pmc (prints no bench line)")
+    ap.add_argument("--cfg5", action="store_true",
+                    help="the N=10^6 box with frames in HBM and S4 on |q| shells (its own JSON line)")
     ap.add_argument("--only-batched", action="store_true",
                     help="batched ensemble only (no plain, serving or CPU legs), e.g. under a profiler")
     a = ap.parse_args()
@@ -288,6 +359,9 @@ def main():
     ac4.require_gpu()
     if a.pmc_probe:
         pmc_probe(ac4)
+        return
+    if a.cfg5:
+        print(json.dumps(cfg5(ac4)), flush=True)
         return
 
     ps = ensemble_params(ac4, a.replicas)

@@ -146,6 +146,38 @@ int ac4_philox4x32(const uint32_t *ctr4, const uint32_t *key2, uint32_t *out4);
  * builders produce the same lists in the same order. */
 int ac4_set_tuning(int tpp, int nlist_mode);
 
+/* ------------------------------------------------- device frame store --
+ * A trajectory of one replica kept in HBM (x_u, box-scaled, reference
+ * order), so analyses can pair any origin with any lag after the run.  No
+ * reference counterpart: the reference writes data_coor.out and rereads it
+ * (MDcode/print.c:4-16, Analysis/read_all_data.c:61-91).  A frame of N
+ * particles takes 24*N bytes of device memory. */
+typedef struct ac4_frames ac4_frames;
+int ac4_frames_create(int n, double L, int cap, ac4_frames **out);
+int ac4_frames_destroy(ac4_frames *f);
+int ac4_frames_push(ac4_frames *f, const double *xu, const double *yu, const double *zu, long long step);
+/* Appends replica r's current x_u (device to device; synchronises the MD stream). */
+int ac4_frames_push_md(ac4_frames *f, ac4_md *md, int r, long long step);
+int ac4_frames_count(const ac4_frames *f);
+/* Index of the frame pushed with this step number, or -1. */
+int ac4_frames_find(const ac4_frames *f, long long step);
+/* Overlap count between frames i0 (origin) and i1 with the arithmetic of
+ * Analysis/overlap.c:61-84. */
+int ac4_frames_overlap(ac4_frames *f, int i0, int i1, double a, int *count);
+/* ndir unit directions (n3[3*d..3*d+2]) drawn from seed, uniform on the sphere. */
+int ac4_shell_directions(int ndir, uint64_t seed, double *n3);
+/* S4(q, t) on |q| shells, the overlap-weighted four-point structure factor
+ * of Analysis/S4Qt.c:4-267 with seeded random directions in place of the
+ * lattice vectors of GetQ.  Pair p = (origin frame orig[p], lag frame
+ * lagf[p]); |q_k| = q0 + k*dq for k < nshell; ndir directions from seed
+ * (ndir * ceil(nshell/27) <= 256).  s4_ref[k] = ndir*(Var_p Cbar + Var_p
+ * Sbar)/N with Cbar the direction mean (S4Qt's structure, the directions as
+ * its permutations); s4_dir[k] = mean_d (Var_p C + Var_p S)/N.  phases
+ * (nullable): [npairs][ndir][nshell][2] sums C, S.  Each may be NULL. */
+int ac4_frames_s4_shells(ac4_frames *f, int npairs, const int *orig, const int *lagf, double a, double q0,
+                         double dq, int nshell, int ndir, uint64_t seed, double *s4_ref, double *s4_dir,
+                         double *phases);
+
 /* ------------------------------------------------------ slots (serving) --
  * A handle can be created with idle slots of fixed capacity; jobs are
  * assigned to slots between launches and released when done.  No reference
