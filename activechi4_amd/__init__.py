@@ -104,7 +104,8 @@ _SIGS = {
     "ac4_md_set_nve": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int]),
     "ac4_chain_init": (None, [ctypes.c_int, ctypes.c_double, ctypes.c_uint64, _D]),
     "ac4_autotune": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ReplicaParams), ctypes.c_int, ctypes.c_int,
-                                    ctypes.c_int, _I, _D, ctypes.c_int, _I]),
+                                    ctypes.c_int, _D, _D, ctypes.c_int, _I]),
+    "ac4_set_skin": (ctypes.c_int, [ctypes.c_double]),
     "ac4_md_create_slots": (ctypes.c_int, [ctypes.c_int, _I, ctypes.c_int, ctypes.POINTER(_VP)]),
     "ac4_md_assign": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.POINTER(ReplicaParams)]),
     "ac4_md_release": (ctypes.c_int, [_VP, ctypes.c_int]),
@@ -526,15 +527,22 @@ def chain_init(n, T, seed):
 
 def autotune(params, precision=FP32, relax=300, steps=200):
     """Times every kernel shape on this replica set and keeps the fastest for
-    handles created afterwards (ac4_autotune).  Returns (tpp, nlist_mode) and
-    the table of rows (tpp, nlist_mode, particle-steps/s)."""
+    handles created afterwards (ac4_autotune).  Returns (tpp, nlist_mode,
+    skin) and the table of rows (tpp, nlist_mode, skin, particle-steps/s)."""
     load()
     arr = (ReplicaParams * len(params))(*params)
-    best = (ctypes.c_int * 2)()
-    table = np.zeros((32, 3))
+    best = np.zeros(3)
+    table = np.zeros((32, 4))
     nc = ctypes.c_int(0)
-    _check(lib.ac4_autotune(len(params), arr, precision, relax, steps, best, _dp(table), 32, ctypes.byref(nc)))
-    return (best[0], best[1]), table[: nc.value]
+    _check(lib.ac4_autotune(len(params), arr, precision, relax, steps, _dp(best), _dp(table), 32,
+                            ctypes.byref(nc)))
+    return (int(best[0]), int(best[1]), float(best[2])), table[: nc.value]
+
+
+def set_skin(skin=0.35):
+    """Verlet skin for handles created afterwards (ac4_set_skin)."""
+    load()
+    _check(lib.ac4_set_skin(skin))
 
 
 def set_tuning(tpp=0, nlist_mode=4):

@@ -49,10 +49,11 @@ struct RepC {
     int nblk, blk_off, nc_ref, tpp;    // nc_ref: the reference's grid (Parallel.c:72-80)
     int fnblk, fblk_off, ltpp, cap_n;  // force-kernel block table (tpp = 1<<ltpp lanes per particle)
     int cap_cells, lblk_off;           // lblk: list-build blocks of LPB particles
-    int lnblk, pad2;
+    int lnblk, lc;                     // lc: log2 of the list chunk (below)
     double L, invL, inv_cell_ref, T, f0, dof, density;
     uint32_t key0, key1;
     double qscale;  // L / 2^32: real length of one unit of a fixed-point coordinate
+    double skin;    // Verlet skin: list radii 2.5 sigma_ab + skin, rebuild criterion
     float qs2, pad3; // qscale^2 in FP32
 };
 
@@ -133,7 +134,7 @@ __device__ __forceinline__ bool need_rebuild(const RepC &R, const ChainS &c) {
     if (c.force_rebuild) return true;
     double maxD = __longlong_as_double((long long)c.maxbits);
     maxD *= R.L * R.L;  // time_evolve.c:553
-    return (c.crit == 1 ? 3.0 : 4.0) * maxD > K_SKIN * K_SKIN;  // :508 (crit 1)
+    return (c.crit == 1 ? 3.0 : 4.0) * maxD > R.skin * R.skin;  // :508 (crit 1, skin 0.35 there)
 }
 
 // Rebuild work of one step: the blocks of the replicas due for a rebuild,
@@ -495,20 +496,22 @@ __device__ __forceinline__ int stencil_axis(int c, int nc, int *out) {
 }
 
 // List radius per species pair: its force cutoff 2.5*sigma_ab plus the skin
-// 0.35 (MDcode/global.h:25-31 apply 2.85 to every pair); pairs are stored when
-// ((dx*dx+dy*dy)+dz*dz)*L*L < (2.5 sigma_ab + 0.35)^2.  The reference list
-// (< 2.85^2 for every pair, cell_list+forces.c:129-130) is counted beside it.
-__device__ __forceinline__ double list_cut2(int ab) {
-    const double rc = ab == 0 ? (2.5 * 1.00) + (.35 * 1.00) : (ab == 1 ? (2.5 * 0.80) + (.35 * 1.00) : (2.5 * 0.88) + (.35 * 1.00));
+// (0.35 in the reference, MDcode/global.h:25-31, which applies 2.85 to every
+// pair); pairs are stored when ((dx*dx+dy*dy)+dz*dz)*L*L < (2.5 sigma_ab +
+// skin)^2.  The reference list (< 2.85^2 for every pair,
+// cell_list+forces.c:129-130) is counted beside it.
+__device__ __forceinline__ double list_cut2(int ab, double skin) {
+    const double rc = (ab == 0 ? (2.5 * 1.00) : (ab == 1 ? (2.5 * 0.80) : (2.5 * 0.88))) + skin;
     return rc * rc;
 }
 
 // Each replica owns cap*n list entries from off*cap.  Neighbour k of local
-// particle i (teams of tpp lanes) lives at ((k/tpp)*n + i)*tpp + k%tpp in that
-// region: a wave's lanes (64/tpp particles x tpp) read 64 consecutive entries
-// for every chunk of tpp neighbours.
+// particle i lives at ((k/C)*n + i)*C + k%C in that region, C = 2^lc: with a
+// team of tpp > 1 lanes per particle C = tpp, so the team reads tpp
+// consecutive entries per chunk; with one lane per particle C = 8, so a lane
+// reads its next 8 entries as two 16-byte loads and a wave a contiguous 2 KB.
 __device__ __forceinline__ size_t nl_at(const RepC &R, int cap, int k, int i) {
-    return (size_t)R.off * cap + (((size_t)(k >> R.ltpp) * R.n + i) << R.ltpp) + (k & (R.tpp - 1));
+    return (size_t)R.off * cap + (((size_t)(k >> R.lc) * R.n + i) << R.lc) + (k & ((1 << R.lc) - 1));
 }
 
 // Dynamics-list screen shared by every builder (so all of them keep exactly
@@ -551,7 +554,7 @@ __global__ void __launch_bounds__(BLK) k_nlist(const RepC *__restrict__ reps, Ch
     int ax[3], ay[3], az[3];
     const int nx = stencil_axis(cx, nc, ax), ny = stencil_axis(cy, nc, ay), nz = stencil_axis(cz, nc, az);
     const double L = R.L;
-    const double cut_a = list_cut2(ti), cut_b = list_cut2(ti + 1);
+    const double cut_a = list_cut2(ti, R.skin), cut_b = list_cut2(ti + 1, R.skin);
     const float cfa = (float)cut_a + 1e-3f, cfb = (float)cut_b + 1e-3f;
     int count = 0, cref = 0;
     for (int iz = 0; iz < nz; iz++)
@@ -625,7 +628,8 @@ __device__ __forceinline__ void nlist_team_one(const RepC &R, int r, int i, Chai
     const int cx = c % nc, cy = (c / nc) % nc, cz = c / (nc * nc);
     int ax[3], ay[3], az[3];
     const int nx = stencil_axis(cx, nc, ax), ny = stencil_axis(cy, nc, ay), nz = stencil_axis(cz, nc, az);
-    const float cfa = (float)list_cut2(ti) + 1e-3f, cfb = (float)list_cut2(ti + 1) + 1e-3f, qs2 = R.qs2;
+    const float cfa = (float)list_cut2(ti, R.skin) + 1e-3f, cfb = (float)list_cut2(ti + 1, R.skin) + 1e-3f,
+                qs2 = R.qs2;
     int count = 0;
     auto inlist = [&](const uint4 &q) -> bool { return qscreen(pi, q, qs2, cfa, cfb); };
     auto emit = [&](bool hit, int k) {
@@ -741,7 +745,7 @@ __global__ void __launch_bounds__(64) k_nlist_cell(const RepC *__restrict__ reps
     for (int h = 0; h < hn; h++) {
         const uint4 pi = s_c[hoff + h];
         const int ti = (int)pi.w, kh = hoff + h;
-        const float cfa = (float)list_cut2(ti) + 1e-3f, cfb = (float)list_cut2(ti + 1) + 1e-3f;
+        const float cfa = (float)list_cut2(ti, R.skin) + 1e-3f, cfb = (float)list_cut2(ti + 1, R.skin) + 1e-3f;
         int base = 0;
         for (int k0 = 0; k0 < total; k0 += 64) {
             const int k = k0 + lane;
@@ -911,15 +915,16 @@ struct PairAcc<false> {  // FP64: the reference's per-pair arithmetic
 template <>
 struct PairAcc<true> {
     // FP32 pair forces from fixed-point positions (qpos): exact minimum
-    // image in integers, separations in units of L/2^32 rounded once to FP32,
-    // per-particle Kahan sums in those units, scaled to real units at the end.
-    // The pair constants of the two possible partner species are set once per
-    // particle; one hardware reciprocal of r^2 (v_rcp_f32, within 1 ulp)
-    // replaces the two divisions, since sigma^2/r^2 = (sigma/r)^2 and
-    // 1/r^2 appears in the force factor.
-    float fx = 0, fy = 0, fz = 0, cx = 0, cy = 0, cz = 0;
-    double pe = 0, vir = 0;
-    float isg[2], sg2[2], e4[2], e8[2], rc2[2];
+    // image in integers, separations in units of L/2^32 rounded once to FP32.
+    // Each FP32 pair term is widened to FP64 and summed there (exactly the
+    // compensation a Kahan sum aims at, and on CDNA4 a widening add costs
+    // two FP64 instructions against five FP32 ones); the force is scaled to
+    // real units at the end.  The pair constants of the two possible partner
+    // species are set once per particle; one hardware reciprocal of r^2
+    // (v_rcp_f32, within 1 ulp) replaces the two divisions, since
+    // sigma^2/r^2 = (sigma/r)^2 and 1/r^2 appears in the force factor.
+    double fx = 0, fy = 0, fz = 0, pe = 0, vir = 0;
+    float sg2[2], e4[2], e8[2], rc2[2], c2i[2];
     __device__ __forceinline__ explicit PairAcc(int ti) {
         // (ti, tj) -> ab = ti + tj: sigma {1, 0.8, 0.88}, epsilon {1, 1.5, 0.5}
         // (cell_list+forces.c:19-20); cutoff (r/sigma)^2 < 6.25 (:189)
@@ -927,18 +932,12 @@ struct PairAcc<true> {
             const int ab = ti + tj;
             const double sig = ab == 0 ? 1.00 : (ab == 1 ? 0.80 : 0.88);
             const double eps = ab == 0 ? 1.00 : (ab == 1 ? 1.50 : 0.50);
-            isg[tj] = (float)(1.0 / (sig * sig));
             sg2[tj] = (float)(sig * sig);
             e4[tj] = (float)(4.0 * eps);
             e8[tj] = (float)(8.0 * eps);
             rc2[tj] = (float)(K_LCUTSQR * sig * sig);
+            c2i[tj] = (float)(K_C2 / (sig * sig));  // C2 / sigma^2
         }
-    }
-    __device__ __forceinline__ static void kahan(float &s, float &c, float v) {
-        float y = v - c;
-        float t = s + y;
-        c = (t - s) - y;
-        s = t;
     }
     __device__ __forceinline__ void add(const uint4 &pi, int, const uint4 &pj, const RepC &R) {
         const float dx = (float)(int)(pi.x - pj.x);
@@ -947,23 +946,25 @@ struct PairAcc<true> {
         const bool b = pj.w != 0u;
         const float r2 = (dx * dx + dy * dy + dz * dz) * R.qs2;  // real units
         if (r2 < (b ? rc2[1] : rc2[0])) {
-            const float isig2 = b ? isg[1] : isg[0], sig2 = b ? sg2[1] : sg2[0];
+            const float c2 = b ? c2i[1] : c2i[0];
             const float inv = __builtin_amdgcn_rcpf(r2);
-            const float s2 = sig2 * inv;  // (sigma/r)^2
+            const float s2 = (b ? sg2[1] : sg2[0]) * inv;  // (sigma/r)^2
             const float s6 = s2 * s2 * s2;
-            const float rs2 = r2 * isig2;
-            pe += (double)((b ? e4[1] : e4[0]) * ((s6 * (s6 - 1.0f)) + (float)K_C0 + (float)K_C2 * rs2));
-            const float grad = (b ? e8[1] : e8[0]) * (6.0f * s6 * (s6 - 0.5f) * inv - (float)K_C2 * isig2);
-            kahan(fx, cx, grad * dx);
-            kahan(fy, cy, grad * dy);
-            kahan(fz, cz, grad * dz);
+            // cell_list+forces.c:194-203: 4 eps [s6 (s6 - 1) + C0 + C2 (r/sigma)^2],
+            // 8 eps [6 s6 (s6 - 1/2) / r^2 - C2 / sigma^2]
+            const float pterm = (b ? e4[1] : e4[0]) * ((s6 * (s6 - 1.0f)) + (float)K_C0 + c2 * r2);
+            const float grad = (b ? e8[1] : e8[0]) * (6.0f * s6 * (s6 - 0.5f) * inv - c2);
+            fx += (double)(grad * dx);
+            fy += (double)(grad * dy);
+            fz += (double)(grad * dz);
+            pe += (double)pterm;
             vir += (double)(grad * r2);
         }
     }
     __device__ __forceinline__ void result(double &ox, double &oy, double &oz, const RepC &R) const {
-        ox = ((double)fx - (double)cx) * R.qscale;
-        oy = ((double)fy - (double)cy) * R.qscale;
-        oz = ((double)fz - (double)cz) * R.qscale;
+        ox = fx * R.qscale;
+        oy = fy * R.qscale;
+        oz = fz * R.qscale;
     }
 };
 
@@ -1000,6 +1001,32 @@ __global__ void __launch_bounds__(BLK) k_force(const RepC *__restrict__ reps, Ch
         const int ti = (int)pi.w;
         PairAcc<FP32> acc(ti);
         const int m = nn[g];
+        if (R.tpp == 1) {
+            // chunks of 8 entries (two 16-byte loads); the next chunk's
+            // indices are loaded while this chunk's positions are gathered.
+            // Entries past m in the last chunk are loaded, never used.
+            const int *lp = nl + nl_at(R, cap, 0, i);
+            const size_t cst = (size_t)R.n * 8;
+            int4 a0 = make_int4(0, 0, 0, 0), a1 = a0;
+            if (m > 0) { a0 = *(const int4 *)lp; a1 = *(const int4 *)(lp + 4); }
+            int k = 0;
+            while (k < m) {
+                const int jj[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
+                const int valid = min(8, m - k);
+                PosT q[8];
+#pragma unroll
+                for (int u = 0; u < 8; u++) q[u] = pos[u < valid ? jj[u] : g];
+                k += 8;
+                lp += cst;
+                if (k < m) { a0 = *(const int4 *)lp; a1 = *(const int4 *)(lp + 4); }
+#pragma unroll
+                for (int u = 0; u < 8; u++)
+                    if (u < valid) acc.add(pi, ti, q[u], R);
+            }
+            acc.result(Fx, Fy, Fz, R);
+            v_pe = acc.pe;
+            v_vir = acc.vir;
+        } else {
         // entry t + c*tpp of particle i: chunk c of the replica's region
         const int *lp = nl + nl_at(R, cap, t, i);
         const size_t cstep = (size_t)R.n << R.ltpp;
@@ -1028,6 +1055,7 @@ __global__ void __launch_bounds__(BLK) k_force(const RepC *__restrict__ reps, Ch
         acc.result(Fx, Fy, Fz, R);
         v_pe = acc.pe;
         v_vir = acc.vir;
+        }
     }
     for (int o = 1; o < tpp; o <<= 1) {  // team butterfly (teams are aligned lane groups)
         Fx += __shfl_xor(Fx, o, 64);
@@ -1219,6 +1247,15 @@ static void free_parts(PartArrays &a) {
 // 16/32/64 lanes per particle.
 #define TPP_LANES 65536.0
 static int g_tpp = 0, g_nlist_mode = 4;
+static double g_skin = K_SKIN;
+
+// Verlet skin of handles created afterwards (0.2 ... 0.6; the reference's is
+// 0.35).  A larger skin rebuilds less often and gives longer lists.
+extern "C" int ac4_set_skin(double skin) {
+    AC4_REQUIRE(skin >= 0.2 && skin <= 0.6, AC4_EARG, "ac4_set_skin: skin must be in [0.2, 0.6]");
+    g_skin = skin;
+    return AC4_OK;
+}
 
 extern "C" int ac4_set_tuning(int tpp, int nlist_mode) {
     AC4_REQUIRE((tpp == 0 || tpp == 1 || tpp == 2 || tpp == 4 || tpp == 8 || tpp == 16) &&
@@ -1237,7 +1274,8 @@ static const char *configure_rep(RepC &R, const ac4_replica_params &q) {
     const int nact = q.n_act < 0 ? (int)((double)q.n * 0.1000) : q.n_act;
     if (nact > q.n) return "n_act larger than n";
     const double L = pow((double)q.n / q.density, 0.33333333);  // MDcode/main.c:465
-    if (L <= 2.0 * K_CELL_CUT) return "box too small for the 2.85 list cutoff";
+    const double cell_cut = std::max(K_CELL_CUT, 2.5 + R.skin);  // a cell spans the largest list radius
+    if (L <= 2.0 * cell_cut) return "box too small for the list cutoff";
     R.n = q.n;
     R.nact = nact;
     R.L = L;
@@ -1247,7 +1285,7 @@ static const char *configure_rep(RepC &R, const ac4_replica_params &q) {
     R.nc_ref = (int)((double)((int)Lpp) / K_CELL_CUT);
     if (R.nc_ref < 1) R.nc_ref = 1;
     R.inv_cell_ref = R.L / (Lpp / (double)R.nc_ref);
-    R.nc = (int)(R.L / K_CELL_CUT);  // list grid: cells at least 2.85 wide
+    R.nc = (int)(R.L / cell_cut);  // list grid: cells at least one list radius (and 2.85) wide
     R.ncell = R.nc * R.nc * R.nc;
     if (R.ncell > R.cap_cells) return "density below the slot's cell capacity (0.5)";
     R.qscale = R.L / 4294967296.0;
@@ -1299,6 +1337,7 @@ extern "C" int ac4_md_create_slots(int nslots, const int *cap_n, int precision, 
         const int cn = cap_n[r];
         if (cn < 2) { delete md; ac4_set_error("slot %d: capacity %d < 2", r, cn); return AC4_EARG; }
         R.cap_n = cn;
+        R.skin = g_skin;
         R.off = off;
         R.off_act = off_act;
         R.off_cell = off_cell;
@@ -1307,6 +1346,7 @@ extern "C" int ac4_md_create_slots(int nslots, const int *cap_n, int precision, 
         R.cap_cells = cap_cells;
         R.tpp = tpp;
         R.ltpp = ltpp;
+        R.lc = tpp == 1 ? 3 : ltpp;
         R.nblk = (cn + BLK - 1) / BLK;
         R.blk_off = blk_off;
         for (int k = 0; k < R.nblk; k++) { brep.push_back(r); bfirst.push_back(k * BLK); bidx.push_back(k); }

@@ -5,7 +5,9 @@
  * state, replays `steps` graph steps, and the fastest shape is kept for the
  * handles created afterwards (ac4_set_tuning).  Candidates: force-kernel lanes
  * per particle tpp in {1, 2, 4, 8, 16} (so 256/tpp particles per workgroup)
- * crossed with the thread-per-particle and 16/32/64-lane-team list builders.
+ * crossed with the thread-per-particle and 16/32/64-lane-team list builders,
+ * then the Verlet skin {0.30, 0.35, 0.45, 0.55} for the best of those (the
+ * skin sets the list radii, the rebuild interval and the list-cell size).
  */
 #include <stdint.h>
 #include <stdlib.h>
@@ -46,16 +48,35 @@ static void free_states(rep_state *st, int n_rep) {
     free(st);
 }
 
-int ac4_autotune(int n_rep, const ac4_replica_params *p, int precision, int relax_steps, int steps, int *best2,
+static double time_shape(int n_rep, const ac4_replica_params *p, int precision, int steps, rep_state *st,
+                         long long nsum, int tpp, int mode, double skin) {
+    ac4_md *md = NULL;
+    double rate = 0.0;
+    int ok = ac4_set_tuning(tpp, mode) == AC4_OK && ac4_set_skin(skin) == AC4_OK &&
+             ac4_md_create(n_rep, p, precision, &md) == AC4_OK;
+    if (ok) ok = load_all(md, n_rep, st) == AC4_OK && ac4_md_step_graph(md, 20) == AC4_OK && ac4_md_sync(md) == AC4_OK;
+    if (ok) {
+        const double t0 = now_s();
+        ok = ac4_md_step_graph(md, steps) == AC4_OK && ac4_md_sync(md) == AC4_OK;  // sync reports list overflow
+        const double t1 = now_s();
+        if (ok && t1 > t0) rate = (double)nsum * steps / (t1 - t0);
+    }
+    if (md) ac4_md_destroy(md);
+    return rate;
+}
+
+int ac4_autotune(int n_rep, const ac4_replica_params *p, int precision, int relax_steps, int steps, double *best3,
                  double *table, int table_cap, int *n_cand) {
     if (n_rep < 1 || !p || relax_steps < 1 || steps < 1) {
         ac4_set_error("ac4_autotune: bad arguments");
         return AC4_EARG;
     }
     static const int tpps[5] = {1, 2, 4, 8, 16}, modes[4] = {1, 2, 3, 4};
+    static const double skins[4] = {0.30, 0.35, 0.45, 0.55};
     rep_state *st = calloc((size_t)n_rep, sizeof *st);
     ac4_md *md = NULL;
     int rc = ac4_set_tuning(0, 4);
+    if (rc == AC4_OK) rc = ac4_set_skin(0.35);
     if (rc == AC4_OK) rc = ac4_md_create(n_rep, p, precision, &md);
     long long nsum = 0;
     for (int r = 0; r < n_rep && rc == AC4_OK; r++) {
@@ -82,43 +103,46 @@ int ac4_autotune(int n_rep, const ac4_replica_params *p, int precision, int rela
                               d + 8 * n, q + 2 * n, q + 2 * n + na, q + 2 * n + 2 * na);
     }
     ac4_md_destroy(md);
-    md = NULL;
     if (rc != AC4_OK) {
         free_states(st, n_rep);
         ac4_set_tuning(0, 4);
+        ac4_set_skin(0.35);
         return rc;
     }
-    double best = -1.0;
+    double best = -1.0, bs = 0.35;
     int bt = 0, bm = 4, nc = 0;
+#define RECORD(T_, M_, S_, V_)                                                        \
+    do {                                                                               \
+        if (table && nc < table_cap) {                                                 \
+            table[4 * nc] = (T_); table[4 * nc + 1] = (M_);                            \
+            table[4 * nc + 2] = (S_); table[4 * nc + 3] = (V_);                        \
+        }                                                                              \
+        nc++;                                                                          \
+    } while (0)
+    /* stage 1: workgroup shape and list builder at the reference skin */
     for (int a = 0; a < 5; a++)
         for (int b = 0; b < 4; b++) {
-            double rate = 0.0;
-            int ok = ac4_set_tuning(tpps[a], modes[b]) == AC4_OK && ac4_md_create(n_rep, p, precision, &md) == AC4_OK;
-            if (ok) ok = load_all(md, n_rep, st) == AC4_OK && ac4_md_step_graph(md, 20) == AC4_OK &&
-                         ac4_md_sync(md) == AC4_OK;
-            if (ok) {
-                const double t0 = now_s();
-                ok = ac4_md_step_graph(md, steps) == AC4_OK && ac4_md_sync(md) == AC4_OK;
-                const double t1 = now_s();
-                if (ok && t1 > t0) rate = (double)nsum * steps / (t1 - t0);
-            }
-            if (md) ac4_md_destroy(md);
-            md = NULL;
-            if (table && nc < table_cap) {
-                table[3 * nc] = tpps[a];
-                table[3 * nc + 1] = modes[b];
-                table[3 * nc + 2] = rate;
-            }
-            nc++;
+            const double rate = time_shape(n_rep, p, precision, steps, st, nsum, tpps[a], modes[b], 0.35);
+            RECORD(tpps[a], modes[b], 0.35, rate);
             if (rate > best) { best = rate; bt = tpps[a]; bm = modes[b]; }
         }
+    /* stage 2: the skin (list radius and cell size) for that shape */
+    for (int k = 0; k < 4; k++) {
+        if (skins[k] == 0.35) continue;
+        const double rate = time_shape(n_rep, p, precision, steps, st, nsum, bt, bm, skins[k]);
+        RECORD(bt, bm, skins[k], rate);
+        if (rate > best) { best = rate; bs = skins[k]; }
+    }
+#undef RECORD
     free_states(st, n_rep);
     if (n_cand) *n_cand = nc;
     if (best <= 0.0) {
         ac4_set_tuning(0, 4);
+        ac4_set_skin(0.35);
         ac4_set_error("ac4_autotune: no candidate ran");
         return AC4_EHIP;
     }
-    if (best2) { best2[0] = bt; best2[1] = bm; }
-    return ac4_set_tuning(bt, bm);
+    if (best3) { best3[0] = bt; best3[1] = bm; best3[2] = bs; }
+    rc = ac4_set_tuning(bt, bm);
+    return rc == AC4_OK ? ac4_set_skin(bs) : rc;
 }

@@ -78,6 +78,12 @@ def start(ac4, ps, precision, relax):
     return md
 
 
+def default_shape(ac4):
+    """Kernel shape chosen automatically from the handle size, reference skin 0.35."""
+    ac4.set_tuning(0, 4)
+    ac4.set_skin(0.35)
+
+
 def log(msg):
     print("bench: " + msg, file=sys.stderr, flush=True)
 
@@ -124,7 +130,7 @@ def chi4_report(md, ps, sample_every):
 
 def plain_sequential(ac4, ps, relax, steps):
     """The same replicas one after another: FP64, eager launches."""
-    ac4.set_tuning(0, 4)
+    default_shape(ac4)
     tot_ps, tot_t = 0.0, 0.0
     for p in ps:
         md = start(ac4, [p], ac4.FP64, relax)
@@ -140,7 +146,7 @@ def plain_sequential(ac4, ps, relax, steps):
 
 
 def plain_rate(ac4, n, T, f0, tau, seed, relax=150, steps=200):
-    ac4.set_tuning(0, 4)
+    default_shape(ac4)
     md = start(ac4, [ac4.ReplicaParams(n, 1.2, T, f0, -1, tau, seed)], ac4.FP64, relax)
     md.step(30)
     md.sync()
@@ -167,7 +173,7 @@ def serving(ac4, njobs, rate_per_s, seed=4242):
     # share of jobs it takes under the N distribution (about 1.2 KB per
     # particle of capacity incl. the 192-entry neighbour list)
     caps = [1024] * 8 + [2048] * 10 + [4096] * 10 + [8192] * 8 + [20480] * 4
-    ac4.set_tuning(0, 4)
+    default_shape(ac4)
     srv = ac4.Server(caps, precision=ac4.FP32, chunk=100, relax=300)
     ids = [srv.submit(j["n"], j["T"], j["f0"], j["tau_p"], j["steps"], j["seed"], j["t"]) for j in jobs]
     t0 = time.perf_counter()
@@ -213,7 +219,7 @@ def cpu_baseline(ac4, n=2485, steps=1500):
     import oracle_lib as OL
     lib = OL.load()
     p = ac4.ReplicaParams(n, 1.2, 0.5, 1.0, -1, 0, 99)
-    ac4.set_tuning(0, 4)
+    default_shape(ac4)
     md = start(ac4, [p], ac4.FP64, 300)
     g = md.get_state(0)
     chain = md.get_chain(0)
@@ -237,7 +243,7 @@ def cfg5(ac4, n=1_000_000, steps=20000, nframes=500, relax=500, ndir=64, seed=51
     origin/lag walk (a=0.3) give the peak lag tau*; S4(q, tau*) then follows
     on |q| shells from 0.1 to 7.25 in steps of 2 pi/L with ndir seeded random
     directions per shell, weighted by the overlap of each origin at tau*."""
-    ac4.set_tuning(0, 4)
+    default_shape(ac4)
     p = ac4.ReplicaParams(n, 1.2, 0.5, 1.0, -1, 200, seed)
     t_all = time.perf_counter()
     md = start(ac4, [p], ac4.FP32, relax)
@@ -369,9 +375,9 @@ def main():
     tune = None
     if not a.no_tune:
         best, table = ac4.autotune(ps, ac4.FP32, relax=a.relax, steps=200)
-        tune = {"tpp": best[0], "nlist_mode": best[1],
-                "candidates": [[int(t), int(m), float(v)] for t, m, v in table]}
-        log("autotune: tpp=%d nlist_mode=%d" % best)
+        tune = {"tpp": best[0], "nlist_mode": best[1], "skin": best[2],
+                "candidates": [[int(t), int(m), float(sk), float(v)] for t, m, sk, v in table]}
+        log("autotune: tpp=%d nlist_mode=%d skin=%.2f" % best)
     md = start(ac4, ps, ac4.FP32, a.relax)
     sample_every = 100
     md.overlap_enable(sample_every, 1000, 21, 200, 0.3)

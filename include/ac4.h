@@ -145,6 +145,10 @@ int ac4_philox4x32(const uint32_t *ctr4, const uint32_t *key2, uint32_t *out4);
  * 2/3/4 teams of 16/32/64 lanes per particle (4 is the default).  All
  * builders produce the same lists in the same order. */
 int ac4_set_tuning(int tpp, int nlist_mode);
+/* Verlet skin of handles created afterwards, in [0.2, 0.6] (default 0.35 =
+ * 2.85 - 2.5, MDcode/global.h:25-31).  Sets the per-species list radii
+ * 2.5 sigma_ab + skin, the rebuild criterion and the list-cell size. */
+int ac4_set_skin(double skin);
 
 /* ------------------------------------------------- device frame store --
  * A trajectory of one replica kept in HBM (x_u, box-scaled, reference
@@ -325,10 +329,12 @@ void ac4_chain_init(int n, double T, uint64_t seed, double *chain9);
 /* Autotuner (no reference counterpart): from one relaxed start of the given
  * replica set, times `steps` graph-replayed steps for every kernel shape
  * (force lanes per particle 1..16, i.e. 256..16 particles per workgroup;
- * neighbour-list builder) and keeps the fastest with ac4_set_tuning.
- * table (rows of {tpp, nlist_mode, particle-steps/s}) may be NULL. */
+ * neighbour-list builder), then the skin for the best shape, and keeps the
+ * fastest with ac4_set_tuning / ac4_set_skin.  best3 = {tpp, nlist_mode,
+ * skin}; table (rows of {tpp, nlist_mode, skin, particle-steps/s}) may be
+ * NULL. */
 int ac4_autotune(int n_rep, const ac4_replica_params *p, int precision, int relax_steps, int steps,
-                 int *best2, double *table, int table_cap, int *n_cand);
+                 double *best3, double *table, int table_cap, int *n_cand);
 /* Ensemble chi4 over replicas (no reference counterpart; chi4 as an
  * ensemble fluctuation over independent replicas): q[r*nor+o]; out2 = {pooled Nbar*(<Q^2>-<Q>^2), mean_r N_r*Var_o Q_r}. */
 int ac4_chi4_ensemble(int nrep, int nor, const double *q, const int *n_r,

@@ -31,6 +31,7 @@ def tuning(ac4, tpp, nlist_mode=4):
         yield
     finally:
         ac4.set_tuning(0, 4)
+        ac4.set_skin(0.35)
 
 
 _RELAXED = {}
@@ -93,6 +94,32 @@ def test_forces_fp64_match_oracle(gpu, n, f0):
     assert _rel_force_err(md.forces(0), (fx, fy, fz)) < 1e-12
     assert pv[0] == pytest.approx(o[0], rel=1e-12)
     assert pv[1] == pytest.approx(o[1], rel=1e-12)
+
+
+@pytest.mark.parametrize("skin", [0.25, 0.5])
+def test_skin_changes_lists_not_forces(gpu, skin):
+    """A different Verlet skin changes list radii, cell size and rebuild
+    interval, never the forces: FP64 forces still match the oracle, and a
+    run of graph-replayed steps reports no list overflow."""
+    s, L = lattice_state(3000, jitter=0.08, seed=44)
+    gpu.set_skin(skin)
+    try:
+        md = make_md(gpu, [(s, L)], f0=1.0)
+    finally:
+        gpu.set_skin(0.35)
+    md.calculate_forces()
+    cnt, lst, _ = OL.half_list(s["x"], s["y"], s["z"], L)
+    fx, fy, fz, o = OL.forces_list(s, L, 1.0, cnt, lst)
+    assert _rel_force_err(md.forces(0), (fx, fy, fz)) < 1e-12
+    md.close()
+    gpu.set_skin(skin)
+    try:
+        md = make_relaxed_md(gpu, [(2000, 45, 6)], tau=50, prec=1)
+    finally:
+        gpu.set_skin(0.35)
+    md.step(400, graph=True)
+    md.sync()  # raises on a list overflow or a runaway coordinate
+    assert md.rebuilds(0) > 0
 
 
 def test_forces_mixed_batch_match_oracle(gpu):
