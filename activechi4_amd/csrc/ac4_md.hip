@@ -396,6 +396,32 @@ __global__ void __launch_bounds__(BLK) k_scatter(const RepC *__restrict__ reps, 
     }
 }
 
+// Distinct neighbour-cell coordinates along one axis (cells are at least one
+// list radius wide); with fewer than 3 cells per axis every cell is a
+// neighbour once (an aliased stencil counts pairs twice, cf.
+// cell_list+forces.c:37-49 with two cells per axis).
+__device__ __forceinline__ int stencil_axis(int c, int nc, int *out) {
+    if (nc >= 3) { out[0] = (c + nc - 1) % nc; out[1] = c; out[2] = (c + 1) % nc; return 3; }
+    for (int d = 0; d < nc; d++) out[d] = d;
+    return nc;
+}
+
+// Stencil of a cell as contiguous slot ranges: the 9 (z, y) rows of the
+// 27-cell stencil, each as at most two runs of consecutive cells along x (two
+// where the row wraps), in stencil order.  Cells of one x-run are adjacent in
+// cell order, so their members form one slot range.  NRANGE = 18 entries per
+// cell, empty ones (s == e) where the grid has fewer than 3 cells per axis.
+#define NRANGE 18
+__device__ __forceinline__ void xruns(int cx, int nc, int *f0, int *l0, int *f1, int *l1) {
+    if (nc >= 3) {
+        if (cx == 0) { *f0 = nc - 1; *l0 = nc - 1; *f1 = 0; *l1 = 1; }
+        else if (cx == nc - 1) { *f0 = nc - 2; *l0 = nc - 1; *f1 = 0; *l1 = 0; }
+        else { *f0 = cx - 1; *l0 = cx + 1; *f1 = 0; *l1 = -1; }
+    } else {
+        *f0 = 0; *l0 = nc - 1; *f1 = 0; *l1 = -1;  // every cell once (stencil_axis)
+    }
+}
+
 // Members of a cell in increasing original index (the binning order of
 // cell_list+forces.c:71-80): a canonical order, so lists and force sums do not
 // depend on launch timing or on which replicas share the launch.  One wave per
@@ -403,7 +429,7 @@ __global__ void __launch_bounds__(BLK) k_scatter(const RepC *__restrict__ reps, 
 #define SORT_CAP 256
 __global__ void __launch_bounds__(BLK) k_cell_sort(const RepC *__restrict__ reps, WorkList wl,
                                                    const int *__restrict__ cnt, const int *__restrict__ start,
-                                                   const int *__restrict__ id, int *members) {
+                                                   const int *__restrict__ id, int *members, int2 *ranges) {
     __shared__ int s_m[BLK / 64][SORT_CAP];
     __shared__ int s_k[BLK / 64][SORT_CAP];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -413,6 +439,23 @@ __global__ void __launch_bounds__(BLK) k_cell_sort(const RepC *__restrict__ reps
         const RepC R = reps[r];
         const int c = (item & 0xffff) * (BLK / 64) + w;  // one cell per wave
         if (c >= R.ncell) continue;
+        if (lane < NRANGE) {  // the cell's stencil as slot ranges, for the list builders
+            const int nc = R.nc, cx = c % nc, cy = (c / nc) % nc, cz = c / (nc * nc);
+            int ay[3], az[3];
+            const int ny = stencil_axis(cy, nc, ay), nz = stencil_axis(cz, nc, az);
+            const int row = lane >> 1, part = lane & 1, iz = row / 3, iy = row - 3 * iz;
+            int2 rg = make_int2(0, 0);
+            if (iz < nz && iy < ny) {
+                int f0, l0, f1, l1;
+                xruns(cx, nc, &f0, &l0, &f1, &l1);
+                const int f = part ? f1 : f0, l = part ? l1 : l0;
+                if (l >= f) {
+                    const int rb = R.off_cell + ay[iy] * nc + az[iz] * nc * nc;
+                    rg = make_int2(R.off + start[rb + f], R.off + start[rb + l] + cnt[rb + l]);
+                }
+            }
+            ranges[(size_t)(R.off_cell + c) * NRANGE + lane] = rg;
+        }
         int *m = members + R.off + start[R.off_cell + c];
         const int n = cnt[R.off_cell + c];
         if (n <= SORT_CAP) {
@@ -485,15 +528,6 @@ __global__ void __launch_bounds__(BLK) k_commit(const RepC *__restrict__ reps, c
     }
 }
 
-// Distinct neighbour-cell coordinates along one axis (cells are at least one
-// list radius wide); with fewer than 3 cells per axis every cell is a
-// neighbour once (an aliased stencil counts pairs twice, cf.
-// cell_list+forces.c:37-49 with two cells per axis).
-__device__ __forceinline__ int stencil_axis(int c, int nc, int *out) {
-    if (nc >= 3) { out[0] = (c + nc - 1) % nc; out[1] = c; out[2] = (c + 1) % nc; return 3; }
-    for (int d = 0; d < nc; d++) out[d] = d;
-    return nc;
-}
 
 // List radius per species pair: its force cutoff 2.5*sigma_ab plus the skin
 // (0.35 in the reference, MDcode/global.h:25-31, which applies 2.85 to every
@@ -616,18 +650,14 @@ __global__ void __launch_bounds__(BLK) k_nlist(const RepC *__restrict__ reps, Ch
 template <int NLT>
 __device__ __forceinline__ void nlist_team_one(const RepC &R, int r, int i, ChainS *chain,
                                                const uint4 *__restrict__ posq, const int *__restrict__ cell_of,
-                                               const int *__restrict__ cnt, const int *__restrict__ start, int *nn,
-                                               int *nl, int cap) {
+                                               const int2 *__restrict__ ranges, int *nn, int *nl, int cap) {
     const int lane = threadIdx.x & 63, t = lane & (NLT - 1);
-    const unsigned long long tmask = (NLT == 64 ? ~0ull : ((1ull << NLT) - 1ull)) << (lane & ~(NLT - 1));
+    const unsigned long long tmask = (NLT == 64 ? ~0ull : ((1ull << (NLT & 63)) - 1ull)) << (lane & ~(NLT - 1));
     const unsigned long long below = (1ull << lane) - 1ull;
-    const int g = R.off + i, nc = R.nc;
+    const int g = R.off + i;
     const uint4 pi = posq[g];
     const int ti = (int)pi.w;
-    const int c = cell_of[g];
-    const int cx = c % nc, cy = (c / nc) % nc, cz = c / (nc * nc);
-    int ax[3], ay[3], az[3];
-    const int nx = stencil_axis(cx, nc, ax), ny = stencil_axis(cy, nc, ay), nz = stencil_axis(cz, nc, az);
+    const int2 *rgs = ranges + (size_t)(R.off_cell + cell_of[g]) * NRANGE;
     const float cfa = (float)list_cut2(ti, R.skin) + 1e-3f, cfb = (float)list_cut2(ti + 1, R.skin) + 1e-3f,
                 qs2 = R.qs2;
     int count = 0;
@@ -640,30 +670,20 @@ __device__ __forceinline__ void nlist_team_one(const RepC &R, int r, int i, Chai
         }
         count += __popcll(m);
     };
-    for (int iz = 0; iz < nz; iz++)
-        for (int iy = 0; iy < ny; iy++) {
-            // cells of one x-row that follow each other in cell order are one
-            // contiguous slot range (two ranges where the row wraps), taken in
-            // stencil order
-            const int row = R.off_cell + ay[iy] * nc + az[iz] * nc * nc;
-            int ix = 0;
-            while (ix < nx) {
-                const int cf = ax[ix];
-                int cl = cf;
-                while (ix + 1 < nx && ax[ix + 1] == cl + 1) { ix++; cl++; }
-                ix++;
-                const int s2 = R.off + start[row + cf], e2 = R.off + start[row + cl] + cnt[row + cl];
-                // two candidates per lane, both loaded before either is used
-                // (indices clamped into [s2, e2), validity applied after)
-                for (int k0 = s2; k0 < e2; k0 += 2 * NLT) {
-                    const int ka = k0 + t, kb = k0 + NLT + t;
-                    const uint4 qa = posq[min(ka, e2 - 1)], qb = posq[min(kb, e2 - 1)];
-                    const bool ha = inlist(qa) && ka < e2 && ka != g, hb = inlist(qb) && kb < e2 && kb != g;
-                    emit(ha, ka);
-                    emit(hb, kb);
-                }
-            }
+    // the home cell's stencil as slot ranges (k_cell_sort), in stencil order
+    for (int q = 0; q < NRANGE; q++) {
+        const int2 rg = rgs[q];
+        const int s2 = rg.x, e2 = rg.y;
+        // two candidates per lane, both loaded before either is used
+        // (indices clamped into [s2, e2), validity applied after)
+        for (int k0 = s2; k0 < e2; k0 += 2 * NLT) {
+            const int ka = k0 + t, kb = k0 + NLT + t;
+            const uint4 qa = posq[min(ka, e2 - 1)], qb = posq[min(kb, e2 - 1)];
+            const bool ha = inlist(qa) && ka < e2 && ka != g, hb = inlist(qb) && kb < e2 && kb != g;
+            emit(ha, ka);
+            emit(hb, kb);
         }
+    }
     if (t == 0) {
         if (count > cap) atomicOr(&chain[r].err, 1);
         nn[g] = min(count, cap);
@@ -678,14 +698,14 @@ __global__ void __launch_bounds__(NLT * LPB) k_nlist_team(const RepC *__restrict
                                                           const int *__restrict__ lblk_rep,
                                                           const int *__restrict__ lblk_first, WorkList wl,
                                                           const uint4 *__restrict__ posq,
-                                                          const int *__restrict__ cell_of, const int *__restrict__ cnt,
-                                                          const int *__restrict__ start, int *nn, int *nl, int cap) {
+                                                          const int *__restrict__ cell_of,
+                                                          const int2 *__restrict__ ranges, int *nn, int *nl, int cap) {
     const int nit = wl.n[1];
     for (int it = blockIdx.x; it < nit; it += gridDim.x) {
         const int b = wl.item[wl.off_l + it], r = lblk_rep[b];
         const RepC R = reps[r];
         const int i = lblk_first[b] + (int)threadIdx.x / NLT;
-        if (i < R.n) nlist_team_one<NLT>(R, r, i, chain, posq, cell_of, cnt, start, nn, nl, cap);  // whole team
+        if (i < R.n) nlist_team_one<NLT>(R, r, i, chain, posq, cell_of, ranges, nn, nl, cap);  // whole team
     }
 }
 
@@ -1185,6 +1205,7 @@ struct ac4_md {
     int shuf_kcap = 0;     // largest Nact whose directions k_shuffle keeps in LDS
     size_t shuf_lds = 0;   // dynamic LDS of k_shuffle
     int *wl_item = nullptr, *wl_n = nullptr;  // rebuild work list (WorkList)
+    int2 *ranges = nullptr;                    // per cell: stencil slot ranges (NRANGE)
     int *lblk_rep = nullptr, *lblk_first = nullptr;  // list-build block table (LPB particles per block)
     double *partial = nullptr;
     PartArrays P{}, A{};  // primary and permutation scratch
@@ -1396,7 +1417,7 @@ extern "C" int ac4_md_create_slots(int nslots, const int *cap_n, int precision, 
 #define ALLOC(ptr, n) if ((rc = dalloc(&md->ptr, (size_t)(n))) != AC4_OK) { ac4_md_destroy(md); return rc; }
     ALLOC(reps, nslots) ALLOC(chain, nslots) ALLOC(blk_rep, md->nblk) ALLOC(blk_first, md->nblk) ALLOC(blk_idx, md->nblk)
     ALLOC(fblk_rep, md->fnblk) ALLOC(fblk_first, md->fnblk) ALLOC(lblk_rep, md->lnblk) ALLOC(lblk_first, md->lnblk)
-    ALLOC(wl_item, (size_t)md->nblk + md->lnblk + md->ncb) ALLOC(wl_n, 4) ALLOC(cblk_rep, off_cell) ALLOC(cblk_cell, off_cell) ALLOC(partial, 4 * (size_t)md->fnblk)
+    ALLOC(wl_item, (size_t)md->nblk + md->lnblk + md->ncb) ALLOC(wl_n, 4) ALLOC(ranges, (size_t)off_cell * NRANGE) ALLOC(cblk_rep, off_cell) ALLOC(cblk_cell, off_cell) ALLOC(partial, 4 * (size_t)md->fnblk)
     ALLOC(fx, off) ALLOC(fy, off) ALLOC(fz, off) ALLOC(xo, off) ALLOC(yo, off) ALLOC(zo, off) ALLOC(pos4, off) ALLOC(posq, off)
     ALLOC(rank, off) ALLOC(nn, off) ALLOC(nn_ref, off) ALLOC(nl, (size_t)md->cap * off) ALLOC(cnt, off_cell) ALLOC(start, off_cell)
     ALLOC(members, off) ALLOC(kx, off_act) ALLOC(ky, off_act) ALLOC(kz, off_act)
@@ -1483,7 +1504,7 @@ extern "C" int ac4_md_destroy(ac4_md *md) {
     if (md->gexec) (void)hipGraphExecDestroy(md->gexec);
     if (md->graph) (void)hipGraphDestroy(md->graph);
     void *ptrs[] = {md->reps, md->chain, md->blk_rep, md->blk_first, md->blk_idx, md->fblk_rep, md->fblk_first,
-                    md->lblk_rep, md->lblk_first, md->wl_item, md->wl_n,
+                    md->lblk_rep, md->lblk_first, md->wl_item, md->wl_n, md->ranges,
                     md->cblk_rep, md->cblk_cell, md->ox, md->oy, md->oz, md->ov_q, md->ov_q2,
                     md->ov_step, md->ov_cnt, md->ov_n,
                     md->partial, md->fx, md->fy, md->fz, md->xo, md->yo, md->zo, md->pos4, md->posq, md->rank, md->nn,
@@ -1759,7 +1780,7 @@ static int launch_rebuild(ac4_md *md, bool with_zero = false) {
     hipLaunchKernelGGL(k_scatter, dim3(gb), dim3(BLK), 0, s, md->reps, md->blk_rep, md->blk_first, wl,
                        md->P.cell_of, md->rank, md->start, md->members);
     hipLaunchKernelGGL(k_cell_sort, dim3(gc), dim3(BLK), 0, s, md->reps, wl, md->cnt, md->start, md->P.id,
-                       md->members);
+                       md->members, md->ranges);
     hipLaunchKernelGGL(k_gather, dim3(gb), dim3(BLK), 0, s, md->reps, md->blk_rep, md->blk_first, wl, md->members,
                        md->P, md->A);
     hipLaunchKernelGGL(k_commit, dim3(gb), dim3(BLK), 0, s, md->reps, md->blk_rep, md->blk_first, wl, md->A, md->P,
@@ -1772,7 +1793,7 @@ static int launch_rebuild(ac4_md *md, bool with_zero = false) {
         const int w = md->nlist_mode == 2 ? 16 : (md->nlist_mode == 3 ? 32 : 64);
         auto kern = w == 16 ? k_nlist_team<16> : (w == 32 ? k_nlist_team<32> : k_nlist_team<64>);
         hipLaunchKernelGGL(kern, dim3(gl), dim3(w * LPB), 0, s, md->reps, md->chain, md->lblk_rep, md->lblk_first, wl,
-                           md->posq, md->P.cell_of, md->cnt, md->start, md->nn, md->nl, md->cap);
+                           md->posq, md->P.cell_of, md->ranges, md->nn, md->nl, md->cap);
     }
     else if (md->nlist_mode == 0)
         hipLaunchKernelGGL(k_nlist_cell, dim3(md->ncell_tot), dim3(64), 0, s, md->reps, md->chain, md->cblk_rep,
