@@ -106,6 +106,9 @@ _SIGS = {
     "ac4_autotune": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ReplicaParams), ctypes.c_int, ctypes.c_int,
                                     ctypes.c_int, _D, _D, ctypes.c_int, _I]),
     "ac4_set_skin": (ctypes.c_int, [ctypes.c_double]),
+    "ac4_md_checkpoint_size": (ctypes.c_longlong, [_VP, ctypes.c_int]),
+    "ac4_md_checkpoint": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_void_p, ctypes.c_longlong]),
+    "ac4_md_restore": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_void_p, ctypes.c_longlong]),
     "ac4_md_create_slots": (ctypes.c_int, [ctypes.c_int, _I, ctypes.c_int, ctypes.POINTER(_VP)]),
     "ac4_md_assign": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.POINTER(ReplicaParams)]),
     "ac4_md_release": (ctypes.c_int, [_VP, ctypes.c_int]),
@@ -124,6 +127,7 @@ _SIGS = {
     "ac4_server_submit": (ctypes.c_int, [_VP, ctypes.POINTER(Job)]),
     "ac4_server_tick": (ctypes.c_int, [_VP, ctypes.c_double]),
     "ac4_server_pending": (ctypes.c_int, [_VP]),
+    "ac4_server_preempt": (ctypes.c_int, [_VP, ctypes.c_int]),
     "ac4_server_job": (ctypes.c_int, [_VP, ctypes.c_int, _D, _D, ctypes.POINTER(ctypes.c_long), _I, _I]),
     "ac4_server_result": (ctypes.c_int, [_VP, ctypes.c_int, _D, _D, _D]),
     "ac4_server_particle_steps": (ctypes.c_longlong, [_VP]),
@@ -270,6 +274,20 @@ class MD:
     def set_nve(self, r, on=True):
         """Plain velocity Verlet for replica r (ac4_md_set_nve); Heff = KE + PE."""
         _check(lib.ac4_md_set_nve(self.h, r, 1 if on else 0))
+
+    def checkpoint(self, r=0):
+        """Exact checkpoint of replica r as bytes (ac4_md_checkpoint)."""
+        size = lib.ac4_md_checkpoint_size(self.h, r)
+        if size < 0:
+            _check(int(size))
+        buf = ctypes.create_string_buffer(size)
+        _check(lib.ac4_md_checkpoint(self.h, r, buf, size))
+        return buf.raw
+
+    def restore(self, r, blob):
+        """Loads an exact checkpoint into replica r (ac4_md_restore)."""
+        buf = ctypes.create_string_buffer(blob, len(blob))
+        _check(lib.ac4_md_restore(self.h, r, buf, len(blob)))
 
     def set_rebuild_criterion(self, crit):
         _check(lib.ac4_md_set_rebuild_criterion(self.h, crit))
@@ -440,6 +458,10 @@ class Server:
 
     def pending(self):
         return lib.ac4_server_pending(self.h)
+
+    def preempt(self, jid):
+        """Checkpoint a running job to host memory and free its slot (ac4_server_preempt)."""
+        _check(lib.ac4_server_preempt(self.h, jid))
 
     def job(self, jid):
         ts, td = ctypes.c_double(), ctypes.c_double()

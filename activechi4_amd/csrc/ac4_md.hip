@@ -2113,3 +2113,129 @@ extern "C" int ac4_md_device_xu(ac4_md *md, int r, const double **xu, const doub
     *n = R.n;
     return AC4_OK;
 }
+
+// ------------------------------------------------------ exact checkpoints
+// A replica's complete device state in slot (cell) order: particle arrays,
+// forces, the last-rebuild positions, the cell tables, the neighbour list and
+// the chain.  Restoring copies it back without recomputing anything, so the
+// continued run is bitwise the one that was interrupted.  Slot and cell
+// indices are stored replica-local, so the target may be another slot, of a
+// handle with the same shape (precision, list capacity and layout, team
+// width, skin) configured with the same replica parameters.
+struct CkHead {
+    char magic[8];
+    int32_t version, n, nact, ncell, cap, lc, tpp, prec;
+    double L, skin, T, f0;
+    int32_t tau, pad;
+    uint64_t seed, chain_bytes, total;
+};
+static const char k_ck_magic[8] = {'A', 'C', '4', 'C', 'K', 'P', 'T', '1'};
+
+static size_t ck_bytes(const ac4_md *md, const RepC &R) {
+    const size_t n = R.n, nc = R.ncell;
+    return sizeof(CkHead) + sizeof(ChainS) + n * (15 * sizeof(double) + sizeof(double4) + sizeof(uint4)) +
+           n * 7 * sizeof(int) + 3 * (size_t)R.nact + nc * 2 * sizeof(int) + n * sizeof(int) +
+           nc * NRANGE * sizeof(int2) + (size_t)md->cap * n * sizeof(int);
+}
+
+extern "C" long long ac4_md_checkpoint_size(ac4_md *md, int r) {
+    if (check_rep(md, r) != AC4_OK) return AC4_EARG;
+    return (long long)ck_bytes(md, md->reps_h[r]);
+}
+
+template <class T>
+static int ck_io(bool save, T *dev, size_t n, char *&p, hipStream_t s) {
+    if (save)
+        TRY(d2h((T *)p, (const T *)dev, n, s));
+    else
+        TRY(h2d(dev, (const T *)p, n, s));
+    p += n * sizeof(T);
+    return AC4_OK;
+}
+
+static int ck_arrays(ac4_md *md, const RepC &R, bool save, char *p) {
+    hipStream_t s = md->stream;
+    const size_t g = R.off, n = R.n, c0 = R.off_cell, nc = R.ncell;
+    double *dv[15] = {md->P.x, md->P.y, md->P.z, md->P.xu, md->P.yu, md->P.zu, md->P.px, md->P.py, md->P.pz,
+                      md->xo, md->yo, md->zo, md->fx, md->fy, md->fz};
+    for (double *a : dv) TRY(ck_io(save, a + g, n, p, s));
+    TRY(ck_io(save, md->pos4 + g, n, p, s));
+    TRY(ck_io(save, md->posq + g, n, p, s));
+    int *iv[7] = {md->P.type, md->P.act, md->P.id, md->P.cell_ref, md->P.cell_of, md->nn, md->nn_ref};
+    for (int *a : iv) TRY(ck_io(save, a + g, n, p, s));
+    TRY(ck_io(save, md->kx + R.off_act, (size_t)R.nact, p, s));
+    TRY(ck_io(save, md->ky + R.off_act, (size_t)R.nact, p, s));
+    TRY(ck_io(save, md->kz + R.off_act, (size_t)R.nact, p, s));
+    TRY(ck_io(save, md->cnt + c0, nc, p, s));
+    TRY(ck_io(save, md->start + c0, nc, p, s));
+    TRY(ck_io(save, md->members + g, n, p, s));
+    int2 *rg = (int2 *)p;
+    TRY(ck_io(save, md->ranges + c0 * NRANGE, nc * NRANGE, p, s));
+    int *nl = (int *)p;
+    TRY(ck_io(save, md->nl + g * md->cap, (size_t)md->cap * n, p, s));
+    AC4_HIP(hipStreamSynchronize(s));
+    if (save) {  // replica-local slot indices
+        for (size_t k = 0; k < nc * NRANGE; k++)
+            if (rg[k].y > rg[k].x) { rg[k].x -= (int)g; rg[k].y -= (int)g; }
+        for (size_t k = 0; k < (size_t)md->cap * n; k++) nl[k] -= (int)g;
+    }
+    return AC4_OK;
+}
+
+extern "C" int ac4_md_checkpoint(ac4_md *md, int r, void *buf, long long cap) {
+    TRY(check_rep(md, r));
+    const RepC &R = md->reps_h[r];
+    AC4_REQUIRE(buf && cap >= (long long)ck_bytes(md, R), AC4_EARG, "checkpoint: buffer smaller than %zu bytes",
+                ck_bytes(md, R));
+    AC4_REQUIRE(R.n > 0, AC4_ESTATE, "checkpoint: slot %d is idle", r);
+    AC4_HIP(hipStreamSynchronize(md->stream));
+    CkHead h;
+    memset(&h, 0, sizeof h);
+    memcpy(h.magic, k_ck_magic, 8);
+    h.version = 1; h.n = R.n; h.nact = R.nact; h.ncell = R.ncell; h.cap = md->cap; h.lc = R.lc; h.tpp = R.tpp;
+    h.prec = md->prec; h.L = R.L; h.skin = R.skin; h.T = R.T; h.f0 = R.f0; h.tau = R.tau;
+    h.seed = (uint64_t)R.key0 | ((uint64_t)R.key1 << 32);
+    h.chain_bytes = sizeof(ChainS);
+    h.total = ck_bytes(md, R);
+    char *p = (char *)buf;
+    memcpy(p, &h, sizeof h);
+    p += sizeof h;
+    ChainS c;
+    TRY(get_chain(md, r, &c));
+    memcpy(p, &c, sizeof c);
+    p += sizeof c;
+    return ck_arrays(md, R, true, p);
+}
+
+extern "C" int ac4_md_restore(ac4_md *md, int r, const void *buf, long long len) {
+    TRY(check_rep(md, r));
+    const RepC &R = md->reps_h[r];
+    AC4_REQUIRE(buf && len >= (long long)sizeof(CkHead), AC4_EARG, "restore: no checkpoint");
+    CkHead h;
+    memcpy(&h, buf, sizeof h);
+    AC4_REQUIRE(!memcmp(h.magic, k_ck_magic, 8) && h.version == 1 && h.chain_bytes == sizeof(ChainS), AC4_EARG,
+                "restore: not a checkpoint of this library version");
+    AC4_REQUIRE((long long)h.total == len && h.total == ck_bytes(md, R), AC4_EARG,
+                "restore: checkpoint of %lld bytes does not fit slot %d", len, r);
+    AC4_REQUIRE(h.n == R.n && h.nact == R.nact && h.ncell == R.ncell && h.cap == md->cap && h.lc == R.lc &&
+                    h.tpp == R.tpp && h.prec == md->prec && h.L == R.L && h.skin == R.skin,
+                AC4_EARG, "restore: the slot's replica or handle shape differs from the checkpoint's");
+    std::vector<char> tmp((const char *)buf, (const char *)buf + len);  // local copy: indices are rebased
+    char *p = tmp.data() + sizeof h;
+    ChainS c;
+    memcpy(&c, p, sizeof c);
+    p += sizeof c;
+    const size_t g = R.off, n = R.n, nc = R.ncell;
+    // rebase the replica-local indices onto this slot before copying
+    char *q = p + n * (15 * sizeof(double) + sizeof(double4) + sizeof(uint4)) + n * 7 * sizeof(int) +
+              3 * (size_t)R.nact + nc * 2 * sizeof(int) + n * sizeof(int);
+    int2 *rg = (int2 *)q;
+    for (size_t k = 0; k < nc * NRANGE; k++)
+        if (rg[k].y > rg[k].x) { rg[k].x += (int)g; rg[k].y += (int)g; }
+    int *nl = (int *)(q + nc * NRANGE * sizeof(int2));
+    for (size_t k = 0; k < (size_t)md->cap * n; k++) nl[k] += (int)g;
+    TRY(ck_arrays(md, R, false, p));
+    c.done_blocks = 0u;
+    c.bin_done = 0u;
+    return put_chain(md, r, &c);
+}

@@ -25,10 +25,12 @@ int ac4_md_load_slot(ac4_md *md, int r, const double *x, const double *y, const 
 typedef struct {
     ac4_job job;
     int state; /* 0 queued, 1 running, 2 done */
-    int slot, frames;
+    int slot, frames, preemptions;
     long steps_done, target;
     double t_start, t_done;
-    double *result; /* final x_u,y_u,z_u (3n) */
+    double *result;  /* final x_u,y_u,z_u (3n) */
+    char *ckpt;      /* exact checkpoint of a preempted job, resumed on admission */
+    long long ckpt_len;
 } job_rec;
 
 struct ac4_server {
@@ -68,7 +70,7 @@ int ac4_server_create(int nslots, const int *slot_cap, int precision, int chunk,
 int ac4_server_destroy(ac4_server *s) {
     if (!s) return AC4_OK;
     ac4_md_destroy(s->md);
-    for (int j = 0; j < s->njobs; j++) free(s->jobs[j].result);
+    for (int j = 0; j < s->njobs; j++) { free(s->jobs[j].result); free(s->jobs[j].ckpt); }
     free(s->jobs); free(s->cap); free(s->slot_job); free(s->hx); free(s->hy); free(s->hz);
     free(s);
     return AC4_OK;
@@ -101,6 +103,16 @@ static int admit(ac4_server *s, int j, int slot, double now) {
     ac4_replica_params p = {q->n, 1.2, q->T, q->f0, -1, tau, q->seed};
     int rc = ac4_md_assign(s->md, slot, &p);
     if (rc != AC4_OK) return rc;
+    if (r->ckpt) { /* a preempted job continues exactly where it stopped */
+        rc = ac4_md_restore(s->md, slot, r->ckpt, r->ckpt_len);
+        if (rc != AC4_OK) return rc;
+        free(r->ckpt);
+        r->ckpt = NULL;
+        r->state = 1;
+        r->slot = slot;
+        s->slot_job[slot] = j;
+        return AC4_OK;
+    }
     int n = q->n, nact = (int)((double)n * 0.1000);
     double *buf = malloc(sizeof(double) * 9 * (size_t)n);
     int *ib = malloc(sizeof(int) * (2 * (size_t)n + 3 * (size_t)(nact + 1)));
@@ -133,8 +145,12 @@ int ac4_server_tick(ac4_server *s, double now) {
      * fixed when the server is created and every kernel sums in a fixed
      * order, so a job's arithmetic depends on its own parameters and the
      * server's configuration only, never on which other jobs are running. */
+    /* two passes: jobs that have not run yet, then preempted jobs (a
+     * preemption yields the slot to the work that was waiting) */
+    for (int pass = 0; pass < 2; pass++)
     for (int j = 0; j < s->njobs; j++) {
         if (s->jobs[j].state != 0 || s->jobs[j].job.t_arrive > now) continue;
+        if ((s->jobs[j].ckpt != NULL) != (pass == 1)) continue;
         int cls = 0;
         for (int k = 0; k < s->nslots; k++)
             if (s->cap[k] >= s->jobs[j].job.n && (cls == 0 || s->cap[k] < cls)) cls = s->cap[k];
@@ -173,6 +189,25 @@ int ac4_server_tick(ac4_server *s, double now) {
         }
     }
     return running;
+}
+
+int ac4_server_preempt(ac4_server *s, int id) {
+    if (!s || id < 0 || id >= s->njobs) { ac4_set_error("preempt: bad id"); return AC4_EARG; }
+    job_rec *r = &s->jobs[id];
+    if (r->state != 1) { ac4_set_error("preempt: job %d is not running", id); return AC4_ESTATE; }
+    long long len = ac4_md_checkpoint_size(s->md, r->slot);
+    if (len < 0) return (int)len;
+    r->ckpt = malloc((size_t)len);
+    if (!r->ckpt) { ac4_set_error("preempt: cannot allocate %lld bytes", len); return AC4_ECAP; }
+    int rc = ac4_md_checkpoint(s->md, r->slot, r->ckpt, len);
+    if (rc != AC4_OK) { free(r->ckpt); r->ckpt = NULL; return rc; }
+    r->ckpt_len = len;
+    s->slot_job[r->slot] = -1;
+    rc = ac4_md_release(s->md, r->slot);
+    r->slot = -1;
+    r->state = 0; /* queued again; its arrival time keeps its place in the FIFO */
+    r->preemptions++;
+    return rc;
 }
 
 int ac4_server_pending(const ac4_server *s) {

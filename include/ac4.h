@@ -182,6 +182,20 @@ int ac4_frames_s4_shells(ac4_frames *f, int npairs, const int *orig, const int *
                          double dq, int nshell, int ndir, uint64_t seed, double *s4_ref, double *s4_dir,
                          double *phases);
 
+/* ------------------------------------------------------ exact checkpoints --
+ * A superset of MDcode's restart_file.dat (main.c:272-334), which keeps only
+ * positions, velocities and tags: the complete device state of replica r
+ * (particles in cell order, forces, last-rebuild positions, cell tables,
+ * neighbour list, active directions, Nose-Hoover chain and step/shuffle
+ * counters), so a run resumed from it is bitwise the uninterrupted run (the
+ * reference's restart is not exact: it redraws the chain momenta).  Restore
+ * into a slot configured with the same replica parameters in a handle of the
+ * same shape (precision, team width, skin); other replicas of the handle are
+ * not touched.  Sizes in bytes. */
+long long ac4_md_checkpoint_size(ac4_md *md, int r);
+int ac4_md_checkpoint(ac4_md *md, int r, void *buf, long long cap);
+int ac4_md_restore(ac4_md *md, int r, const void *buf, long long len);
+
 /* ------------------------------------------------------ slots (serving) --
  * A handle can be created with idle slots of fixed capacity; jobs are
  * assigned to slots between launches and released when done.  No reference
@@ -242,6 +256,11 @@ int ac4_server_submit(ac4_server *s, const ac4_job *job);
  * Returns the number of jobs that ran in the chunk. */
 int ac4_server_tick(ac4_server *s, double now);
 int ac4_server_pending(const ac4_server *s);
+/* Preemption (no reference counterpart): takes running job `id` off the GPU
+ * with an exact checkpoint (ac4_md_checkpoint) held in host memory and frees
+ * its slot; the job is queued again behind the jobs that have not started
+ * and, when admitted, continues bitwise as if it had never been interrupted. */
+int ac4_server_preempt(ac4_server *s, int id);
 /* state: 0 queued, 1 running, 2 done. */
 int ac4_server_job(const ac4_server *s, int id, double *t_start, double *t_done,
                    long *steps_done, int *frames, int *state);

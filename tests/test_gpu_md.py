@@ -42,7 +42,9 @@ def relaxed_state(ac4, n, seed, T=0.6, f0=1.5):
     reference does with coldstart/breakxal before Nose-Hoover dynamics."""
     key = (n, seed, T, f0)
     if key not in _RELAXED:
-        s, L = lattice_state(n, T=T, seed=seed)
+        # unjittered sites (spacing ~0.9 sigma): a jittered lattice can put a
+        # pair close enough for the first kick to blow the run up
+        s, L = lattice_state(n, T=T, seed=seed, jitter=0.0)
         md = make_md(ac4, [(s, L)], f0=f0, T=T)
         md.relax(300, 1.5, 1.0)
         g = md.get_state(0)
@@ -231,6 +233,33 @@ def test_list_builders_give_identical_trajectories(gpu, prec):
                 assert np.array_equal(o[r][k], out[0][r][k]), (r, k)
 
 
+@pytest.mark.parametrize("prec", [0, 1])
+def test_checkpoint_restore_is_exact(gpu, prec):
+    """A replica checkpointed mid-run and restored into another slot of
+    another handle continues bitwise as the uninterrupted run; a checkpoint
+    does not load into a slot of a different shape."""
+    with tuning(gpu, 4):
+        a = make_relaxed_md(gpu, [(1500, 81, 3), (2200, 82, 4)], tau=20, prec=prec)
+        b = make_relaxed_md(gpu, [(2200, 82, 4)], tau=20, prec=prec)
+        c = make_relaxed_md(gpu, [(2000, 83, 5)], tau=20, prec=prec)
+    a.step(150, graph=True)
+    blob = a.checkpoint(1)
+    a.step(170, graph=True)
+    a.sync()
+    b.restore(0, blob)
+    b.step(170, graph=True)
+    b.sync()
+    ga, gb = a.get_state(1), b.get_state(0)
+    for k in ("x", "xu", "yu", "zu", "px", "py", "pz", "kx"):
+        assert np.array_equal(ga[k], gb[k]), k
+    assert np.array_equal(a.get_chain(1), b.get_chain(0))
+    sa, sb = a.stats(1), b.stats(0)
+    assert sa["Heff"] == sb["Heff"] and sa["steps"] == sb["steps"] == 320
+    assert a.rebuilds(1) == b.rebuilds(0) > 0
+    with pytest.raises(gpu.AC4Error):
+        c.restore(0, blob)
+
+
 def test_graph_replay_equals_eager(gpu):
     spec = [(1200, 41, 1), (2000, 42, 2)]
     e = make_relaxed_md(gpu, spec, tau=25, prec=1)
@@ -284,6 +313,35 @@ def test_server_job_alone_equals_in_mix(gpu):
             assert np.array_equal(a[u], m[u]), (k, u)
         alone.close()
     assert mix.particle_steps() > 0
+
+
+def test_server_preempted_job_resumes_exactly(gpu):
+    """A job taken off the GPU mid-run (exact checkpoint in host memory),
+    whose slot then serves another job, finishes bitwise as when it runs
+    uninterrupted."""
+    caps = [2048, 4096]
+    a_job = (1500, 0.5, 1.0, 1.0, 700, 21, 0)
+    alone, aid = _run_server(gpu, caps, [a_job])
+    ref = alone.result(aid[0])
+    alone.close()
+    srv = gpu.Server(caps, precision=gpu.FP32, chunk=100, relax=200)
+    ia = srv.submit(*a_job)
+    srv.tick(0.0)
+    srv.tick(1.0)
+    srv.tick(2.0)  # A has run 300 steps (relaxation ramp + chain)
+    srv.preempt(ia)
+    assert srv.job(ia)["state"] == 0
+    ib = srv.submit(1800, 0.6, 0.5, 0.1, 500, 22, 3.0)  # takes the freed 2048 slot
+    t = 3.0
+    while srv.pending() and t < 200:
+        srv.tick(t)
+        t += 1.0
+    assert srv.job(ia)["state"] == 2 and srv.job(ib)["state"] == 2
+    assert srv.job(ib)["t_start"] < srv.job(ia)["t_done"]
+    got = srv.result(ia)
+    for u in range(3):
+        assert np.array_equal(got[u], ref[u]), u
+    srv.close()
 
 
 def test_insitu_overlap_accumulators(gpu):
