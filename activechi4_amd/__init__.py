@@ -106,6 +106,8 @@ _SIGS = {
     "ac4_autotune": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ReplicaParams), ctypes.c_int, ctypes.c_int,
                                     ctypes.c_int, _D, _D, ctypes.c_int, _I]),
     "ac4_set_skin": (ctypes.c_int, [ctypes.c_double]),
+    "ac4_format_frame": (ctypes.c_longlong, [ctypes.c_int, _D, _D, _D, _I, ctypes.c_char_p, ctypes.c_longlong]),
+    "ac4_parse_frame": (ctypes.c_longlong, [ctypes.c_char_p, ctypes.c_longlong, ctypes.c_int, _D, _D, _D, _I]),
     "ac4_md_checkpoint_size": (ctypes.c_longlong, [_VP, ctypes.c_int]),
     "ac4_md_checkpoint": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_void_p, ctypes.c_longlong]),
     "ac4_md_restore": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_void_p, ctypes.c_longlong]),
@@ -559,6 +561,28 @@ def autotune(params, precision=FP32, relax=300, steps=200):
     _check(lib.ac4_autotune(len(params), arr, precision, relax, steps, _dp(best), _dp(table), 32,
                             ctypes.byref(nc)))
     return (int(best[0]), int(best[1]), float(best[2])), table[: nc.value]
+
+
+def format_frame(x, y, z, typ):
+    """data_coor.out text of one frame (ac4_format_frame), as bytes."""
+    load()
+    x, y, z, t = _f64(x), _f64(y), _f64(z), _i32(typ)
+    buf = ctypes.create_string_buffer(80 * len(x) + 1)
+    k = lib.ac4_format_frame(len(x), _dp(x), _dp(y), _dp(z), _ip(t), buf, 80 * len(x) + 1)
+    if k < 0:
+        _check(int(k))
+    return buf.raw[:k]
+
+
+def parse_frame(text, n):
+    """Inverse of format_frame (ac4_parse_frame): (x, y, z, type, bytes consumed)."""
+    load()
+    x, y, z = np.empty(n), np.empty(n), np.empty(n)
+    t = np.empty(n, dtype=np.int32)
+    k = lib.ac4_parse_frame(text, len(text), n, _dp(x), _dp(y), _dp(z), _ip(t))
+    if k < 0:
+        _check(int(k))
+    return x, y, z, t, k
 
 
 def set_skin(skin=0.35):

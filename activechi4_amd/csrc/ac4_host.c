@@ -396,3 +396,112 @@ void ac4_chain_init(int n, double T, uint64_t seed, double *c9) {
         c9[3 + k] = sqrt(T * c9[6 + k]) * sqrt(-2.0 * log(1.0 - u[0])) * cos(2.0 * 3.14159265358979 * u[1]);
     }
 }
+
+/* ---------------------------------------------------- trajectory text I/O
+ * data_coor.out lines are "%.5lf\t%.5lf\t%.5lf\t%d\n" (MDcode/print.c:4-16).
+ * fmt5 prints a double exactly as glibc's %.5lf: the value m*2^e is
+ * multiplied by 10^5 in 128-bit integers and rounded to nearest with ties to
+ * even (the correctly rounded decimal that printf produces), the sign kept
+ * for values that round to zero.  Non-finite or huge values (never in a
+ * trajectory) go through snprintf. */
+static char *put_u64(char *o, uint64_t v) {
+    char t[24];
+    int k = 0;
+    do { t[k++] = (char)('0' + v % 10); v /= 10; } while (v);
+    while (k) *o++ = t[--k];
+    return o;
+}
+
+static char *fmt5(char *o, double v) {
+    uint64_t bits;
+    memcpy(&bits, &v, sizeof bits);
+    const int neg = (int)(bits >> 63), ex = (int)((bits >> 52) & 0x7ff);
+    const uint64_t frac = bits & ((1ull << 52) - 1);
+    if (ex == 0x7ff || fabs(v) >= 1e12) return o + sprintf(o, "%.5lf", v);
+    const uint64_t m = ex ? (frac | (1ull << 52)) : frac;
+    const int e = ex ? ex - 1075 : -1074; /* v = m * 2^e */
+    unsigned __int128 q = 0;
+    if (m != 0) {
+        const unsigned __int128 t = (unsigned __int128)m * 100000u; /* < 2^70 */
+        if (e >= 0) {
+            q = t << e; /* |v| < 1e12: e <= 0 in practice */
+        } else if (-e <= 71) {
+            const int sh = -e;
+            const unsigned __int128 one = 1;
+            const unsigned __int128 rem = t & ((one << sh) - 1), half = one << (sh - 1);
+            q = t >> sh;
+            if (rem > half || (rem == half && (q & 1))) q++;
+        } /* else t / 2^sh < 1/2: rounds to 0 */
+    }
+    if (neg) *o++ = '-';
+    o = put_u64(o, (uint64_t)(q / 100000u));
+    *o++ = '.';
+    uint64_t fp = (uint64_t)(q % 100000u);
+    for (int d = 4; d >= 0; d--) { o[d] = (char)('0' + fp % 10); fp /= 10; }
+    return o + 5;
+}
+
+long long ac4_format_frame(int n, const double *x, const double *y, const double *z, const int *type, char *out,
+                           long long cap) {
+    if (n < 0 || !x || !y || !z || !type || !out || cap < 80LL * n) {
+        ac4_set_error("format_frame: bad arguments (buffer needs 80 bytes a line)");
+        return AC4_EARG;
+    }
+    char *o = out;
+    for (int i = 0; i < n; i++) {
+        o = fmt5(o, x[i]);
+        *o++ = '\t';
+        o = fmt5(o, y[i]);
+        *o++ = '\t';
+        o = fmt5(o, z[i]);
+        *o++ = '\t';
+        if (type[i] < 0) { *o++ = '-'; o = put_u64(o, (uint64_t)(-(long long)type[i])); }
+        else o = put_u64(o, (uint64_t)type[i]);
+        *o++ = '\n';
+    }
+    return (long long)(o - out);
+}
+
+/* One number of a data_coor.out line: "[-]digits.digits" with at most 15
+ * significant digits is D / 10^f with D < 2^53, and (double)D / 10^f is one
+ * correctly rounded division, i.e. the value strtod returns; anything else
+ * (exponents, long mantissas) goes through strtod. */
+static const char *parse_num(const char *p, double *out) {
+    static const double p10[16] = {1e0, 1e1, 1e2, 1e3, 1e4, 1e5, 1e6, 1e7, 1e8, 1e9, 1e10, 1e11, 1e12, 1e13, 1e14, 1e15};
+    const char *s = p;
+    while (*s == ' ' || *s == '\t') s++;
+    const char *st = s;
+    int neg = 0, nd = 0, nf = 0;
+    if (*s == '-' || *s == '+') { neg = *s == '-'; s++; }
+    uint64_t d = 0;
+    while (*s >= '0' && *s <= '9' && nd < 16) { d = d * 10 + (uint64_t)(*s - '0'); s++; nd++; }
+    if (*s == '.') {
+        s++;
+        while (*s >= '0' && *s <= '9' && nd < 16) { d = d * 10 + (uint64_t)(*s - '0'); s++; nd++; nf++; }
+    }
+    if (nd == 0 || nd > 15 || (*s >= '0' && *s <= '9') || *s == 'e' || *s == 'E') {
+        char *e;
+        *out = strtod(st, &e);
+        return e;
+    }
+    const double v = (double)d / p10[nf];
+    *out = neg ? -v : v;
+    return s;
+}
+
+long long ac4_parse_frame(const char *text, long long len, int n, double *x, double *y, double *z, int *type) {
+    if (!text || n < 0 || !x || !y || !z || !type) { ac4_set_error("parse_frame: bad arguments"); return AC4_EARG; }
+    const char *p = text, *end = text + len;
+    for (int i = 0; i < n; i++) {
+        if (p >= end) { ac4_set_error("parse_frame: text ends at line %d", i); return AC4_EIO; }
+        p = parse_num(p, &x[i]);
+        p = parse_num(p, &y[i]);
+        p = parse_num(p, &z[i]);
+        char *e;
+        type[i] = (int)strtol(p, &e, 10);
+        p = e;
+        while (p < end && *p != '\n') p++;
+        if (p < end) p++;
+    }
+    return (long long)(p - text);
+}

@@ -7,11 +7,15 @@
  * Analysis/main.c:53-59 calls), plus Fsqt_a_0.30.dat when AC4_FS=1.
  * N is compile-time in the reference (Analysis/global.h:11, 50000); here it
  * is the optional 4th argument.  AC4_DIR overrides the job directory.
+ * AC4_BIN=1 reads the frames from data_coor.bin (written by run with
+ * AC4_BIN=1: the exact values data_coor.out parses to, so every output file
+ * is the same) instead of parsing the text.
  */
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <stdint.h>
 #include "ac4.h"
 
 static void die(const char *what, int rc) {
@@ -56,21 +60,63 @@ int main(int argc, char **argv) {
     CK(ac4_ana_create(n, L, nframes, fidx, 0, 1.4, 0.3, obs, 0, &an));
     double *x = malloc(sizeof(double) * n), *y = malloc(sizeof(double) * n), *z = malloc(sizeof(double) * n);
     int *ty = malloc(sizeof(int) * n);
-    snprintf(path, sizeof path, "%s/data_coor.out", sub_dir);
-    f = fopen(path, "rb");
-    if (!f) { fprintf(stderr, "DATA FILE DOES NOT EXIST %s\n", path); return 1; }
-    for (int fr = 0; fr < nframes; fr++) {
-        for (int i = 0; i < n; i++) {
-            if (!fgets(buff, sizeof buff, f)) { fprintf(stderr, "analyse: data ends in frame %d\n", fr); return 1; }
-            char *e;
-            x[i] = strtod(buff, &e);
-            y[i] = strtod(e, &e);
-            z[i] = strtod(e, &e);
-            ty[i] = (int)strtol(e, &e, 10);
+    if (getenv("AC4_BIN") && atoi(getenv("AC4_BIN"))) {
+        snprintf(path, sizeof path, "%s/data_coor.bin", sub_dir);
+        f = fopen(path, "rb");
+        if (!f) { fprintf(stderr, "DATA FILE DOES NOT EXIST %s\n", path); return 1; }
+        char magic[8];
+        int32_t hd[2];
+        if (fread(magic, 1, 8, f) != 8 || memcmp(magic, "AC4FRM01", 8) || fread(hd, sizeof hd, 1, f) != 1 ||
+            hd[0] != n) {
+            fprintf(stderr, "analyse: %s is not a frame file of N=%d\n", path, n);
+            return 1;
         }
-        CK(ac4_ana_push_frame(an, fr, x, y, z, ty));
+        for (int fr = 0; fr < nframes; fr++) {
+            uint64_t st;
+            if (fread(&st, sizeof st, 1, f) != 1 || fread(x, sizeof(double), n, f) != (size_t)n ||
+                fread(y, sizeof(double), n, f) != (size_t)n || fread(z, sizeof(double), n, f) != (size_t)n ||
+                fread(ty, sizeof(int), n, f) != (size_t)n) {
+                fprintf(stderr, "analyse: data ends in frame %d\n", fr);
+                return 1;
+            }
+            CK(ac4_ana_push_frame(an, fr, x, y, z, ty));
+        }
+        fclose(f);
+    } else {
+        /* read_all_data.c:61-91: n lines per frame, parsed by ac4_parse_frame
+         * (values identical to fscanf's) from a large buffer */
+        snprintf(path, sizeof path, "%s/data_coor.out", sub_dir);
+        f = fopen(path, "rb");
+        if (!f) { fprintf(stderr, "DATA FILE DOES NOT EXIST %s\n", path); return 1; }
+        size_t cap = (size_t)n * 96 + (1u << 20), have = 0, pos = 0;
+        char *buf = malloc(cap);
+        for (int fr = 0; fr < nframes; fr++) {
+            for (;;) { /* enough text for n lines? */
+                size_t k = pos;
+                int lines = 0;
+                while (lines < n) {
+                    const char *nl = memchr(buf + k, '\n', have - k);
+                    if (!nl) break;
+                    k = (size_t)(nl - buf) + 1;
+                    lines++;
+                }
+                if (lines == n) break;
+                memmove(buf, buf + pos, have - pos);
+                have -= pos;
+                pos = 0;
+                if (have == cap) { cap *= 2; buf = realloc(buf, cap); }
+                const size_t got = fread(buf + have, 1, cap - have, f);
+                if (got == 0) { fprintf(stderr, "analyse: data ends in frame %d\n", fr); return 1; }
+                have += got;
+            }
+            const long long used = ac4_parse_frame(buf + pos, (long long)(have - pos), n, x, y, z, ty);
+            if (used < 0) die("ac4_parse_frame", (int)used);
+            pos += (size_t)used;
+            CK(ac4_ana_push_frame(an, fr, x, y, z, ty));
+        }
+        free(buf);
+        fclose(f);
     }
-    fclose(f);
     CK(ac4_ana_finalize(an));
     CK(ac4_ana_write(an, sub_dir));
     printf("Overlap Calculated\n");

@@ -7,8 +7,18 @@
  * formats.  All particle work runs in libac4.so on the GPU.
  *
  * N is compile-time in the reference (MDcode/global.h:15, 50000); here it is
- * the optional 4th argument (default 50000).  Environment: AC4_SEED (RNG key,
- * default derived from jobno), AC4_FP32=1 (FP32 pair forces).
+ * the optional 4th argument (default 50000).  Environment:
+ *   AC4_SEED           RNG key (default derived from jobno)
+ *   AC4_FP32=1         FP32 pair forces
+ *   AC4_BIN=1          also write data_coor.bin: per frame the step, the
+ *                      values the text parses to (x_u, y_u, z_u) and types
+ *   AC4_RESUME=1       with restart_run=1: continue the interrupted run
+ *                      exactly from restart_file.ac4 (the reference's
+ *                      restart_run starts the production again from step 0
+ *                      with a new chain, MDcode/main.c:219-236)
+ *   AC4_RESTART_EVERY  production steps between restart files (100000)
+ *   AC4_STOP_AT        end the run after the restart written at this step
+ *                      (for time-limited batch jobs; resume with AC4_RESUME)
  */
 #include <errno.h>
 #include <math.h>
@@ -91,12 +101,80 @@ static void print_stuff(ac4_md *md, const char *file, int n) { /* MDcode/print.c
     fclose(f);
 }
 
-static void print_data(ac4_md *md, state_t *S, const char *file) { /* MDcode/print.c:4-16 */
+/* frame text buffer and the sidecar's parsed copy */
+static char *g_txt;
+static double *g_bx, *g_by, *g_bz;
+static int *g_bt;
+
+static void print_data(ac4_md *md, state_t *S, const char *file, const char *bin_file, long step) {
+    /* MDcode/print.c:4-16, one formatted block and one write per frame */
     CK(ac4_md_get_state(md, 0, NULL, NULL, NULL, S->xu, S->yu, S->zu, NULL, NULL, NULL, NULL, NULL, NULL));
+    const long long cap = 80LL * S->n;
+    if (!g_txt) g_txt = malloc((size_t)cap);
+    const long long k = ac4_format_frame(S->n, S->xu, S->yu, S->zu, S->type, g_txt, cap);
+    if (k < 0) die("ac4_format_frame", (int)k);
     FILE *f = fopen(file, "ab");
     if (!f) return;
-    for (int i = 0; i < S->n; i++) fprintf(f, "%.5lf\t%.5lf\t%.5lf\t%d\n", S->xu[i], S->yu[i], S->zu[i], S->type[i]);
+    fwrite(g_txt, 1, (size_t)k, f);
     fclose(f);
+    if (!bin_file) return;
+    if (!g_bx) {
+        g_bx = malloc(sizeof(double) * S->n); g_by = malloc(sizeof(double) * S->n);
+        g_bz = malloc(sizeof(double) * S->n); g_bt = malloc(sizeof(int) * S->n);
+    }
+    if (ac4_parse_frame(g_txt, k, S->n, g_bx, g_by, g_bz, g_bt) != k) die("ac4_parse_frame", AC4_EIO);
+    f = fopen(bin_file, "ab");
+    if (!f) return;
+    if (ftell(f) == 0) { /* header: magic, N */
+        const char magic[8] = {'A', 'C', '4', 'F', 'R', 'M', '0', '1'};
+        const int32_t hd[2] = {S->n, 0};
+        fwrite(magic, 1, 8, f);
+        fwrite(hd, sizeof hd, 1, f);
+    }
+    const uint64_t st = (uint64_t)step;
+    fwrite(&st, sizeof st, 1, f);
+    fwrite(g_bx, sizeof(double), (size_t)S->n, f);
+    fwrite(g_by, sizeof(double), (size_t)S->n, f);
+    fwrite(g_bz, sizeof(double), (size_t)S->n, f);
+    fwrite(g_bt, sizeof(int), (size_t)S->n, f);
+    fclose(f);
+}
+
+/* restart_file.ac4: the loop position plus an exact device checkpoint */
+static void write_exact(ac4_md *md, const char *file, long next, int index) {
+    long long len = ac4_md_checkpoint_size(md, 0);
+    if (len < 0) die("ac4_md_checkpoint_size", (int)len);
+    char *buf = malloc((size_t)len);
+    CK(ac4_md_checkpoint(md, 0, buf, len));
+    FILE *f = fopen(file, "wb");
+    if (!f) { free(buf); return; }
+    const char magic[8] = {'A', 'C', '4', 'R', 'U', 'N', '0', '1'};
+    const int64_t hd[3] = {next, index, len};
+    fwrite(magic, 1, 8, f);
+    fwrite(hd, sizeof hd, 1, f);
+    fwrite(buf, 1, (size_t)len, f);
+    fclose(f);
+    free(buf);
+}
+
+static int read_exact(ac4_md *md, const char *file, long *next, int *index) {
+    FILE *f = fopen(file, "rb");
+    if (!f) return -1;
+    char magic[8];
+    int64_t hd[3];
+    if (fread(magic, 1, 8, f) != 8 || memcmp(magic, "AC4RUN01", 8) || fread(hd, sizeof hd, 1, f) != 1) {
+        fclose(f);
+        return -1;
+    }
+    char *buf = malloc((size_t)hd[2]);
+    const int ok = fread(buf, 1, (size_t)hd[2], f) == (size_t)hd[2];
+    fclose(f);
+    if (!ok) { free(buf); return -1; }
+    CK(ac4_md_restore(md, 0, buf, hd[2]));
+    free(buf);
+    *next = (long)hd[0];
+    *index = (int)hd[1];
+    return 0;
 }
 
 static void write_restart(ac4_md *md, state_t *S, const char *file, double L, int iter, int last) {
@@ -175,6 +253,14 @@ int main(int argc, char **argv) {
     snprintf(data_file, sizeof data_file, "%s/data_coor.out", sub_dir);
     snprintf(stuff_file, sizeof stuff_file, "%s/stuff.dat", sub_dir);
     snprintf(restart_file, sizeof restart_file, "%s/restart_file.dat", sub_dir);
+    char exact_file[320], bin_path[320];
+    snprintf(exact_file, sizeof exact_file, "%s/restart_file.ac4", sub_dir);
+    snprintf(bin_path, sizeof bin_path, "%s/data_coor.bin", sub_dir);
+    const char *bin_file = getenv("AC4_BIN") && atoi(getenv("AC4_BIN")) ? bin_path : NULL;
+    const int resume = restart_run && getenv("AC4_RESUME") && atoi(getenv("AC4_RESUME"));
+    const long every = getenv("AC4_RESTART_EVERY") ? atol(getenv("AC4_RESTART_EVERY")) : 100000L;
+    const long stop_at = getenv("AC4_STOP_AT") ? atol(getenv("AC4_STOP_AT")) : -1L;
+    if (every < 1) { fprintf(stderr, "run: AC4_RESTART_EVERY must be positive\n"); return 2; }
     const char *es = getenv("AC4_SEED");
     uint64_t seed = es ? strtoull(es, NULL, 0) : (uint64_t)(0x5eed0000ull + (uint64_t)jobno);
     int prec = getenv("AC4_FP32") && atoi(getenv("AC4_FP32")) ? AC4_FP32 : AC4_FP64;
@@ -273,26 +359,48 @@ int main(int argc, char **argv) {
     for (int i = 0; i < nfr; i++) fprintf(tf, "%llu\n", (unsigned long long)next_idx[i]);
     fclose(tf);
     /* run, MDcode/main.c:208-270 */
-    fix_drift(&S);
-    push(md, &S);
-    init_chain(md, n, T, &gs);
+    long i0 = 0;
+    int index = 0;
+    if (resume) {
+        if (read_exact(md, exact_file, &i0, &index)) {
+            fprintf(stderr, "run: cannot resume from %s\n", exact_file);
+            return 1;
+        }
+        printf("Run resumed exactly at step %ld\n", i0);
+    } else {
+        fix_drift(&S);
+        push(md, &S);
+        init_chain(md, n, T, &gs);
+    }
     snprintf(path, sizeof path, "%s/ActiveDirectionRun.dat", sub_dir);
     long steps = (long)(prodDuration / TIME_STEP);
-    int index = 0;
     printf("Starting run and generating hell lot of data\n");
-    for (long i = 0; i < steps;) {
+    for (long i = i0; i < steps;) {
         /* the step index i is the one about to finish; events fire after it */
         long ev = steps - 1;
         if (index < cap && (long)next_idx[index] >= i && (long)next_idx[index] < ev) ev = (long)next_idx[index];
         long a = ((i + frame_step - 1) / frame_step) * (long)frame_step;
-        long b = ((i + 99999) / 100000) * 100000L;
+        long b = ((i + every - 1) / every) * every;
         if (a < ev) ev = a;
         if (b < ev) ev = b;
         CK(ac4_md_step(md, (int)(ev - i + 1)));
-        if (index < cap && (unsigned long long)ev == next_idx[index]) { print_data(md, &S, data_file); index++; }
+        if (index < cap && (unsigned long long)ev == next_idx[index]) {
+            print_data(md, &S, data_file, bin_file, ev);
+            index++;
+        }
         if (ev % frame_step == 0) print_stuff(md, stuff_file, n);
-        if (ev % 100000 == 0) { write_restart(md, &S, restart_file, L, (int)ev, index - 1); save_dirs(&S, path); }
         i = ev + 1;
+        if (ev % every == 0) {
+            write_restart(md, &S, restart_file, L, (int)ev, index - 1);
+            save_dirs(&S, path);
+            write_exact(md, exact_file, i, index);
+            if (stop_at >= 0 && ev >= stop_at) {
+                CK(ac4_md_sync(md));
+                printf("Stopped after step %ld (resume with AC4_RESUME=1)\n", ev);
+                ac4_md_destroy(md);
+                return 0;
+            }
+        }
     }
     CK(ac4_md_sync(md));
     printf("Here it ends\n");

@@ -340,6 +340,14 @@ int ac4_schedule(uint64_t run_length, int n_origins, double factor,
 int ac4_init_lattice(int n, double T, int n_act, uint64_t seed, double *x, double *y, double *z, double *xu,
                      double *yu, double *zu, double *px, double *py, double *pz, int *type, int *act_tag, int *kx,
                      int *ky, int *kz);
+/* n lines of data_coor.out, "%.5lf\t%.5lf\t%.5lf\t%d\n" (MDcode/print.c:4-16),
+ * byte-identical to glibc's printf; out needs 80*n bytes.  Returns the bytes
+ * written or a negative error. */
+long long ac4_format_frame(int n, const double *x, const double *y, const double *z, const int *type, char *out,
+                           long long cap);
+/* Reads n such lines (Analysis/read_all_data.c:61-91); every number equals
+ * strtod's.  Returns the bytes consumed or a negative error. */
+long long ac4_parse_frame(const char *text, long long len, int n, double *x, double *y, double *z, int *type);
 /* read_all_data.c:36-48 */
 int ac4_origin_gap(const int *fidx, int nframes);
 /* InitialiseNoseHoover (MDcode/time_evolve.c:476-493) with a seeded draw:

@@ -53,3 +53,39 @@ def test_run_then_analyse(gpu, tmp_path):
     b = np.loadtxt(ref / "msd.dat")
     assert a.shape == b.shape and np.allclose(a, b, rtol=1e-12, atol=1e-14)
     assert os.path.getsize(job / "S4qt_a_0.30.dat") > 1000
+
+
+def _run(cwd, args, **env):
+    e = dict(os.environ, AC4_SEED="777", **env)
+    r = subprocess.run([os.path.join(BIN, "run")] + args, cwd=cwd, env=e, capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr[-2000:]
+    return r
+
+
+def test_run_resumes_exactly_and_sidecar_matches_text(gpu, tmp_path):
+    """A production run stopped after a restart file and resumed with
+    AC4_RESUME=1 writes byte for byte the files of the uninterrupted run; the
+    binary sidecar holds exactly the values the text parses to, so analyse
+    on it writes the same output files."""
+    n = 1000
+    para = "eqDuration\tprodDuration\tframe_step\n1.0\t10.0\t500\nDENSITY\tT\tP\n1.20\t0.60\t1.0\nf0\n1.5\n"
+    for d in ("a", "b"):
+        (tmp_path / d).mkdir()
+        (tmp_path / d / "para_file").write_text(para)
+    _run(tmp_path / "a", ["1", "0", "0", str(n)], AC4_BIN="1", AC4_RESTART_EVERY="500")
+    r = _run(tmp_path / "b", ["1", "0", "0", str(n)], AC4_BIN="1", AC4_RESTART_EVERY="500", AC4_STOP_AT="1000")
+    assert "Stopped after step 1000" in r.stdout
+    r = _run(tmp_path / "b", ["1", "0", "1", str(n)], AC4_BIN="1", AC4_RESTART_EVERY="500", AC4_RESUME="1")
+    assert "resumed exactly at step 1001" in r.stdout
+    ja, jb = tmp_path / "a" / "0.600" / "data_001", tmp_path / "b" / "0.600" / "data_001"
+    for f in ("data_coor.out", "data_coor.bin", "stuff.dat", "time_index_file.dat", "ActiveDirectionRun.dat"):
+        assert filecmp.cmp(ja / f, jb / f, shallow=False), f
+    # analyse from the text and from the sidecar
+    outs = {}
+    for mode in ("0", "1"):
+        env = dict(os.environ, AC4_DIR=str(ja), AC4_BIN=mode)
+        r = subprocess.run([os.path.join(BIN, "analyse"), "1", "0.6", "1.0", str(n)], cwd=tmp_path, env=env,
+                           capture_output=True, text=True, timeout=600)
+        assert r.returncode == 0, r.stderr[-2000:]
+        outs[mode] = {f: (ja / f).read_bytes() for f in ("msd.dat", "relax_a_0.30.dat", "S4qt_a_0.30.dat")}
+    assert outs["0"] == outs["1"]

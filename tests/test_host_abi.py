@@ -1,6 +1,7 @@
 """C-ABI library checks that need no GPU: exported symbols, loud failure
 without a device, host-side logic (schedule, origin gap, walk) against the
 oracle, Philox known-answer vectors."""
+import ctypes
 import os
 import re
 import subprocess
@@ -81,3 +82,50 @@ def test_init_lattice_invariants(ac4):
     assert np.all((pos >= 0) & (pos < 1))
     s2 = ac4.init_lattice(n, T, 42)
     assert np.array_equal(s["x"], s2["x"]) and np.array_equal(s["px"], s2["px"])
+
+
+def _glibc_line(libc, x, y, z, t):
+    buf = ctypes.create_string_buffer(256)
+    libc.snprintf(buf, 256, b"%.5lf\t%.5lf\t%.5lf\t%d\n", ctypes.c_double(x), ctypes.c_double(y), ctypes.c_double(z),
+                  ctypes.c_int(t))
+    return buf.value
+
+
+def test_frame_format_is_glibc_printf(ac4):
+    """ac4_format_frame against glibc's own %.5lf on values a trajectory holds
+    and on the hard cases: exact ties (k/2^j), values that round to -0.00000,
+    subnormals, large unwrapped coordinates."""
+    libc = ctypes.CDLL(None)
+    rng = np.random.default_rng(5)
+    vals = np.concatenate([
+        rng.uniform(-3, 3, 4000), rng.uniform(-5000, 5000, 1000), rng.normal(0, 1e-6, 500),
+        np.arange(-4096, 4096) / 2.0 ** 12, np.arange(0, 2000) / 2.0 ** 17 - 0.0076,
+        [0.0, -0.0, 5e-324, -5e-324, 1e-300, 0.000005, -0.000005, 0.015625, 0.999995, 1 - 1e-17, 12345.678905,
+         2.5e-6, -2.5e-6, 7.5e-6, 0.000015, 1e11 + 0.5],
+    ])
+    n = len(vals) // 3 * 3
+    v = vals[:n].reshape(3, -1)
+    t = (rng.random(v.shape[1]) < 0.2).astype(np.int32)
+    text = ac4.format_frame(v[0], v[1], v[2], t)
+    want = b"".join(_glibc_line(libc, a, b, c, int(d)) for a, b, c, d in zip(v[0], v[1], v[2], t))
+    assert text == want
+
+
+def test_frame_parse_is_strtod(ac4):
+    libc = ctypes.CDLL(None)
+    libc.strtod.restype = ctypes.c_double
+    libc.strtod.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_char_p)]
+    rng = np.random.default_rng(6)
+    m = 3000
+    v = rng.uniform(-40, 40, (3, m))
+    t = (rng.random(m) < 0.2).astype(np.int32)
+    text = ac4.format_frame(v[0], v[1], v[2], t)
+    # also other spellings the reader must accept: exponents, long mantissas
+    text += b"1e-3\t-2.5E1\t0.12345678901234567\t1\n"
+    x, y, z, tt, k = ac4.parse_frame(text, m + 1)
+    assert k == len(text)
+    toks = text.split()
+    want = np.array([libc.strtod(tok, None) for tok in toks]).reshape(-1, 4)
+    assert np.array_equal(x, want[:, 0]) and np.array_equal(y, want[:, 1]) and np.array_equal(z, want[:, 2])
+    assert np.array_equal(np.signbit(x), np.signbit(want[:, 0]))
+    assert np.array_equal(tt, want[:, 3].astype(np.int32))
