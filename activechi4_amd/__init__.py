@@ -97,6 +97,8 @@ _SIGS = {
     "ac4_frames_count": (ctypes.c_int, [_VP]),
     "ac4_frames_find": (ctypes.c_int, [_VP, ctypes.c_longlong]),
     "ac4_frames_overlap": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_double, _I]),
+    "ac4_frames_block": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_double, ctypes.c_char_p]),
     "ac4_shell_directions": (ctypes.c_int, [ctypes.c_int, ctypes.c_uint64, _D]),
     "ac4_frames_s4_shells": (ctypes.c_int, [_VP, ctypes.c_int, _I, _I, ctypes.c_double, ctypes.c_double,
                                             ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, _D, _D,
@@ -531,6 +533,11 @@ class Frames:
                                         int(seed) & (2**64 - 1), _dp(s4r), _dp(s4d),
                                         _dp(ph) if phases else None))
         return (s4r, s4d, ph) if phases else (s4r, s4d)
+
+    def block(self, gap, cell_cut, sub_dir, factor=1.4, a=0.3, vh_time=1500.0):
+        """Block MSD, overlap and van Hove files for one block edge under
+        sub_dir/BlockAnalysis/ (ac4_frames_block)."""
+        _check(lib.ac4_frames_block(self.h, int(gap), factor, a, cell_cut, vh_time, os.fsencode(sub_dir)))
 
 
 def shell_directions(ndir, seed):

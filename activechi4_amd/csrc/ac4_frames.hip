@@ -185,6 +185,19 @@ extern "C" int ac4_frames_destroy(ac4_frames *f) {
 
 extern "C" int ac4_frames_count(const ac4_frames *f) { return f ? f->count : 0; }
 
+int ac4_frames_view_get(const ac4_frames *f, ac4_frames_view *v) {
+    AC4_REQUIRE(f && v, AC4_EARG, "frames view: bad arguments");
+    v->n = f->n;
+    v->count = f->count;
+    v->L = f->L;
+    v->x = f->x;
+    v->y = f->y;
+    v->z = f->z;
+    v->step = f->step.data();
+    v->s = f->s;
+    return AC4_OK;
+}
+
 extern "C" int ac4_frames_find(const ac4_frames *f, long long step) {
     if (!f) return -1;
     for (int k = 0; k < f->count; k++)

@@ -76,3 +76,14 @@ __device__ inline int ac4_wave_sum_i(int v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
     return v;
 }
+
+// Device view of a frame store (ac4_frames.hip) for the analyses built on it:
+// frame k of particle i is x[k*n + i]; step[k] is its step index.
+struct ac4_frames_view {
+    int n, count;
+    double L;
+    const double *x, *y, *z;
+    const long long *step;
+    hipStream_t s;
+};
+int ac4_frames_view_get(const ac4_frames *f, ac4_frames_view *v);

@@ -182,6 +182,21 @@ int ac4_frames_s4_shells(ac4_frames *f, int npairs, const int *orig, const int *
                          double dq, int nshell, int ndir, uint64_t seed, double *s4_ref, double *s4_dir,
                          double *phases);
 
+/* Block analysis of the stored trajectory for one block edge: replaces
+ * Analysis_Block/msd_pos.c:3-240 (mean_square_displacement), overlap.c:4-232
+ * (overlap_correlation_function) and vanHove.c:3-183 (VanHove), which
+ * Analysis_Block/main.c:53-63 calls for cellCutOff = L/scale, scale = 3..19.
+ * The frames must have been pushed in the order of time_index_file.dat, with
+ * their step numbers; gap is the origin spacing (ac4_origin_gap), factor the
+ * lag ratio (1.4), a the overlap length (0.3), cell_cut the nominal block
+ * edge (the grid has (int)(L/cell_cut) blocks per edge, at most 25), vh_time
+ * the van Hove time (1500.0 in the reference).  Writes, under
+ * sub_dir/BlockAnalysis/ (created when missing), MSD/MSDLb<Lb>.dat,
+ * Relax/RelaxLb<Lb>.dat, Relax/RelaxTotLb<Lb>.dat and vanHove<Lb>.dat in the
+ * reference's formats, Lb = L/blocks per edge as "%.3lf". */
+int ac4_frames_block(ac4_frames *f, int gap, double factor, double a, double cell_cut, double vh_time,
+                     const char *sub_dir);
+
 /* ------------------------------------------------------ exact checkpoints --
  * A superset of MDcode's restart_file.dat (main.c:272-334), which keeps only
  * positions, velocities and tags: the complete device state of replica r

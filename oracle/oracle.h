@@ -180,4 +180,14 @@ int or_analyse_dir(const char *sub_dir, int N, int do_msd, int do_overlap,
  * pooled = Nbar*(<Q^2>-<Q>^2) over all (r,o); mean_of_r = mean_r N_r*Var_o(Q_r). */
 void or_chi4_ensemble(int nrep, int nor, const double *q, const int *n_r,
                       double *out2);
+/* Analysis_Block/{msd_pos,overlap,vanHove}.c for one block edge cell_cut:
+ * writes MSDLb, RelaxLb, RelaxTotLb and vanHove files under
+ * sub_dir/BlockAnalysis/ (the directories must exist).  param = a*a. */
+int or_block_scale(int N, double L, int nframes, const int *fidx, int gap,
+                   const double *X, const double *Y, const double *Z,
+                   double factor, double param, double cell_cut,
+                   double vh_time, const char *sub_dir);
+/* Analysis_Block/main.c:45-70 for one job directory (scale 3..19). */
+int or_block_analyse_dir(const char *sub_dir, int N, double vh_time);
+
 #endif

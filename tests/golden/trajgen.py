@@ -53,9 +53,20 @@ def make_frames(n, frames, seed, step_sigma=0.06, wild=0.0):
 
 
 def quantize(a):
-    """The doubles a reader gets back from the ``%.5lf`` text of ``a``."""
+    """The doubles a reader gets back from the ``%.5lf`` text of ``a``.
+
+    ``%.5f`` rounds the exact value to k * 1e-5 and the reader returns the
+    double nearest k * 1e-5, which is k / 1e5 (both exact, one IEEE division).
+    rint(a * 1e5) finds k except within a rounding error of a tie; those few
+    values go through the text itself."""
     a = np.asarray(a, dtype=np.float64)
-    return np.char.mod("%.5f", a).astype(np.float64)
+    s = a * 1e5
+    k = np.rint(s)
+    out = k / 1e5
+    amb = np.abs(np.abs(s - k) - 0.5) < 1e-6
+    if amb.any():
+        out[amb] = np.char.mod("%.5f", a[amb]).astype(np.float64)
+    return out
 
 
 def write_job(sub_dir, frames, X, Y, Z, T, active=None):
@@ -82,6 +93,15 @@ def write_job(sub_dir, frames, X, Y, Z, T, active=None):
 CASES = [
     ("walk_g20", 50000, 2, 20, 20261202, 0.06, 0.0),
     ("wild_g24", 50000, 3, 24, 7, 0.05, 1.5),
+]
+
+
+# Block analysis (Analysis_Block, N fixed at 50000 there): the two cases above
+# plus lags long enough (up to 1900 time units) for the van Hove lag search.
+BLOCK_CASES = [
+    ("walk_g20", 50000, 2, 20, 20261202, 0.06, 0.0),
+    ("wild_g24", 50000, 3, 24, 7, 0.05, 1.5),
+    ("long_g400k", 50000, 2, 400000, 11, 0.002, 0.0),
 ]
 
 

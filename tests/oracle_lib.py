@@ -53,6 +53,10 @@ SIGS = {
     "or_analyse_dir": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                       ctypes.c_int]),
     "or_chi4_ensemble": (None, [ctypes.c_int, ctypes.c_int, D, I, D]),
+    "or_block_scale": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_int, I, ctypes.c_int, D, D, D,
+                                      ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                      ctypes.c_char_p]),
+    "or_block_analyse_dir": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_double]),
 }
 
 
@@ -204,3 +208,24 @@ def phase(kind, N, L, X, Y, Z, T, fidx, gap, a=0.3, flag=0, factor=1.4):
                                 dp(ss), dp(cs2), dp(ss2), ip(cn), dp(dt), dp(qv), dp(pcs))
     return dict(cs=cs.reshape(ns, 162), ss=ss.reshape(ns, 162), cs2=cs2.reshape(ns, 162), ss2=ss2.reshape(ns, 162),
                 counters=cn, delta_t=dt, qval=qv, pairs=pcs[:npairs * 324].reshape(-1, 324), mask=pm[:npairs])
+
+
+def block_scales(L):
+    """Block edges of Analysis_Block/main.c:45-63: L/scale, scale = 3..19."""
+    out, scale = [], 3.0
+    while scale < 20.0:
+        out.append(L / scale)
+        scale = scale + 1.0
+    return out
+
+
+def block_all(N, L, X, Y, Z, fidx, gap, sub_dir, vh_time=1500.0, factor=1.4, a=0.3, cuts=None):
+    """or_block_scale for every block edge; files under sub_dir/BlockAnalysis."""
+    for d in ("BlockAnalysis", "BlockAnalysis/MSD", "BlockAnalysis/Relax"):
+        os.makedirs(os.path.join(sub_dir, d), exist_ok=True)
+    X, Y, Z = (np.ascontiguousarray(v, dtype=np.float64) for v in (X, Y, Z))
+    f = np.ascontiguousarray(fidx, dtype=np.int32)
+    for cut in (cuts if cuts is not None else block_scales(L)):
+        rc = load().or_block_scale(N, L, len(f), ip(f), gap, dp(X), dp(Y), dp(Z), factor, a * a, cut, vh_time,
+                                   os.fsencode(sub_dir))
+        assert rc == 0
