@@ -99,6 +99,10 @@ _SIGS = {
     "ac4_frames_overlap": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_double, _I]),
     "ac4_frames_block": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                         ctypes.c_double, ctypes.c_char_p]),
+    "ac4_frames_overlap_cage": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                               ctypes.c_char_p, _I, ctypes.c_int, _I]),
+    "ac4_frames_guu": (ctypes.c_int, [_VP, _I, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_char_p,
+                                      _D, _I, _I, _I]),
     "ac4_shell_directions": (ctypes.c_int, [ctypes.c_int, ctypes.c_uint64, _D]),
     "ac4_frames_s4_shells": (ctypes.c_int, [_VP, ctypes.c_int, _I, _I, ctypes.c_double, ctypes.c_double,
                                             ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, _D, _D,
@@ -494,6 +498,7 @@ class Frames:
         _check(lib.ac4_frames_create(n, L, cap, ctypes.byref(h)))
         self.h = h
         self.n = n
+        self.L = L
 
     def close(self):
         if self.h:
@@ -538,6 +543,25 @@ class Frames:
         """Block MSD, overlap and van Hove files for one block edge under
         sub_dir/BlockAnalysis/ (ac4_frames_block)."""
         _check(lib.ac4_frames_block(self.h, int(gap), factor, a, cell_cut, vh_time, os.fsencode(sub_dir)))
+
+    def overlap_cage(self, gap, path=None, factor=1.4, a=0.3, cell_cut=2.85, cap=1 << 20):
+        """Cage-relative overlap (ac4_frames_overlap_cage); returns the count of
+        every (origin, lag) pair in walk order."""
+        pc = np.zeros(cap, dtype=np.int32)
+        npairs = ctypes.c_int()
+        _check(lib.ac4_frames_overlap_cage(self.h, int(gap), factor, a, cell_cut,
+                                           os.fsencode(path) if path else None, _ip(pc), cap, ctypes.byref(npairs)))
+        return pc[:min(npairs.value, cap)]
+
+    def guu(self, act_tag, gap, tau, path=None, factor=1.4):
+        """g_uu(r) (ac4_frames_guu); returns (hist[3][maxbin], frame_num, n_origins)."""
+        at = _i32(act_tag)
+        mb = int(0.5 * self.L / 0.005)
+        h = np.zeros(3 * max(mb, 1))
+        out = [ctypes.c_int() for _ in range(3)]
+        _check(lib.ac4_frames_guu(self.h, _ip(at), int(gap), factor, tau, os.fsencode(path) if path else None,
+                                  _dp(h), *[ctypes.byref(o) for o in out]))
+        return h[:3 * out[0].value].reshape(3, out[0].value), out[1].value, out[2].value
 
 
 def shell_directions(ndir, seed):

@@ -197,6 +197,29 @@ int ac4_frames_s4_shells(ac4_frames *f, int npairs, const int *orig, const int *
 int ac4_frames_block(ac4_frames *f, int gap, double factor, double a, double cell_cut, double vh_time,
                      const char *sub_dir);
 
+/* Cage-relative overlap of the stored trajectory: replaces
+ * Analysis/overlap_cage_rel.c:4-184 (overlap_cage; its call at
+ * Analysis/main.c:57 is commented out and the file is not in
+ * Analysis/makefile).  Each particle's displacement is taken relative to the
+ * mean displacement of its neighbours within 1.3 sigma at the origin,
+ * listed in the order of GetList (:185-251) on a grid of (int)(L/cell_cut)
+ * cells per edge (at least 3; cell_cut 2.85 is MDcode's Cell_cut); the
+ * overlap count uses a*a.  Origins and lags as ac4_frames_block.  Writes path
+ * (nullable) in the relaxMPICR1_a_<a>.dat format (:156-167); pair_counts
+ * (nullable, cap_pairs entries) receives the count of every (origin, lag)
+ * pair in walk order, *npairs (nullable) their number. */
+int ac4_frames_overlap_cage(ac4_frames *f, int gap, double factor, double a, double cell_cut, const char *path,
+                            int *pair_counts, int cap_pairs, int *npairs);
+/* Displacement-displacement correlation g_uu(r) of the active particles
+ * (act_tag[i] == 1, ActiveFileIndex.dat) at the lag nearest tau on the first
+ * frame's ladder: replaces Analysis/guu.c:3-300 (the guuTauAlphaAct.dat
+ * branch; its call at Analysis/main.c:64 is commented out).  Bins of 0.005
+ * up to L/2.  Writes path (nullable) in the reference's format; hist3
+ * (nullable, 3*maxbin) receives hist, hist1, hist2; maxbin, frame_num (the
+ * lag in steps) and n_origins (origins used) are returned when not NULL. */
+int ac4_frames_guu(ac4_frames *f, const int *act_tag, int gap, double factor, double tau, const char *path,
+                   double *hist3, int *maxbin, int *frame_num, int *n_origins);
+
 /* ------------------------------------------------------ exact checkpoints --
  * A superset of MDcode's restart_file.dat (main.c:272-334), which keeps only
  * positions, velocities and tags: the complete device state of replica r

@@ -190,4 +190,21 @@ int or_block_scale(int N, double L, int nframes, const int *fidx, int gap,
 /* Analysis_Block/main.c:45-70 for one job directory (scale 3..19). */
 int or_block_analyse_dir(const char *sub_dir, int N, double vh_time);
 
+/* Analysis/guu.c:3-300 (the guuTauAlphaAct.dat branch): writes path when
+ * not NULL; hist3 (nullable) receives hist, hist1, hist2 [3][maxbin]. */
+int or_guu(int N, double L, int nframes, const int *fidx, int gap,
+           const double *X, const double *Y, const double *Z, const int *act,
+           double factor, double tau, const char *path, double *hist3,
+           int *maxbin, int *frame_num, int *n_origins);
+/* GetList of Analysis/overlap_cage_rel.c:185-251 on wrapped positions:
+ * nn[i] neighbours nb[i*cap + k] in traversal order; -1 on overflow. */
+int or_cage_list(int N, double L, double cell_cut, const double *x0,
+                 const double *y0, const double *z0, int cap, int *nn, int *nb);
+/* Analysis/overlap_cage_rel.c:4-184: writes path (relaxMPICR1 format) when
+ * not NULL; pair_counts (nullable) receives the count of every pair. */
+int or_overlap_cage(int N, double L, int nframes, const int *fidx, int gap,
+                    const double *X, const double *Y, const double *Z,
+                    double factor, double a, double cell_cut, const char *path,
+                    int *pair_counts, int cap_pairs, int *npairs);
+
 #endif

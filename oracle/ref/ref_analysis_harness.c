@@ -8,7 +8,9 @@
  * and then calls the observables of Analysis/main.c:53-62 selected by a
  * letter string: m = mean_square_displacement(1.4,0,0),
  * o = overlap_correlation_function(1.4,0.3,0,0), s = S4Qt(1.4,0.3),
- * f = Fsqt(1.4,0.3,0).  usage: analyse_ref <sub_dir> <letters>
+ * f = Fsqt(1.4,0.3,0), g = guu(1.4, tauAlpha) (the call Analysis/main.c:64 has
+ * commented out; guuTauAlphaAct.dat).
+ * usage: analyse_ref <sub_dir> <letters> [tauAlpha]
  */
 #include <string.h>
 #include "global.h"
@@ -16,7 +18,7 @@
 
 int main(int argc, char **argv) {
     if (argc < 3) {
-        fprintf(stderr, "usage: %s <sub_dir> <letters: m o s f>\n", argv[0]);
+        fprintf(stderr, "usage: %s <sub_dir> <letters: m o s f g> [tauAlpha]\n", argv[0]);
         return 2;
     }
     double factor = 1.4000000;
@@ -37,6 +39,7 @@ int main(int argc, char **argv) {
     if (strchr(w, 'o')) overlap_correlation_function(factor, 0.3, 0, 0);
     if (strchr(w, 's')) S4Qt(factor, 0.3);
     if (strchr(w, 'f')) Fsqt(factor, 0.3, 0);
+    if (strchr(w, 'g')) guu(factor, argc > 3 ? atof(argv[3]) : 0.0);
     MPI_Barrier(MPI_COMM_WORLD);
     MPI_Finalize();
     return 0;

@@ -22,6 +22,8 @@ sys.path.insert(0, HERE)
 import trajgen  # noqa: E402
 
 OUTPUTS = ["msd.dat", "relax_a_0.30.dat", "S4qt_a_0.30.dat", "Fsqt_a_0.30.dat"]
+# g_uu(r) at the lag nearest tau (Analysis/guu.c, called through the harness)
+GUU_TAU = {"walk_g20": 0.05, "wild_g24": 0.07}
 
 
 def main():
@@ -34,13 +36,14 @@ def main():
             job = os.path.join(tmp, "job")
             trajgen.write_job(job, frames, X, Y, Z, T)
             subprocess.check_call([exe, job, "mosf"], stdout=subprocess.DEVNULL)
+            subprocess.check_call([exe, job, "g", repr(GUU_TAU[name])], stdout=subprocess.DEVNULL)
             dst = os.path.join(HERE, name)
             os.makedirs(dst, exist_ok=True)
-            for o in OUTPUTS:
+            for o in OUTPUTS + ["guuTauAlphaAct.dat"]:
                 shutil.copy(os.path.join(job, o), os.path.join(dst, o))
         with open(os.path.join(dst, "case.json"), "w") as f:
             json.dump({"n": n, "n_origins": n_or, "gap": gap, "seed": seed,
-                       "step_sigma": sig, "wild": wild, "frames": frames,
+                       "step_sigma": sig, "wild": wild, "frames": frames, "guu_tau": GUU_TAU[name],
                        "producer": "reference Analysis (oracle/ref/Makefile)"}, f, indent=1)
         print("golden", name, "frames", len(frames))
 

@@ -57,6 +57,12 @@ SIGS = {
                                       ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                       ctypes.c_char_p]),
     "or_block_analyse_dir": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_double]),
+    "or_guu": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_int, I, ctypes.c_int, D, D, D, I,
+                              ctypes.c_double, ctypes.c_double, ctypes.c_char_p, D, I, I, I]),
+    "or_cage_list": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_double, D, D, D, ctypes.c_int, I, I]),
+    "or_overlap_cage": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_int, I, ctypes.c_int, D, D, D,
+                                       ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_char_p, I,
+                                       ctypes.c_int, I]),
 }
 
 
@@ -229,3 +235,42 @@ def block_all(N, L, X, Y, Z, fidx, gap, sub_dir, vh_time=1500.0, factor=1.4, a=0
         rc = load().or_block_scale(N, L, len(f), ip(f), gap, dp(X), dp(Y), dp(Z), factor, a * a, cut, vh_time,
                                    os.fsencode(sub_dir))
         assert rc == 0
+
+
+def guu(N, L, X, Y, Z, act, fidx, gap, tau, path=None, factor=1.4):
+    """or_guu: returns (hist[3][maxbin], frame_num, n_origins)."""
+    X, Y, Z = (np.ascontiguousarray(v, dtype=np.float64) for v in (X, Y, Z))
+    f = np.ascontiguousarray(fidx, dtype=np.int32)
+    act = np.ascontiguousarray(act, dtype=np.int32)
+    mb = int(0.5 * L / 0.005)
+    h = np.zeros(3 * mb)
+    out = [ctypes.c_int() for _ in range(3)]
+    rc = load().or_guu(N, L, len(f), ip(f), gap, dp(X), dp(Y), dp(Z), ip(act), factor, tau,
+                       os.fsencode(path) if path else None, dp(h), *[ctypes.byref(o) for o in out])
+    assert rc == 0 and out[0].value == mb
+    return h.reshape(3, mb), out[1].value, out[2].value
+
+
+CAGE_CUT = 2.5 + 0.35  # MDcode/global.h:30 (Lcuttoff + skin)
+
+
+def cage_list(x0, y0, z0, L, cap=400):
+    n = len(x0)
+    nn = np.zeros(n, dtype=np.int32)
+    nb = np.full(n * cap, -1, dtype=np.int32)
+    x0, y0, z0 = (np.ascontiguousarray(v, dtype=np.float64) for v in (x0, y0, z0))
+    assert load().or_cage_list(n, L, CAGE_CUT, dp(x0), dp(y0), dp(z0), cap, ip(nn), ip(nb)) == 0
+    return nn, nb.reshape(n, cap)
+
+
+def overlap_cage(N, L, X, Y, Z, fidx, gap, path=None, a=0.3, factor=1.4):
+    """or_overlap_cage: returns the per-pair counts in walk order."""
+    X, Y, Z = (np.ascontiguousarray(v, dtype=np.float64) for v in (X, Y, Z))
+    f = np.ascontiguousarray(fidx, dtype=np.int32)
+    cap = len(f) * 64 + 64
+    pc = np.zeros(cap, dtype=np.int32)
+    npairs = ctypes.c_int()
+    rc = load().or_overlap_cage(N, L, len(f), ip(f), gap, dp(X), dp(Y), dp(Z), factor, a, CAGE_CUT,
+                                os.fsencode(path) if path else None, ip(pc), cap, ctypes.byref(npairs))
+    assert rc == 0
+    return pc[:npairs.value]
