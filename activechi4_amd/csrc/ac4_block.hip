@@ -14,16 +14,19 @@
 //   BlockAnalysis/Relax/RelaxTotLb<Lb>.dat, BlockAnalysis/vanHove<Lb>.dat.
 //
 // Kernels:
-//   k_blk_sort  one wave per origin: block ids and a stable counting sort, so
-//               each block's members are listed in ascending particle index;
+//   k_blk_count, k_blk_offsets, k_blk_place  block ids and a stable counting
+//               sort per origin (16 waves each), so each block's members are
+//               listed in ascending particle index;
 //   k_blk_pair  one thread per (pair, block): walks the block's members in
 //               that order, so every sum is formed in the reference's order
 //               and is bit-identical to it;
 //   k_blk_slot  one thread per (slot, block): accumulates the pairs of a lag
 //               slot in origin order (overlap.c:144-148, msd_pos.c:172);
-//   k_blk_tot   one thread per slot and sum: the block-summed accumulators
-//               (overlap.c:149-151, msd_pos.c:173-179) in pair-then-block
-//               order.
+//   k_blk_tot   one wave per slot and sum: the block-summed accumulators
+//               (overlap.c:149-151, msd_pos.c:173-179), whose addends
+//               k_blk_pair has formed, added one after another in
+//               pair-then-block order (64 addends per coalesced load, taken
+//               lane by lane with readlane: the chain stays sequential).
 // The final averages over blocks run on the host in the reference's order.
 //
 // Empty blocks: the reference divides by a zero block count and carries the
@@ -45,6 +48,7 @@
 
 #define BLK_THREADS 256
 #define BLK_SORT_MAX 16384     // blocks whose cursors fit the sort's 64 KB of LDS
+#define BLK_SORT_WAVES 16      // chunks of consecutive particles per origin
 #define BLK_TIME_STEP 0.005    // Analysis_Block/global.h:14
 #define BLK_DIVIDE_TO 2.0      // Analysis_Block/global.h:16
 
@@ -78,44 +82,81 @@ __device__ __forceinline__ void blk_corr(double &acc, double d) {  // msd_pos.c:
         acc += 1.0;
 }
 
-// One wave per origin: start[o][b] = first slot of block b in members[o][*],
-// start[o][ncell] = n; members of a block in ascending particle index.
-__global__ void __launch_bounds__(64) k_blk_sort(const double *__restrict__ X, const double *__restrict__ Y,
-                                                 const double *__restrict__ Z, int n, const int *__restrict__ ofr,
-                                                 int ncx, int ncell, int *__restrict__ start,
-                                                 int *__restrict__ members) {
+// Stable counting sort by block, per origin, in three passes over
+// BLK_SORT_WAVES chunks of consecutive particles: k_blk_count counts each
+// chunk's particles per block, k_blk_offsets turns the counts into the
+// first slot of every (chunk, block) and start[o][b], k_blk_place writes the
+// members.  Chunks are placed in chunk order and, inside a chunk, lanes of
+// the same block in lane order, so members[o][start[o][b] ...] lists block
+// b's particles in ascending index (start[o][ncell] = n).
+__global__ void __launch_bounds__(64) k_blk_count(const double *__restrict__ X, const double *__restrict__ Y,
+                                                  const double *__restrict__ Z, int n, const int *__restrict__ ofr,
+                                                  int ncx, int ncell, int chunk, int *__restrict__ cnt) {
     extern __shared__ int s_cur[];
-    const int o = blockIdx.x, lane = threadIdx.x;
+    const int o = blockIdx.x, w = blockIdx.y, lane = threadIdx.x;
     const size_t fb = (size_t)ofr[o] * n;
     for (int b = lane; b < ncell; b += 64) s_cur[b] = 0;
     __syncthreads();
-    for (int i = lane; i < n; i += 64)
+    const int i1 = min(n, (w + 1) * chunk);
+    for (int i = w * chunk + lane; i < i1; i += 64)
         atomicAdd(&s_cur[blk_cell(blk_wrap(X[fb + i]), blk_wrap(Y[fb + i]), blk_wrap(Z[fb + i]), ncx, ncell)], 1);
     __syncthreads();
+    int *c = cnt + ((size_t)o * gridDim.y + w) * ncell;
+    for (int b = lane; b < ncell; b += 64) c[b] = s_cur[b];
+}
+
+__global__ void __launch_bounds__(256) k_blk_offsets(int ncell, int nw, int n, int *__restrict__ cnt,
+                                                     int *__restrict__ start) {
+    __shared__ int s_x[256];
+    const int o = blockIdx.x, t = threadIdx.x;
+    int *c = cnt + (size_t)o * nw * ncell;
     int *st = start + (size_t)o * (ncell + 1);
     int run = 0;
-    for (int b0 = 0; b0 < ncell; b0 += 64) {
-        const int b = b0 + lane;
-        const int v = b < ncell ? s_cur[b] : 0;
-        int incl = v;
-        for (int d = 1; d < 64; d <<= 1) {
-            const int u = __shfl_up(incl, d);
-            if (lane >= d) incl += u;
+    for (int b0 = 0; b0 < ncell; b0 += 256) {
+        const int b = b0 + t;
+        int tot = 0;
+        if (b < ncell)
+            for (int w = 0; w < nw; w++) tot += c[(size_t)w * ncell + b];
+        s_x[t] = tot;
+        __syncthreads();
+        for (int d = 1; d < 256; d <<= 1) {
+            const int u = t >= d ? s_x[t - d] : 0;
+            __syncthreads();
+            s_x[t] += u;
+            __syncthreads();
         }
         if (b < ncell) {
-            s_cur[b] = run + incl - v;
-            st[b] = run + incl - v;
+            int acc = run + s_x[t] - tot;
+            st[b] = acc;
+            for (int w = 0; w < nw; w++) {
+                const int k = c[(size_t)w * ncell + b];
+                c[(size_t)w * ncell + b] = acc;
+                acc += k;
+            }
         }
-        run += __shfl(incl, 63);
+        run += s_x[255];
+        __syncthreads();
     }
-    if (lane == 0) st[ncell] = n;
+    if (t == 0) st[ncell] = n;
+}
+
+__global__ void __launch_bounds__(64) k_blk_place(const double *__restrict__ X, const double *__restrict__ Y,
+                                                  const double *__restrict__ Z, int n, const int *__restrict__ ofr,
+                                                  int ncx, int ncell, int chunk, const int *__restrict__ cur,
+                                                  int *__restrict__ members) {
+    extern __shared__ int s_cur[];
+    const int o = blockIdx.x, w = blockIdx.y, lane = threadIdx.x;
+    const size_t fb = (size_t)ofr[o] * n;
+    const int *c = cur + ((size_t)o * gridDim.y + w) * ncell;
+    for (int b = lane; b < ncell; b += 64) s_cur[b] = c[b];
     __syncthreads();
     int *mem = members + (size_t)o * n;
     const unsigned long long below = (1ull << lane) - 1ull;
-    for (int i0 = 0; i0 < n; i0 += 64) {
+    const int i1 = min(n, (w + 1) * chunk);
+    for (int i0 = w * chunk; i0 < i1; i0 += 64) {
         const int i = i0 + lane;
-        const int b = i < n ? blk_cell(blk_wrap(X[fb + i]), blk_wrap(Y[fb + i]), blk_wrap(Z[fb + i]), ncx, ncell) : -1;
-        unsigned long long todo = __ballot(i < n);
+        const int b = i < i1 ? blk_cell(blk_wrap(X[fb + i]), blk_wrap(Y[fb + i]), blk_wrap(Z[fb + i]), ncx, ncell) : -1;
+        unsigned long long todo = __ballot(i < i1);
         while (todo) {  // one block at a time; lanes of that block keep lane order
             const int leader = __ffsll((long long)todo) - 1;
             const int bl = __shfl(b, leader);
@@ -134,7 +175,8 @@ __global__ void __launch_bounds__(BLK_THREADS) k_blk_pair(
     const double *__restrict__ X, const double *__restrict__ Y, const double *__restrict__ Z, int n,
     const int2 *__restrict__ pairs, const int *__restrict__ ofr, long long total, int ncell,
     const int *__restrict__ start, const int *__restrict__ members, double L, double param, int *__restrict__ ov,
-    double *__restrict__ s2, double *__restrict__ ex, double *__restrict__ ey, double *__restrict__ ez) {
+    double *__restrict__ s2, double *__restrict__ ex, double *__restrict__ ey, double *__restrict__ ez,
+    double *__restrict__ add) {
     const long long t = (long long)blockIdx.x * BLK_THREADS + threadIdx.x;
     if (t >= total) return;
     const int p = (int)(t / ncell), b = (int)(t - (long long)p * ncell);
@@ -172,6 +214,13 @@ __global__ void __launch_bounds__(BLK_THREADS) k_blk_pair(
     ex[t] = cx - ax;
     ey[t] = cy - ay;
     ez[t] = cz - az;
+    // the pair's addends to the block-summed accumulators: overlap.c:149,
+    // msd_pos.c:173-177 (NumInCell = the member count; 0 gives NaN, as there)
+    const double nb = (double)(k1 - st[b]);
+    const double dx = (cx - ax) / nb, dy = (cy - ay) / nb, dz = (cz - az) / nb;
+    add[t] = (double)cnt / nb;
+    add[total + t] = L * L * (dx * dx + dy * dy + dz * dz);
+    add[2 * total + t] = L * L * L * L * (dx * dx * dx * dx + dy * dy * dy * dy + dz * dz * dz * dz);
 }
 
 // One thread per (slot, block): the pairs of the slot in walk (origin) order.
@@ -204,43 +253,25 @@ __global__ void __launch_bounds__(BLK_THREADS) k_blk_slot(int noi, int ncell, co
     msd[o] = m;
 }
 
-// Threads 0..noi-1: the overlap sum over pairs and blocks of a slot;
-// threads noi..2*noi-1: the block-COM MSD sums of the slot.
-__global__ void k_blk_tot(int noi, int ncell, const int *__restrict__ sptr, const int *__restrict__ spairs,
-                          const int2 *__restrict__ pairs, const int *__restrict__ start, const int *__restrict__ ov,
-                          const double *__restrict__ ex, const double *__restrict__ ey, const double *__restrict__ ez,
-                          double L, double *cT, int *cTn, double *mc, double *mc2, int *binder) {
-    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (tid >= 2 * noi) return;
-    const int slot = tid % noi;
-    const bool com = tid >= noi;
-    const double L2 = L * L, L4 = L * L * L * L;
-    double a = 0.0, a2 = 0.0;
-    int k = 0;
+// Wave (slot, q): out[q][slot] = sum over the slot's pairs (walk order) and
+// blocks (ascending) of add[q][pair][block], one addition at a time.
+__global__ void __launch_bounds__(64) k_blk_tot(int noi, int ncell, long long total, const int *__restrict__ sptr,
+                                                const int *__restrict__ spairs, const double *__restrict__ add,
+                                                double *out) {
+    const int slot = blockIdx.x / 3, q = blockIdx.x - 3 * (blockIdx.x / 3), lane = threadIdx.x;
+    const double *src = add + (size_t)q * total;
+    double acc = 0.0;
     for (int j = sptr[slot]; j < sptr[slot + 1]; j++) {
-        const int p = spairs[j];
-        const int *st = start + (size_t)pairs[p].x * (ncell + 1);
-        const size_t q0 = (size_t)p * ncell;
-        for (int b = 0; b < ncell; b++) {
-            const double nb = (double)(st[b + 1] - st[b]);
-            if (!com) {
-                a += (double)ov[q0 + b] / nb;
-            } else {
-                const double dx = ex[q0 + b] / nb, dy = ey[q0 + b] / nb, dz = ez[q0 + b] / nb;
-                a += L2 * (dx * dx + dy * dy + dz * dz);
-                a2 += L4 * (dx * dx * dx * dx + dy * dy * dy * dy + dz * dz * dz * dz);
-            }
-            k++;
+        const double *row = src + (size_t)spairs[j] * ncell;
+        for (int b0 = 0; b0 < ncell; b0 += 64) {
+            const double v = b0 + lane < ncell ? row[b0 + lane] : 0.0;
+            const int lo = __double2loint(v), hi = __double2hiint(v);
+            const int m = min(64, ncell - b0);
+            for (int k = 0; k < m; k++)
+                acc += __hiloint2double(__builtin_amdgcn_readlane(hi, k), __builtin_amdgcn_readlane(lo, k));
         }
     }
-    if (!com) {
-        cT[slot] = a;
-        cTn[slot] = k;
-    } else {
-        mc[slot] = a;
-        mc2[slot] = a2;
-        binder[slot] = k;
-    }
+    if (lane == 0) out[q * noi + slot] = acc;
 }
 
 namespace {
@@ -388,12 +419,13 @@ extern "C" int ac4_frames_block(ac4_frames *f, int gap, double factor, double a,
 
     const size_t res = (size_t)npt * ncell, bs = (size_t)ncell * noi;
     DevBuf B;
-    int *d_ofr, *d_start, *d_mem, *d_ov, *d_sptr, *d_spairs, *d_cn, *d_cTn, *d_binder;
+    int *d_ofr, *d_start, *d_mem, *d_ov, *d_sptr, *d_spairs, *d_cn, *d_wcnt;
     int2 *d_pairs;
-    double *d_s2, *d_ex, *d_ey, *d_ez, *d_c, *d_x4, *d_msd, *d_cT, *d_mc, *d_mc2;
+    double *d_s2, *d_ex, *d_ey, *d_ez, *d_add, *d_c, *d_x4, *d_msd, *d_tot;
     BTRY(B.get(&d_ofr, nor));
     BTRY(B.get(&d_start, (size_t)nor * (ncell + 1)));
     BTRY(B.get(&d_mem, (size_t)nor * n));
+    BTRY(B.get(&d_wcnt, (size_t)nor * BLK_SORT_WAVES * ncell));
     BTRY(B.get(&d_pairs, npt));
     BTRY(B.get(&d_ov, res));
     BTRY(B.get(&d_s2, res));
@@ -406,31 +438,32 @@ extern "C" int ac4_frames_block(ac4_frames *f, int gap, double factor, double a,
     BTRY(B.get(&d_x4, bs));
     BTRY(B.get(&d_cn, bs));
     BTRY(B.get(&d_msd, bs));
-    BTRY(B.get(&d_cT, noi));
-    BTRY(B.get(&d_cTn, noi));
-    BTRY(B.get(&d_mc, noi));
-    BTRY(B.get(&d_mc2, noi));
-    BTRY(B.get(&d_binder, noi));
+    BTRY(B.get(&d_add, 3 * res));
+    BTRY(B.get(&d_tot, (size_t)3 * noi));
     hipStream_t s = v.s;
     AC4_HIP(hipMemcpyAsync(d_ofr, orig.data(), sizeof(int) * nor, hipMemcpyHostToDevice, s));
     AC4_HIP(hipMemcpyAsync(d_pairs, pairs.data(), sizeof(int2) * npt, hipMemcpyHostToDevice, s));
     AC4_HIP(hipMemcpyAsync(d_sptr, sptr.data(), sizeof(int) * (noi + 1), hipMemcpyHostToDevice, s));
     if (np) AC4_HIP(hipMemcpyAsync(d_spairs, spairs.data(), sizeof(int) * np, hipMemcpyHostToDevice, s));
 
-    hipLaunchKernelGGL(k_blk_sort, dim3(nor), dim3(64), sizeof(int) * ncell, s, v.x, v.y, v.z, n, d_ofr, ncx, ncell,
-                       d_start, d_mem);
+    const int chunk = (n + BLK_SORT_WAVES - 1) / BLK_SORT_WAVES;
+    hipLaunchKernelGGL(k_blk_count, dim3(nor, BLK_SORT_WAVES), dim3(64), sizeof(int) * ncell, s, v.x, v.y, v.z, n,
+                       d_ofr, ncx, ncell, chunk, d_wcnt);
+    hipLaunchKernelGGL(k_blk_offsets, dim3(nor), dim3(256), 0, s, ncell, BLK_SORT_WAVES, n, d_wcnt, d_start);
+    hipLaunchKernelGGL(k_blk_place, dim3(nor, BLK_SORT_WAVES), dim3(64), sizeof(int) * ncell, s, v.x, v.y, v.z, n,
+                       d_ofr, ncx, ncell, chunk, d_wcnt, d_mem);
     AC4_HIP(hipGetLastError());
     if (res) {
         hipLaunchKernelGGL(k_blk_pair, dim3((unsigned)((res + BLK_THREADS - 1) / BLK_THREADS)), dim3(BLK_THREADS), 0,
                            s, v.x, v.y, v.z, n, d_pairs, d_ofr, (long long)res, ncell, d_start, d_mem, L, a * a, d_ov,
-                           d_s2, d_ex, d_ey, d_ez);
+                           d_s2, d_ex, d_ey, d_ez, d_add);
         AC4_HIP(hipGetLastError());
     }
     hipLaunchKernelGGL(k_blk_slot, dim3((unsigned)((bs + BLK_THREADS - 1) / BLK_THREADS)), dim3(BLK_THREADS), 0, s,
                        noi, ncell, d_sptr, d_spairs, d_pairs, d_start, d_ov, d_s2, d_c, d_x4, d_cn, d_msd);
     AC4_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_blk_tot, dim3((2 * noi + 63) / 64), dim3(64), 0, s, noi, ncell, d_sptr, d_spairs, d_pairs,
-                       d_start, d_ov, d_ex, d_ey, d_ez, L, d_cT, d_cTn, d_mc, d_mc2, d_binder);
+    hipLaunchKernelGGL(k_blk_tot, dim3(3 * noi), dim3(64), 0, s, noi, ncell, (long long)res, d_sptr, d_spairs, d_add,
+                       d_tot);
     AC4_HIP(hipGetLastError());
 
     std::vector<double> c(bs), x4(bs), msd(bs), cT(noi), mc(noi), mc2(noi);
@@ -441,11 +474,10 @@ extern "C" int ac4_frames_block(ac4_frames *f, int gap, double factor, double a,
     AC4_HIP(hipMemcpyAsync(x4.data(), d_x4, sizeof(double) * bs, hipMemcpyDeviceToHost, s));
     AC4_HIP(hipMemcpyAsync(msd.data(), d_msd, sizeof(double) * bs, hipMemcpyDeviceToHost, s));
     AC4_HIP(hipMemcpyAsync(cn.data(), d_cn, sizeof(int) * bs, hipMemcpyDeviceToHost, s));
-    AC4_HIP(hipMemcpyAsync(cT.data(), d_cT, sizeof(double) * noi, hipMemcpyDeviceToHost, s));
-    AC4_HIP(hipMemcpyAsync(cTn.data(), d_cTn, sizeof(int) * noi, hipMemcpyDeviceToHost, s));
-    AC4_HIP(hipMemcpyAsync(mc.data(), d_mc, sizeof(double) * noi, hipMemcpyDeviceToHost, s));
-    AC4_HIP(hipMemcpyAsync(mc2.data(), d_mc2, sizeof(double) * noi, hipMemcpyDeviceToHost, s));
-    AC4_HIP(hipMemcpyAsync(binder.data(), d_binder, sizeof(int) * noi, hipMemcpyDeviceToHost, s));
+    AC4_HIP(hipMemcpyAsync(cT.data(), d_tot, sizeof(double) * noi, hipMemcpyDeviceToHost, s));
+    AC4_HIP(hipMemcpyAsync(mc.data(), d_tot + noi, sizeof(double) * noi, hipMemcpyDeviceToHost, s));
+    AC4_HIP(hipMemcpyAsync(mc2.data(), d_tot + 2 * noi, sizeof(double) * noi, hipMemcpyDeviceToHost, s));
+    for (int t = 0; t < noi; t++) cTn[t] = binder[t] = (sptr[t + 1] - sptr[t]) * ncell;  // countersT, CounterBinder
     AC4_HIP(hipMemcpyAsync(last.data(), d_start + (size_t)(nor - 1) * (ncell + 1), sizeof(int) * (ncell + 1),
                            hipMemcpyDeviceToHost, s));
     if (nvh) {

@@ -304,6 +304,119 @@ def cfg5(ac4, n=1_000_000, steps=20000, nframes=500, relax=500, ndir=64, seed=51
             "S4_dir_avg": [float(s4d[i]) for i in pick], "total_seconds": time.perf_counter() - t_all}
 
 
+def rows(ac4, n=50000, n_origins=20, gap=100, seed=777):
+    """The analysis rows beside the MD step, each on the GPU (libac4) and on
+    the oracle (one host core) over the same input:
+      - trajectory text I/O of one N=10^6 frame (ac4_format_frame /
+        ac4_parse_frame against glibc sprintf / strtod per line);
+      - block analysis (ac4_frames_block, Analysis_Block: MSD, overlap and van
+        Hove per block) for the 17 block edges L/3 ... L/19; the oracle runs
+        the edge L/3 and the GPU is timed on that edge too;
+      - g_uu(r) of the active particles (ac4_frames_guu, Analysis/guu.c);
+      - cage-relative overlap (ac4_frames_overlap_cage,
+        Analysis/overlap_cage_rel.c).
+    Input: a seeded random walk of N particles (trajgen), unwrapped box
+    fractions quantised as the %.5lf text, on the two-level frame schedule with
+    n_origins origins gap steps apart, every frame in HBM."""
+    import tempfile
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+    import oracle_lib as OL
+    import trajgen
+    lib = OL.load()
+    out = {"metric": "analysis rows: GPU seconds vs oracle seconds (one host core), same input"}
+    # trajectory text I/O
+    rng = np.random.default_rng(seed)
+    m = 1_000_000
+    x, y, z = (trajgen.quantize(rng.random(m) * 3.0 - 1.0) for _ in range(3))
+    t = (rng.random(m) < 0.2).astype(np.int32)
+    import ctypes
+    buf = ctypes.create_string_buffer(80 * m + 1)
+    t0 = time.perf_counter()
+    k_ref = lib.or_frame_printf(m, OL.dp(x), OL.dp(y), OL.dp(z), OL.ip(t), buf)
+    t_printf = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    txt = ac4.format_frame(x, y, z, t)
+    t_fmt = time.perf_counter() - t0
+    assert txt == buf.raw[:k_ref]
+    xs, ys, zs = np.empty(m), np.empty(m), np.empty(m)
+    ts = np.empty(m, dtype=np.int32)
+    t0 = time.perf_counter()
+    assert lib.or_frame_strtod(txt, m, OL.dp(xs), OL.dp(ys), OL.dp(zs), OL.ip(ts)) == len(txt)
+    t_strtod = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    xp, yp, zp, tp, used = ac4.parse_frame(txt, m)
+    t_parse = time.perf_counter() - t0
+    assert used == len(txt) and np.array_equal(xp, xs) and np.array_equal(tp, ts) and np.array_equal(zp, zs)
+    out["trajectory_io"] = {"frame_bytes": len(txt), "format_s": t_fmt, "sprintf_s": t_printf,
+                            "parse_s": t_parse, "strtod_s": t_strtod,
+                            "format_GBps": len(txt) / t_fmt / 1e9, "parse_GBps": len(txt) / t_parse / 1e9}
+    log("rows: frame text %.1f MB: format %.3f s (sprintf %.3f s), parse %.3f s (strtod %.3f s)"
+        % (len(txt) / 1e6, t_fmt, t_printf, t_parse, t_strtod))
+    # trajectory
+    frames = trajgen.case_frames(n_origins, gap)
+    X, Y, Z, _ = trajgen.make_frames(n, frames, seed, 0.06, 0.5)
+    L = (n / 1.2) ** (1.0 / 3.0)
+    act = np.zeros(n, dtype=np.int32)
+    act[np.arange(0, n, 10)] = 1
+    fr = ac4.Frames(n, L, len(frames))
+    for k_, st in enumerate(frames):
+        fr.push(X[k_], Y[k_], Z[k_], st)
+    pc, _, _, _ = OL.walk(np.array(frames, dtype=np.int32), gap, 1)
+    work = {"N": n, "frames": len(frames), "origins": n_origins, "gap_steps": gap, "pairs": len(pc),
+            "active": int(act.sum())}
+    out["workload"] = work
+    log("rows: N=%d, %d frames, %d (origin, lag) pairs" % (n, len(frames), len(pc)))
+    with tempfile.TemporaryDirectory() as tmp:
+        cuts = OL.block_scales(L)
+        fr.block(gap, cuts[0], os.path.join(tmp, "warm"))  # first call: module load
+        t0 = time.perf_counter()
+        fr.block(gap, cuts[0], os.path.join(tmp, "g1"))
+        t_g1 = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        for c in cuts:
+            fr.block(gap, c, os.path.join(tmp, "g"))
+        t_gall = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        OL.block_all(n, L, X, Y, Z, frames, gap, os.path.join(tmp, "o"), cuts=cuts[:1])
+        t_o1 = time.perf_counter() - t0
+        for d in ("MSD", "Relax"):
+            for f_ in os.listdir(os.path.join(tmp, "o", "BlockAnalysis", d)):
+                a_ = open(os.path.join(tmp, "o", "BlockAnalysis", d, f_), "rb").read()
+                assert a_ == open(os.path.join(tmp, "g1", "BlockAnalysis", d, f_), "rb").read(), f_
+        out["block"] = {"gpu_one_edge_s": t_g1, "oracle_one_edge_s": t_o1, "speedup": t_o1 / t_g1,
+                        "gpu_17_edges_s": t_gall, "files_identical": True}
+        log("rows: block analysis, edge L/3: GPU %.3f s, oracle %.2f s; 17 edges on the GPU %.2f s"
+            % (t_g1, t_o1, t_gall))
+        tau = 0.005 * gap * 2
+        t0 = time.perf_counter()
+        hg, fn, no = fr.guu(act, gap, tau, path=os.path.join(tmp, "guu_g.dat"))
+        t_gg = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        ho, fno, noo = OL.guu(n, L, X, Y, Z, act, frames, gap, tau, path=os.path.join(tmp, "guu_o.dat"))
+        t_go = time.perf_counter() - t0
+        assert (fn, no) == (fno, noo) and np.array_equal(hg[0], ho[0])
+        out["guu"] = {"gpu_s": t_gg, "oracle_s": t_go, "speedup": t_go / t_gg, "lag_steps": fn, "origins": no,
+                      "active_pairs_per_origin": int(act.sum() * (act.sum() - 1) // 2),
+                      "max_rel_dev_weighted": float(np.max(np.abs(hg[1:] - ho[1:]) / (np.abs(ho[1:]) + 1e-300)))}
+        log("rows: g_uu, lag %d steps, %d origins: GPU %.3f s, oracle %.2f s" % (fn, no, t_gg, t_go))
+        t0 = time.perf_counter()
+        cg = fr.overlap_cage(gap, path=os.path.join(tmp, "cage_g.dat"))
+        t_cg = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        co = OL.overlap_cage(n, L, X, Y, Z, frames, gap, path=os.path.join(tmp, "cage_o.dat"))
+        t_co = time.perf_counter() - t0
+        assert np.array_equal(cg, co)
+        assert open(os.path.join(tmp, "cage_g.dat"), "rb").read() == open(os.path.join(tmp, "cage_o.dat"),
+                                                                           "rb").read()
+        out["cage_overlap"] = {"gpu_s": t_cg, "oracle_s": t_co, "speedup": t_co / t_cg, "files_identical": True}
+        log("rows: cage overlap: GPU %.3f s, oracle %.2f s" % (t_cg, t_co))
+    fr.close()
+    out["cores"] = 1
+    out["kind"] = "port"
+    return out
+
+
 def pmc_probe(ac4):
     """For hardware-counter runs (rocprofv3 --pmc): the batched workload with
     eager launches (every dispatch is counted), then three device-to-device
@@ -354,6 +467,8 @@ def main():
                     help="short eager run + 256 MiB copies, for rocprofv3 --pmc (prints no bench line)")
     ap.add_argument("--cfg5", action="store_true",
                     help="the N=10^6 box with frames in HBM and S4 on |q| shells (its own JSON line)")
+    ap.add_argument("--rows", action="store_true",
+                    help="the analysis rows (frame text I/O, block analysis, g_uu, cage overlap), GPU vs oracle")
     ap.add_argument("--only-batched", action="store_true",
                     help="batched ensemble only (no plain, serving or CPU legs), e.g. under a profiler")
     a = ap.parse_args()
@@ -368,6 +483,9 @@ def main():
         return
     if a.cfg5:
         print(json.dumps(cfg5(ac4)), flush=True)
+        return
+    if a.rows:
+        print(json.dumps(rows(ac4)), flush=True)
         return
 
     ps = ensemble_params(ac4, a.replicas)

@@ -207,4 +207,11 @@ int or_overlap_cage(int N, double L, int nframes, const int *fidx, int gap,
                     double factor, double a, double cell_cut, const char *path,
                     int *pair_counts, int cap_pairs, int *npairs);
 
+/* glibc baselines of the frame text (MDcode/print.c:10,
+ * Analysis/read_all_data.c:81): sprintf / strtod per line. */
+long long or_frame_printf(int n, const double *x, const double *y,
+                          const double *z, const int *t, char *buf);
+long long or_frame_strtod(const char *buf, int n, double *x, double *y,
+                          double *z, int *t);
+
 #endif

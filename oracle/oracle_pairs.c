@@ -6,7 +6,10 @@
  *                   nearest tau (the frameindx = 3 branch, guuTauAlphaAct.dat);
  *   or_overlap_cage Analysis/overlap_cage_rel.c:4-269, the overlap of each
  *                   particle's displacement relative to the mean displacement
- *                   of its 1.3-sigma neighbours at the origin.
+ *                   of its 1.3-sigma neighbours at the origin;
+ *   or_frame_printf, or_frame_strtod  glibc sprintf / strtod per line of the
+ *                   frame text (MDcode/print.c:10, Analysis/read_all_data.c:81),
+ *                   the CPU baseline of the library's frame formatter/parser.
  * One MPI rank: NPP = N, GatherVec and MPI_Reduce return the local values.
  *
  * Parity pins (tests/test_oracle_pairs.py):
@@ -334,4 +337,30 @@ int or_overlap_cage(int N, double L, int nframes, const int *fidx, int gap, cons
     free(c); free(x4); free(delta); free(counters);
     free(nn); free(nb); free(wx); free(wy); free(wz); free(DX); free(DY); free(DZ);
     return rc;
+}
+
+/* ------------------------------------------------------- trajectory text I/O */
+
+/* MDcode/print.c:10: one "%.5lf\t%.5lf\t%.5lf\t%d\n" line per particle with
+ * glibc's printf; returns the bytes written (buf must hold 80*n). */
+long long or_frame_printf(int n, const double *x, const double *y, const double *z, const int *t, char *buf) {
+    char *o = buf;
+    for (int i = 0; i < n; i++) o += sprintf(o, "%.5lf\t%.5lf\t%.5lf\t%d\n", x[i], y[i], z[i], t[i]);
+    return (long long)(o - buf);
+}
+
+/* Analysis/read_all_data.c:81: the values fscanf("%lf %lf %lf %d") returns,
+ * via strtod/strtol; returns the bytes consumed or -1. */
+long long or_frame_strtod(const char *buf, int n, double *x, double *y, double *z, int *t) {
+    char *e = (char *)buf;
+    for (int i = 0; i < n; i++) {
+        const char *s0 = e;
+        x[i] = strtod(e, &e);
+        y[i] = strtod(e, &e);
+        z[i] = strtod(e, &e);
+        t[i] = (int)strtol(e, &e, 10);
+        if (e == s0) return -1;
+        if (*e == '\n') e++;
+    }
+    return (long long)(e - buf);
 }

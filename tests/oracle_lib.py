@@ -59,6 +59,8 @@ SIGS = {
     "or_block_analyse_dir": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_double]),
     "or_guu": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_int, I, ctypes.c_int, D, D, D, I,
                               ctypes.c_double, ctypes.c_double, ctypes.c_char_p, D, I, I, I]),
+    "or_frame_printf": (ctypes.c_longlong, [ctypes.c_int, D, D, D, I, ctypes.c_char_p]),
+    "or_frame_strtod": (ctypes.c_longlong, [ctypes.c_char_p, ctypes.c_int, D, D, D, I]),
     "or_cage_list": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_double, D, D, D, ctypes.c_int, I, I]),
     "or_overlap_cage": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_int, I, ctypes.c_int, D, D, D,
                                        ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_char_p, I,
