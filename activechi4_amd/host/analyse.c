@@ -10,6 +10,11 @@
  * AC4_BIN=1 reads the frames from data_coor.bin (written by run with
  * AC4_BIN=1: the exact values data_coor.out parses to, so every output file
  * is the same) instead of parsing the text.
+ * Observables whose calls Analysis/main.c has commented out: AC4_GUU=1 adds
+ * guuTauAlphaAct.dat (guu(factor, tauAlpha), main.c:64; active particles
+ * from ActiveFileIndex.dat, read_all_data.c:49-58) and AC4_CAGE=1 adds
+ * relaxMPICR1_a_0.30.dat (overlap_cage, main.c:57).  Both keep every frame
+ * in HBM (an ac4_frames store, 24 B per particle and frame).
  */
 #include <math.h>
 #include <stdio.h>
@@ -18,6 +23,8 @@
 #include <stdint.h>
 #include "ac4.h"
 #include "frame_reader.h"
+
+static int env_on(const char *k) { return getenv(k) && atoi(getenv(k)); }
 
 static void die(const char *what, int rc) {
     fprintf(stderr, "analyse: %s failed (%d): %s\n", what, rc, ac4_last_error());
@@ -48,16 +55,42 @@ int main(int argc, char **argv) {
     CK(ac4_ana_create(n, L, nframes, fidx, 0, 1.4, 0.3, obs, 0, &an));
     double *x = malloc(sizeof(double) * n), *y = malloc(sizeof(double) * n), *z = malloc(sizeof(double) * n);
     int *ty = malloc(sizeof(int) * n);
+    const int guu = env_on("AC4_GUU"), cage = env_on("AC4_CAGE");
+    ac4_frames *fs = NULL;
+    if (guu || cage) CK(ac4_frames_create(n, L, nframes, &fs));
     frame_reader rd;
-    if (fr_open(&rd, sub_dir, n, getenv("AC4_BIN") && atoi(getenv("AC4_BIN")))) return 1;
+    if (fr_open(&rd, sub_dir, n, env_on("AC4_BIN"))) return 1;
     for (int fr = 0; fr < nframes; fr++) {
         if (fr_next(&rd, x, y, z, ty)) return 1;
         CK(ac4_ana_push_frame(an, fr, x, y, z, ty));
+        if (fs) CK(ac4_frames_push(fs, x, y, z, fidx[fr]));
     }
     fr_close(&rd);
     CK(ac4_ana_finalize(an));
     CK(ac4_ana_write(an, sub_dir));
     printf("Overlap Calculated\n");
     ac4_ana_destroy(an);
+    char path[640];
+    if (fs) {
+        const int gap = ac4_origin_gap(fidx, nframes);
+        if (gap <= 0) die("ac4_origin_gap", gap);
+        if (cage) {
+            snprintf(path, sizeof path, "%s/relaxMPICR1_a_%.2lf.dat", sub_dir, 0.3); /* overlap_cage_rel.c:158 */
+            CK(ac4_frames_overlap_cage(fs, gap, 1.4, 0.3, 2.5 + 0.35, path, NULL, 0, NULL));
+        }
+        if (guu) {
+            int *act = calloc(n, sizeof(int)), k; /* read_all_data.c:49-58 */
+            snprintf(path, sizeof path, "%s/ActiveFileIndex.dat", sub_dir);
+            FILE *fa = fopen(path, "rb");
+            if (!fa) { fprintf(stderr, "analyse: no %s\n", path); return 1; }
+            while (fscanf(fa, "%d", &k) == 1)
+                if (k >= 0 && k < n) act[k] = 1;
+            fclose(fa);
+            snprintf(path, sizeof path, "%s/guuTauAlphaAct.dat", sub_dir); /* guu.c:262 */
+            CK(ac4_frames_guu(fs, act, gap, 1.4, atof(argv[3]), path, NULL, NULL, NULL, NULL));
+            free(act);
+        }
+        ac4_frames_destroy(fs);
+    }
     return 0;
 }

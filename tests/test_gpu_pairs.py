@@ -81,3 +81,25 @@ def test_cage_overlap_needs_three_cells(gpu):
     with pytest.raises(gpu.AC4Error, match="at least 3"):
         fr.overlap_cage(20)
     fr.close()
+
+
+def test_analyse_dropin_guu_and_cage(gpu, tmp_path):
+    """bin/analyse with AC4_GUU=1 and AC4_CAGE=1 on a golden job directory."""
+    import subprocess
+    name = "walk_g20"
+    meta, frames, X, Y, Z, T = golden_case(name)
+    n = meta["n"]
+    job = tmp_path / "job"
+    trajgen.write_job(str(job), frames, X, Y, Z, T)
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "activechi4_amd", "bin",
+                       "analyse")
+    env = dict(os.environ, AC4_DIR=str(job), AC4_GUU="1", AC4_CAGE="1")
+    out = subprocess.run([exe, "0", "0.8", repr(meta["guu_tau"]), str(n)], env=env, capture_output=True, text=True,
+                         timeout=300)
+    assert out.returncode == 0, out.stderr
+    got = [line.split("\t") for line in open(job / "guuTauAlphaAct.dat").read().splitlines()]
+    want = [line.split("\t") for line in open(os.path.join(GOLD, name, "guuTauAlphaAct.dat")).read().splitlines()]
+    assert len(got) == len(want) and all(g[:2] == w[:2] for g, w in zip(got, want))
+    L = (n / 1.2) ** (1.0 / 3.0)
+    OL.overlap_cage(n, L, X, Y, Z, frames, meta["gap"], path=str(tmp_path / "cage.dat"))
+    assert open(job / "relaxMPICR1_a_0.30.dat", "rb").read() == open(tmp_path / "cage.dat", "rb").read()
