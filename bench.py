@@ -414,7 +414,78 @@ def rows(ac4, n=50000, n_origins=20, gap=100, seed=777):
     fr.close()
     out["cores"] = 1
     out["kind"] = "port"
+    out["reference_programs"] = reference_programs(ac4, n, frames, X, Y, Z, act)
     return out
+
+
+def reference_programs(ac4, n, frames, X, Y, Z, act):
+    """The reference's own Analysis_Block program and its Analysis overlap
+    and MSD (built from the reference sources into oracle/_ref/ when those
+    were present at build time) against the drop-in executables on the GPU,
+    on the same job directory: time to completion, text parsing included,
+    and the output files compared byte for byte."""
+    import shutil
+    import subprocess
+    import tempfile
+    ref = os.path.join(ROOT, "oracle", "_ref")
+    blk_ref, ana_ref = os.path.join(ref, "analyse_block_ref"), os.path.join(ref, "analyse_ref")
+    blk_ours, ana_ours = (os.path.join(ROOT, "activechi4_amd", "bin", e) for e in ("analyse_block", "analyse"))
+    if not (os.path.exists(blk_ref) and os.path.exists(ana_ref)) or n != 50000:
+        return None  # the reference programs fix N = 50000 at compile time
+    res = {"kind": "reference", "cores": 1, "sample": "N=%d, %d frames of %%.5lf text" % (n, len(frames))}
+    with tempfile.TemporaryDirectory() as tmp:
+        run = os.path.join(tmp, "run")
+        job = os.path.join(tmp, "0.800", "data_000")
+        os.makedirs(run)
+        os.makedirs(job)
+        with open(os.path.join(run, "para_file"), "w") as f:
+            f.write("DENSITY\tT\tP\tDIAMETER\tnb\ttau_alpha\n1.2 .80000    0.1\t0.3\t1\t.0\n"
+                    "start_ensemble\tend_ensemble\n0\t0\n")
+        with open(os.path.join(job, "time_index_file.dat"), "w") as f:
+            f.write("total timesteps equals %d\n" % len(frames) + "".join("%d\n" % v for v in frames))
+        with open(os.path.join(job, "ActiveFileIndex.dat"), "w") as f:
+            f.write("".join("%d\n" % i for i in np.nonzero(act)[0]))
+        t0 = time.perf_counter()
+        ty = np.zeros(n, dtype=np.int32)
+        ty[: n // 5] = 1
+        with open(os.path.join(job, "data_coor.out"), "wb") as f:
+            for k in range(len(frames)):
+                f.write(ac4.format_frame(X[k], Y[k], Z[k], ty))
+        res["write_text_s"] = time.perf_counter() - t0
+        res["text_MB"] = os.path.getsize(os.path.join(job, "data_coor.out")) / 1e6
+        log("rows: reference programs on %.0f MB of frame text" % res["text_MB"])
+
+        def timed(cmd, **kw):
+            t = time.perf_counter()
+            subprocess.run(cmd, check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, timeout=900, **kw)
+            return time.perf_counter() - t
+
+        def tree(root):
+            return {os.path.relpath(os.path.join(d, f_), root): open(os.path.join(d, f_), "rb").read()
+                    for d, _, fs in os.walk(root) for f_ in fs}
+        ba = os.path.join(job, "BlockAnalysis")
+        t_ref = timed([blk_ref, "0", "0.8", "0"], cwd=run)
+        want = tree(ba)
+        shutil.rmtree(ba)
+        t_gpu = timed([blk_ours, "0", "0.8", "0", str(n)], cwd=run)
+        got = tree(ba)
+        res["analysis_block"] = {"reference_s": t_ref, "gpu_s": t_gpu, "speedup": t_ref / t_gpu,
+                                 "files": len(want), "files_identical": got == want}
+        log("rows: Analysis_Block: reference %.1f s, drop-in on the GPU %.2f s, %d files identical: %s"
+            % (t_ref, t_gpu, len(want), got == want))
+        t_ref = timed([ana_ref, job, "mo"])
+        want = {f_: open(os.path.join(job, f_), "rb").read() for f_ in ("msd.dat", "relax_a_0.30.dat")}
+        t_gpu = timed([ana_ours, "0", "0.8", "1.0", str(n)], env=dict(os.environ, AC4_DIR=job))
+        a = np.loadtxt(os.path.join(job, "msd.dat"))
+        b = np.loadtxt(__import__("io").BytesIO(want["msd.dat"]))
+        dev = float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300))) if a.shape == b.shape else None
+        res["analysis_msd_overlap"] = {"reference_s": t_ref, "gpu_s": t_gpu, "speedup": t_ref / t_gpu,
+                                       "gpu_also_computes": "S4qt_a_0.30.dat",
+                                       "relax_identical": open(os.path.join(job, "relax_a_0.30.dat"), "rb").read()
+                                       == want["relax_a_0.30.dat"], "msd_max_rel_dev": dev}
+        log("rows: Analysis MSD + overlap: reference %.1f s, drop-in on the GPU (also S4) %.2f s"
+            % (t_ref, t_gpu))
+    return res
 
 
 def pmc_probe(ac4):
