@@ -75,6 +75,11 @@ void or_forces_brute(int n, const double *x, const double *y, const double *z,
                      const int *type, double L, const int *act_tag,
                      const int *kx, const int *ky, const int *kz, double f0,
                      double *fx, double *fy, double *fz, double *out2);
+/* Forces on m chosen particles by an O(N) pass each (f3: [m][3]). */
+void or_forces_sample(int n, const double *x, const double *y, const double *z,
+                      const int *type, double L, const int *act_tag,
+                      const int *kx, const int *ky, const int *kz, double f0,
+                      int m, const int *idx, double *f3);
 
 /* MDcode/time_evolve.c:476-482: q[0]=DOF*T*tau^2, q[1..2]=T*tau^2 */
 void or_nhc_masses(int n, double T, double *q3);

@@ -250,6 +250,47 @@ void or_forces_brute(int n, const double *x, const double *y, const double *z,
     out2[1] = vir / (3.0e0);
 }
 
+/* Forces on m chosen particles idx[] by an O(N) pass over every partner
+ * j != i in index order, with the arithmetic of pair_force (the
+ * contribution to i only) plus the active force: a full-size check that
+ * needs no neighbour list.  f3[3*k..3*k+2] = force on particle idx[k]. */
+void or_forces_sample(int n, const double *x, const double *y, const double *z, const int *type, double L,
+                      const int *act_tag, const int *kx, const int *ky, const int *kz, double f0, int m,
+                      const int *idx, double *f3) {
+    for (int k = 0; k < m; k++) {
+        const int i = idx[k], A = type[i];
+        double fx = 0.0, fy = 0.0, fz = 0.0;
+        for (int j = 0; j < n; j++) {
+            if (j == i) continue;
+            const int B = type[j];
+            double dx = minimg_md(x[i] - x[j]);
+            double dy = minimg_md(y[i] - y[j]);
+            double dz = minimg_md(z[i] - z[j]);
+            double dr2 = dr2_md(dx, dy, dz, L);
+            double eps = or_eps[A][B];
+            double sig = or_sigma[A][B];
+            double r_over_sig_2 = dr2 / (sig * sig);
+            if (r_over_sig_2 < OR_LCUTSQR) {
+                double s2 = 1.0 / r_over_sig_2;
+                double s6 = s2 * s2 * s2;
+                double grad = 8.0 * eps * (6.0 * s6 * (s6 - 0.5) / dr2 - OR_C2 / (sig * sig));
+                fx += grad * dx * L;
+                fy += grad * dy * L;
+                fz += grad * dz * L;
+            }
+        }
+        if (act_tag && act_tag[i] != -1) {
+            const int a = act_tag[i];
+            fx += f0 * (double)kx[a];
+            fy += f0 * (double)ky[a];
+            fz += f0 * (double)kz[a];
+        }
+        f3[3 * k] = fx;
+        f3[3 * k + 1] = fy;
+        f3[3 * k + 2] = fz;
+    }
+}
+
 void or_nhc_masses(int n, double T, double *q3) {
     double tau = .20; /* time_evolve.c:477-482 */
     for (int i = 0; i < OR_M; i++) q3[i] = T * tau * tau;

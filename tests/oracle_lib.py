@@ -32,6 +32,8 @@ SIGS = {
     "or_forces_list": (None, [ctypes.c_int, D, D, D, D, D, D, I, ctypes.c_double, I, I, ctypes.c_int, I, I, I, I,
                               ctypes.c_double, D, D, D, D]),
     "or_forces_brute": (None, [ctypes.c_int, D, D, D, I, ctypes.c_double, I, I, I, I, ctypes.c_double, D, D, D, D]),
+    "or_forces_sample": (None, [ctypes.c_int, D, D, D, I, ctypes.c_double, I, I, I, I, ctypes.c_double,
+                                ctypes.c_int, I, D]),
     "or_nhc_masses": (None, [ctypes.c_int, ctypes.c_double, D]),
     "or_nhc_steps": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                     ctypes.c_int, ctypes.c_int, ctypes.c_int] + [D] * 12 + [I] * 5 + [D] * 4 + [D]),
@@ -276,3 +278,14 @@ def overlap_cage(N, L, X, Y, Z, fidx, gap, path=None, a=0.3, factor=1.4):
                                 os.fsencode(path) if path else None, ip(pc), cap, ctypes.byref(npairs))
     assert rc == 0
     return pc[:npairs.value]
+
+
+def forces_sample(s, L, f0, idx):
+    """or_forces_sample: forces on particles idx by an O(N) pass each, (m, 3)."""
+    idx = np.ascontiguousarray(idx, dtype=np.int32)
+    out = np.zeros((len(idx), 3))
+    g = {k: np.ascontiguousarray(s[k], dtype=np.float64) for k in ("x", "y", "z")}
+    g.update({k: np.ascontiguousarray(s[k], dtype=np.int32) for k in ("type", "act_tag", "kx", "ky", "kz")})
+    load().or_forces_sample(len(g["x"]), dp(g["x"]), dp(g["y"]), dp(g["z"]), ip(g["type"]), L, ip(g["act_tag"]),
+                            ip(g["kx"]), ip(g["ky"]), ip(g["kz"]), f0, len(idx), ip(idx), dp(out))
+    return out

@@ -129,3 +129,32 @@ def test_fp32_overlap_statistics_match_fp64(gpu):
     d = q32.mean(axis=0) - q64.mean(axis=0)
     print("Q(t) FP32 - FP64: max |dQ| %.4f, max |dQ|/SE %.2f" % (np.abs(d).max(), (np.abs(d) / se).max()))
     assert np.all(np.abs(d) <= 4.0 * se + 1e-3)
+
+
+def test_forces_at_one_million_particles(gpu):
+    """The single large box (N=10^6, L=94.1): the GPU's FP64 forces against the
+    oracle's O(N) brute force on 256 sampled particles, FP32 forces against
+    FP64 for every particle, and Newton-3 (the pair forces sum to zero)."""
+    n, T, f0 = 1_000_000, 0.5, 1.0
+    s, L = lattice_state(n, T=T, jitter=0.03, seed=31)
+    md64 = make_md(gpu, [(s, L)], f0=f0, T=T, prec=gpu.FP64)
+    md64.calculate_forces()
+    F = md64.forces(0)
+    md64.close()
+    idx = np.random.default_rng(4).choice(n, 256, replace=False)
+    fo = OL.forces_sample(s, L, f0, idx)
+    err64 = _rel_force_err((F[0][idx], F[1][idx], F[2][idx]), (fo[:, 0], fo[:, 1], fo[:, 2]))
+    act = s["act_tag"] >= 0
+    a = s["act_tag"][act]
+    fact = [f0 * np.asarray(s[k], dtype=float)[a].sum() for k in ("kx", "ky", "kz")]
+    fscale = np.sqrt((F[0] ** 2 + F[1] ** 2 + F[2] ** 2).mean())
+    net = max(abs(F[c].sum() - fact[c]) for c in range(3)) / (fscale * n)
+    md32 = make_md(gpu, [(s, L)], f0=f0, T=T, prec=gpu.FP32)
+    md32.calculate_forces()
+    err32 = _rel_force_err(md32.forces(0), F)
+    md32.close()
+    print("N=1e6: FP64 vs oracle %.2e (256 particles), FP32 vs FP64 %.2e, net pair force %.1e per particle"
+          % (err64, err32, net))
+    assert err64 < 1e-12
+    assert err32 <= 1e-5
+    assert net < 1e-10

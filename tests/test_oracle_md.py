@@ -243,3 +243,13 @@ def test_schedule_hand_case(oracle):
     # one origin, run length 4: entries 0,1,2,3 (all distinct) -> 3 is dropped
     sched, raw = OL.schedule(4, n_origins=1)
     assert sched.tolist() == [0, 1, 2] and int(raw[3]) == 3
+
+
+def test_forces_sample_matches_brute_force(oracle):
+    """or_forces_sample (O(N) per chosen particle, the full-size check) gives
+    the O(N^2) brute-force forces of those particles."""
+    s, L = lattice_state(700, jitter=0.08, seed=9)
+    fx, fy, fz, _ = OL.forces_brute(s, L, 1.3)
+    idx = np.arange(0, 700, 7)
+    f = OL.forces_sample(s, L, 1.3, idx)
+    assert _rel_force_err((f[:, 0], f[:, 1], f[:, 2]), (fx[idx], fy[idx], fz[idx])) < 1e-12
