@@ -89,3 +89,41 @@ def test_insitu_push_equals_host_push(gpu):
     for o in (gpu.OBS_MSD, gpu.OBS_OVERLAP):
         assert np.array_equal(a1.pairs(o), a2.pairs(o))
     assert np.array_equal(a1.pairs(gpu.OBS_S4)[0], a2.pairs(gpu.OBS_S4)[0])
+
+
+def test_cfg1_full_schedule_matches_oracle(gpu):
+    """The reference's CPU-runnable configuration at full length: KA N=1000,
+    T=0.6, f0=1.5, tau_p=1.0 (200 steps), 1e4 steps on the GPU with frames on
+    the reference schedule (4449 frames, 200 origins).  The MSD and overlap of
+    those frames on the GPU against the oracle: all 4891 (origin, lag)
+    overlap counts and the per-slot sums exact, MSD within 1e-12."""
+    from test_gpu_md import make_relaxed_md
+    n, steps = 1000, 10000
+    fidx = [int(v) for v in gpu.schedule(steps, 200)[0]]
+    assert len(fidx) == 4449
+    md = make_relaxed_md(gpu, [(n, 7, 5)], tau=200, prec=0)
+    X, Y, Z = (np.empty((len(fidx), n)) for _ in range(3))
+    t = 0
+    for f, fi in enumerate(fidx):
+        if fi > t:
+            md.step(fi - t, graph=True)
+            t = fi
+        g = md.get_state(0)
+        X[f], Y[f], Z[f] = g["xu"], g["yu"], g["zu"]
+    md.close()
+    T = np.zeros((len(fidx), n), dtype=np.int32)
+    L = OL.load().or_ana_box_length(n)
+    an = run_product(gpu, n, L, fidx, X, Y, Z, T, obs=gpu.OBS_MSD | gpu.OBS_OVERLAP)
+    assert an.gap == 50
+    oo = OL.overlap(n, L, X, Y, Z, T, fidx, 50)
+    assert len(oo["pairs"]) == 4891
+    assert np.array_equal(an.pairs(gpu.OBS_OVERLAP), oo["pairs"])
+    ov = an.overlap()
+    mg, mo = ov["counters"] > 0, oo["counters"] > 0
+    assert np.array_equal(np.nonzero(mg)[0], np.nonzero(mo)[0])
+    assert np.array_equal(ov["counters"][mg], oo["counters"][mo])
+    assert np.array_equal(ov["c"][mg], oo["c"][mo]) and np.array_equal(ov["x4"][mg], oo["x4"][mo])
+    om = OL.msd(n, L, X, Y, Z, T, fidx, 50)
+    assert np.allclose(an.pairs(gpu.OBS_MSD), om["pairs"], rtol=1e-12, atol=0)
+    q = oo["c"][mo] / oo["counters"][mo]
+    assert q[0] > 0.9 and q[-1] < q[0]  # the overlap decays over the run
