@@ -97,6 +97,7 @@ _SIGS = {
     "ac4_frames_count": (ctypes.c_int, [_VP]),
     "ac4_frames_find": (ctypes.c_int, [_VP, ctypes.c_longlong]),
     "ac4_frames_overlap": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_double, _I]),
+    "ac4_md_fix_drift": (ctypes.c_int, [_VP, ctypes.c_int, _D]),
     "ac4_frames_block": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                         ctypes.c_double, ctypes.c_char_p]),
     "ac4_frames_overlap_cage": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
@@ -282,6 +283,12 @@ class MD:
     def set_nve(self, r, on=True):
         """Plain velocity Verlet for replica r (ac4_md_set_nve); Heff = KE + PE."""
         _check(lib.ac4_md_set_nve(self.h, r, 1 if on else 0))
+
+    def fix_drift(self, r=0):
+        """Remove the mean momentum of replica r (ac4_md_fix_drift); returns it."""
+        m = np.zeros(3)
+        _check(lib.ac4_md_fix_drift(self.h, r, _dp(m)))
+        return m
 
     def checkpoint(self, r=0):
         """Exact checkpoint of replica r as bytes (ac4_md_checkpoint)."""

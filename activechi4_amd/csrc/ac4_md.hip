@@ -1748,6 +1748,55 @@ extern "C" int ac4_md_set_chain(ac4_md *md, int r, const double *c9) {
     return put_chain(md, r, &c);
 }
 
+// fixDrift (MDcode/initialise.c:104-138) for replica r: the momenta summed in
+// reference particle order and divided by N are subtracted from every
+// particle.  The pending end-of-step U3 factor is applied first (the
+// reference's momenta are final after a step), and the sum of p^2 the next
+// step's thermostat starts from is recomputed.  mean3 (nullable) receives
+// the mean that was removed.
+extern "C" int ac4_md_fix_drift(ac4_md *md, int r, double *mean3) {
+    TRY(check_rep(md, r));
+    const RepC &R = md->reps_h[r];
+    const int n = R.n;
+    AC4_REQUIRE(n > 0, AC4_ESTATE, "fix_drift: replica %d holds no particles", r);
+    std::vector<int> id;
+    TRY(get_ids(md, r, id));
+    ChainS c;
+    TRY(get_chain(md, r, &c));
+    double *dev[3] = {md->P.px, md->P.py, md->P.pz};
+    std::vector<double> p[3], tmp(n);
+    for (int a = 0; a < 3; a++) {
+        p[a].resize(n);
+        TRY(d2h(tmp.data(), dev[a] + R.off, n, md->stream));
+        AC4_HIP(hipStreamSynchronize(md->stream));
+        for (int k = 0; k < n; k++) p[a][id[k]] = tmp[k] * c.p_scale;
+    }
+    double m[3];
+    for (int a = 0; a < 3; a++) {
+        double sum = 0.0;
+        for (int i = 0; i < n; i++) sum += p[a][i];
+        m[a] = sum / (double)n;
+    }
+    double keblk = 0.0;  // time_evolve.c:586-589 order
+    for (int i = 0; i < n; i++) {
+        p[0][i] -= m[0];
+        p[1][i] -= m[1];
+        p[2][i] -= m[2];
+        keblk += p[0][i] * p[0][i] + p[1][i] * p[1][i] + p[2][i] * p[2][i];
+    }
+    for (int a = 0; a < 3; a++) {
+        for (int k = 0; k < n; k++) tmp[k] = p[a][id[k]];
+        TRY(h2d(dev[a] + R.off, tmp.data(), n, md->stream));
+        AC4_HIP(hipStreamSynchronize(md->stream));
+    }
+    c.p_scale = 1.0;
+    c.S_end = keblk;
+    TRY(put_chain(md, r, &c));
+    if (mean3)
+        for (int a = 0; a < 3; a++) mean3[a] = m[a];
+    return AC4_OK;
+}
+
 extern "C" int ac4_md_get_chain(ac4_md *md, int r, double *c9) {
     TRY(check_rep(md, r));
     ChainS c;

@@ -342,6 +342,7 @@ int main(int argc, char **argv) {
             i = next;
             if (i % frame_step == 0) print_stuff(md, stuff_file, n);
             if (i % 20000 == 0) { write_restart(md, &S, restart_file, L, (int)i, 0); save_dirs(&S, path); }
+            if (i % 100000 == 0) CK(ac4_md_fix_drift(md, 0, NULL)); /* MDcode/main.c:188 */
         }
         pull(md, &S);
         for (int i = 0; i < n; i++) { S.xu[i] = S.x[i]; S.yu[i] = S.y[i]; S.zu[i] = S.z[i]; } /* :197-203 */
@@ -381,8 +382,10 @@ int main(int argc, char **argv) {
         if (index < cap && (long)next_idx[index] >= i && (long)next_idx[index] < ev) ev = (long)next_idx[index];
         long a = ((i + frame_step - 1) / frame_step) * (long)frame_step;
         long b = ((i + every - 1) / every) * every;
+        long c = ((i + 100000 - 1) / 100000) * 100000L;
         if (a < ev) ev = a;
         if (b < ev) ev = b;
+        if (c < ev) ev = c;
         CK(ac4_md_step(md, (int)(ev - i + 1)));
         if (index < cap && (unsigned long long)ev == next_idx[index]) {
             print_data(md, &S, data_file, bin_file, ev);
@@ -390,6 +393,7 @@ int main(int argc, char **argv) {
         }
         if (ev % frame_step == 0) print_stuff(md, stuff_file, n);
         i = ev + 1;
+        if (ev % 100000 == 0) CK(ac4_md_fix_drift(md, 0, NULL)); /* MDcode/main.c:261, before the restart */
         if (ev % every == 0) {
             write_restart(md, &S, restart_file, L, (int)ev, index - 1);
             save_dirs(&S, path);

@@ -220,6 +220,12 @@ int ac4_frames_overlap_cage(ac4_frames *f, int gap, double factor, double a, dou
 int ac4_frames_guu(ac4_frames *f, const int *act_tag, int gap, double factor, double tau, const char *path,
                    double *hist3, int *maxbin, int *frame_num, int *n_origins);
 
+/* fixDrift (MDcode/initialise.c:104-138): removes the mean momentum of
+ * replica r (summed in reference particle order, divided by N) from every
+ * particle; mean3 (nullable) receives the mean removed.  The reference calls
+ * it every 1e5 steps of its run loops (MDcode/main.c:188,261). */
+int ac4_md_fix_drift(ac4_md *md, int r, double *mean3);
+
 /* ------------------------------------------------------ exact checkpoints --
  * A superset of MDcode's restart_file.dat (main.c:272-334), which keeps only
  * positions, velocities and tags: the complete device state of replica r

@@ -376,3 +376,33 @@ def test_insitu_overlap_accumulators(gpu):
                 wc[lag] += 1
         assert np.array_equal(c, wc)
         assert np.array_equal(q, wq) and np.array_equal(q2, wq2)
+
+
+def test_fix_drift_removes_mean_momentum(gpu):
+    """ac4_md_fix_drift (fixDrift, MDcode/initialise.c:104-138): the mean
+    momentum, summed in reference particle order and divided by N, is
+    subtracted from every particle exactly; the run then continues."""
+    md = make_relaxed_md(gpu, [(1200, 91, 3)], tau=20, prec=0)
+    md.step(40)
+    md.sync()
+    g = md.get_state(0)
+    s0 = relaxed_state(gpu, 1200, 91)[0]
+    drift = np.array([0.3, -0.2, 0.1])
+    p = {k: g[k] + d for k, d in zip(("px", "py", "pz"), drift)}
+    md.set_state(0, g["x"], g["y"], g["z"], g["xu"], g["yu"], g["zu"], p["px"], p["py"], p["pz"], s0["type"],
+                 s0["act_tag"], g["kx"], g["ky"], g["kz"])
+    md.init_chain(0, seed=91)
+    md.step(10)
+    md.sync()
+    before = md.get_state(0)
+    m = md.fix_drift(0)
+    after = md.get_state(0)
+    for a, k in enumerate(("px", "py", "pz")):
+        assert m[a] == np.cumsum(before[k])[-1] / 1200  # sequential sum, as the reference's loop
+        assert np.array_equal(after[k], before[k] - m[a])
+        assert abs(after[k].sum()) < 1e-12
+    assert np.all(np.abs(m - drift) < 0.05)
+    md.step(20)
+    md.sync()
+    assert 0.3 < md.stats(0)["T"] < 1.2
+    md.close()
