@@ -27,6 +27,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <sys/stat.h>
+#include <time.h>
 #include "ac4.h"
 
 #define TIME_STEP 0.005 /* MDcode/global.h:33 */
@@ -332,6 +333,12 @@ int main(int argc, char **argv) {
         init_chain(md, n, T, &gs);
         long steps = (long)(eqDuration / TIME_STEP);
         snprintf(path, sizeof path, "%s/ActiveDirection.dat", sub_dir);
+        { /* the reference's 1000 timing steps before equilibration, MDcode/main.c:171-177 */
+            time_t t0 = time(0);
+            CK(ac4_md_step(md, 1000));
+            CK(ac4_md_sync(md));
+            printf("Time for 10000 Steps= %ld\n", (long)(time(0) - t0));
+        }
         print_stuff(md, stuff_file, n);
         for (long i = 0; i < steps;) {
             long next = steps;
