@@ -276,24 +276,6 @@ __global__ void __launch_bounds__(64) k_blk_tot(int noi, int ncell, long long to
 
 namespace {
 
-struct DevBuf {  // device allocations of one call, freed on every return path
-    std::vector<void *> p;
-    template <class T>
-    int get(T **out, size_t count) {
-        void *q = nullptr;
-        if (hipMalloc(&q, count ? sizeof(T) * count : 8) != hipSuccess) {
-            ac4_set_error("ac4_frames_block: cannot allocate %zu bytes", sizeof(T) * count);
-            return AC4_EHIP;
-        }
-        p.push_back(q);
-        *out = (T *)q;
-        return AC4_OK;
-    }
-    ~DevBuf() {
-        for (void *q : p) (void)hipFree(q);
-    }
-};
-
 // The NaN an x86-64 SSE division 0.0/0.0 returns (sign bit set; glibc prints
 // it as "-nan"): the only NaN the reference's arithmetic produces.
 double x86_nan(double v) {
@@ -328,12 +310,8 @@ extern "C" int ac4_frames_block(ac4_frames *f, int gap, double factor, double a,
                 "ac4_frames_block: bad arguments (gap %d, %d frames)", gap, v.count);
     const int nf = v.count, n = v.n;
     const double L = v.L;
-    std::vector<int> fidx(nf);
-    for (int k = 0; k < nf; k++) {
-        AC4_REQUIRE(v.step[k] >= 0 && v.step[k] < 2147483647LL, AC4_EARG,
-                    "ac4_frames_block: frame step %lld does not fit the reference's int", v.step[k]);
-        fidx[k] = (int)v.step[k];
-    }
+    std::vector<int> fidx;
+    BTRY(ac4_frames_steps(v, fidx));
     const int ncx = (int)(L / cell_cut);  // msd_pos.c:33
     AC4_REQUIRE(ncx >= 1 && (long long)ncx * ncx * ncx <= BLK_SORT_MAX, AC4_EARG,
                 "ac4_frames_block: %d blocks per edge (1..%d supported)", ncx, 25);
@@ -418,7 +396,7 @@ extern "C" int ac4_frames_block(ac4_frames *f, int gap, double factor, double a,
     }
 
     const size_t res = (size_t)npt * ncell, bs = (size_t)ncell * noi;
-    DevBuf B;
+    AcDevBuf B;
     int *d_ofr, *d_start, *d_mem, *d_ov, *d_sptr, *d_spairs, *d_cn, *d_wcnt;
     int2 *d_pairs;
     double *d_s2, *d_ex, *d_ey, *d_ez, *d_add, *d_c, *d_x4, *d_msd, *d_tot;

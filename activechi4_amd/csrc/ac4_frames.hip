@@ -185,6 +185,16 @@ extern "C" int ac4_frames_destroy(ac4_frames *f) {
 
 extern "C" int ac4_frames_count(const ac4_frames *f) { return f ? f->count : 0; }
 
+int ac4_frames_steps(const ac4_frames_view &v, std::vector<int> &fidx) {
+    fidx.resize(v.count);
+    for (int k = 0; k < v.count; k++) {
+        AC4_REQUIRE(v.step[k] >= 0 && v.step[k] < 2147483647LL, AC4_EARG,
+                    "frame step %lld does not fit the reference's int frame index", v.step[k]);
+        fidx[k] = (int)v.step[k];
+    }
+    return AC4_OK;
+}
+
 int ac4_frames_view_get(const ac4_frames *f, ac4_frames_view *v) {
     AC4_REQUIRE(f && v, AC4_EARG, "frames view: bad arguments");
     v->n = f->n;

@@ -4,6 +4,7 @@
 #include <stdint.h>
 #include <stdarg.h>
 #include <stdio.h>
+#include <vector>
 #include "ac4.h"
 #include "ac4_host.h"
 
@@ -87,3 +88,25 @@ struct ac4_frames_view {
     hipStream_t s;
 };
 int ac4_frames_view_get(const ac4_frames *f, ac4_frames_view *v);
+
+// Frame step numbers as the reference's int frame indices (fidx[k]).
+int ac4_frames_steps(const ac4_frames_view &v, std::vector<int> &fidx);
+
+// Device allocations of one call, freed on every return path.
+struct AcDevBuf {
+    std::vector<void *> p;
+    template <class T>
+    int get(T **out, size_t count) {
+        void *q = nullptr;
+        if (hipMalloc(&q, count ? sizeof(T) * count : 8) != hipSuccess) {
+            ac4_set_error("cannot allocate %zu bytes of device memory", sizeof(T) * count);
+            return AC4_EHIP;
+        }
+        p.push_back(q);
+        *out = (T *)q;
+        return AC4_OK;
+    }
+    ~AcDevBuf() {
+        for (void *q : p) (void)hipFree(q);
+    }
+};

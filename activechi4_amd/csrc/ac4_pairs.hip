@@ -334,38 +334,6 @@ __global__ void __launch_bounds__(GU_TILE) k_gu_hist(const double *__restrict__ 
         }
 }
 
-namespace {
-
-struct PrBuf {  // device allocations of one call, freed on every return path
-    std::vector<void *> p;
-    template <class T>
-    int get(T **out, size_t count) {
-        void *q = nullptr;
-        if (hipMalloc(&q, count ? sizeof(T) * count : 8) != hipSuccess) {
-            ac4_set_error("pair observables: cannot allocate %zu bytes", sizeof(T) * count);
-            return AC4_EHIP;
-        }
-        p.push_back(q);
-        *out = (T *)q;
-        return AC4_OK;
-    }
-    ~PrBuf() {
-        for (void *q : p) (void)hipFree(q);
-    }
-};
-
-int frames_fidx(const ac4_frames_view &v, std::vector<int> &fidx) {
-    fidx.resize(v.count);
-    for (int k = 0; k < v.count; k++) {
-        AC4_REQUIRE(v.step[k] >= 0 && v.step[k] < 2147483647LL, AC4_EARG,
-                    "frame step %lld does not fit the reference's int", v.step[k]);
-        fidx[k] = (int)v.step[k];
-    }
-    return AC4_OK;
-}
-
-}  // namespace
-
 #define PTRY(e) do { int rc_ = (e); if (rc_ != AC4_OK) return rc_; } while (0)
 
 extern "C" int ac4_frames_overlap_cage(ac4_frames *f, int gap, double factor, double a, double cell_cut,
@@ -377,7 +345,7 @@ extern "C" int ac4_frames_overlap_cage(ac4_frames *f, int gap, double factor, do
     const int n = v.n, nf = v.count;
     const double L = v.L;
     std::vector<int> fidx;
-    PTRY(frames_fidx(v, fidx));
+    PTRY(ac4_frames_steps(v, fidx));
     const int nc = (int)(L / cell_cut);  // maps(), :283
     AC4_REQUIRE(nc >= 3, AC4_EARG,
                 "ac4_frames_overlap_cage: %d cells per edge (L/cell_cut); at least 3 are needed", nc);
@@ -397,7 +365,7 @@ extern "C" int ac4_frames_overlap_cage(ac4_frames *f, int gap, double factor, do
         for (q = p; q < np && cur[q] == cur[p]; q++) {}
         maxlag = std::max(maxlag, q - p);
     }
-    PrBuf B;
+    AcDevBuf B;
     double *d_w, *d_D;
     int *d_cell, *d_start, *d_mem, *d_nn, *d_nb, *d_lag, *d_over;
     unsigned int *d_cnt, *d_count;
@@ -482,13 +450,11 @@ extern "C" int ac4_frames_guu(ac4_frames *f, const int *act_tag, int gap, double
     const int n = v.n, nf = v.count;
     const double L = v.L;
     std::vector<int> fidx;
-    PTRY(frames_fidx(v, fidx));
+    PTRY(ac4_frames_steps(v, fidx));
     const double bin = 0.005;
     const double MaxBin = (int)(0.5 * L / bin);  // guu.c:17
     const int maxbin = (int)MaxBin;
     const int maxint = 200 * gap / PR_DIVIDE_TO;  // guu.c:30-31
-    const int noi = 10 + (int)(log((double)maxint) / log(factor));
-    (void)noi;
     // guu.c:67-115: lag ladder of the first frame, FrameNum nearest tau
     int frame_num = 0;
     {
@@ -517,7 +483,7 @@ extern "C" int ac4_frames_guu(ac4_frames *f, const int *act_tag, int gap, double
     for (int i = 0; i < n; i++)
         if (act_tag[i] == 1) act.push_back(i);
     const int na = (int)act.size();
-    PrBuf B;
+    AcDevBuf B;
     int *d_act;
     double *d_g, *d_acc, *d_h1, *d_h2;
     unsigned long long *d_hc;
