@@ -98,6 +98,7 @@ _SIGS = {
     "ac4_frames_find": (ctypes.c_int, [_VP, ctypes.c_longlong]),
     "ac4_frames_overlap": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_double, _I]),
     "ac4_md_fix_drift": (ctypes.c_int, [_VP, ctypes.c_int, _D]),
+    "ac4_md_format_frame": (ctypes.c_longlong, [_VP, ctypes.c_int, ctypes.c_char_p, ctypes.c_longlong]),
     "ac4_frames_block": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                         ctypes.c_double, ctypes.c_char_p]),
     "ac4_frames_overlap_cage": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
@@ -283,6 +284,18 @@ class MD:
     def set_nve(self, r, on=True):
         """Plain velocity Verlet for replica r (ac4_md_set_nve); Heff = KE + PE."""
         _check(lib.ac4_md_set_nve(self.h, r, 1 if on else 0))
+
+    def format_frame(self, r=0):
+        """data_coor.out text of replica r's current frame, formatted on the
+        device (ac4_md_format_frame), as bytes."""
+        n = self.n(r)
+        if getattr(self, "_fmt_buf", None) is None or len(self._fmt_buf) < 80 * n + 1:
+            self._fmt_buf = ctypes.create_string_buffer(80 * n + 1)
+        buf = self._fmt_buf
+        k = lib.ac4_md_format_frame(self.h, r, buf, len(buf))
+        if k < 0:
+            _check(int(k))
+        return ctypes.string_at(buf, k)
 
     def fix_drift(self, r=0):
         """Remove the mean momentum of replica r (ac4_md_fix_drift); returns it."""

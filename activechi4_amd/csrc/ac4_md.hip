@@ -1220,6 +1220,14 @@ struct ac4_md {
     OvCfg ov{0, 0, 0, 0, 0.0};
     double *ox = nullptr, *oy = nullptr, *oz = nullptr, *ov_q = nullptr, *ov_q2 = nullptr;
     long long *ov_step = nullptr;
+    // frame text on the device (ac4_md_format_frame): 80-byte line slots in
+    // reference order, line lengths, the joined text, per-block byte offsets
+    // (the total after the last block) and a flag
+    char *fmt_dev = nullptr, *fmt_txt = nullptr;
+    unsigned char *fmt_len_dev = nullptr;
+    long long *fmt_boff = nullptr;
+    int *fmt_bad = nullptr;
+    size_t fmt_cap = 0;
     unsigned int *ov_cnt = nullptr;
     int *ov_n = nullptr;
     int *et = nullptr;
@@ -1512,6 +1520,9 @@ extern "C" int ac4_md_destroy(ac4_md *md) {
                     md->cnt, md->start, md->members, md->kx, md->ky, md->kz, md->ex, md->ey, md->ez, md->et};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
+    for (void *q : {(void *)md->fmt_dev, (void *)md->fmt_txt, (void *)md->fmt_len_dev, (void *)md->fmt_boff,
+                    (void *)md->fmt_bad})
+        if (q) (void)hipFree(q);
     free_parts(md->P);
     free_parts(md->A);
     if (md->stream) (void)hipStreamDestroy(md->stream);
@@ -1754,6 +1765,213 @@ extern "C" int ac4_md_set_chain(ac4_md *md, int r, const double *c9) {
 // reference's momenta are final after a step), and the sum of p^2 the next
 // step's thermostat starts from is recomputed.  mean3 (nullable) receives
 // the mean that was removed.
+// ---- frame text on the device (MDcode/print.c:4-16 line format)
+// One number as glibc's %.5lf: v = m*2^e exactly, q = round-half-even of
+// m*10^5*2^e (the 70-bit product kept as two 64-bit words), printed as
+// q/10^5 "." q%10^5 with the sign of v.  Values with |v| >= 1e12 or not
+// finite set *bad (the host formats those frames).
+__device__ __forceinline__ char *fmt_put_u64(char *o, unsigned long long v) {
+    char t[24];
+    int k = 0;
+    do {
+        t[k++] = (char)('0' + v % 10ull);
+        v /= 10ull;
+    } while (v);
+    while (k) *o++ = t[--k];
+    return o;
+}
+__device__ char *fmt5_dev(char *o, double v, int *bad) {
+    const unsigned long long bits = (unsigned long long)__double_as_longlong(v);
+    const int neg = (int)(bits >> 63), ex = (int)((bits >> 52) & 0x7ff);
+    const unsigned long long frac = bits & ((1ull << 52) - 1);
+    if (ex == 0x7ff || fabs(v) >= 1e12) {
+        *bad = 1;
+        return o;
+    }
+    const unsigned long long m = ex ? (frac | (1ull << 52)) : frac;
+    const int e = ex ? ex - 1075 : -1074;  // v = m * 2^e, e < 0 for |v| < 1e12 unless v is 0
+    unsigned long long q = 0;
+    if (m != 0 && e < 0 && -e <= 71) {
+        const unsigned long long lo = m * 100000ull, hi = __umul64hi(m, 100000ull);  // t = hi:lo < 2^70
+        const int sh = -e;
+        // q = t >> sh; rounding bit sh-1 and the sticky bits below it
+        q = sh >= 64 ? hi >> (sh - 64) : ((lo >> sh) | (hi << (64 - sh)));
+        const int rb = sh - 1;
+        const unsigned long long rbit = rb >= 64 ? (hi >> (rb - 64)) & 1ull : (lo >> rb) & 1ull;
+        bool sticky;
+        if (rb >= 64)
+            sticky = lo != 0ull || (rb > 64 && (hi & ((1ull << (rb - 64)) - 1ull)) != 0ull);
+        else
+            sticky = rb > 0 && (lo & ((1ull << rb) - 1ull)) != 0ull;
+        if (rbit && (sticky || (q & 1ull))) q++;
+    } else if (m != 0 && e >= 0) {
+        *bad = 1;  // an integer of 2^52 or more: not below 1e12
+        return o;
+    }
+    if (neg) *o++ = '-';
+    o = fmt_put_u64(o, q / 100000ull);
+    *o++ = '.';
+    unsigned long long fp = q % 100000ull;
+    for (int d = 4; d >= 0; d--) {
+        o[d] = (char)('0' + fp % 10ull);
+        fp /= 10ull;
+    }
+    return o + 5;
+}
+
+// line of the particle in slot s of replica R, written to slot id[s] (its
+// reference index) of out (80 bytes each)
+__global__ void __launch_bounds__(BLK) k_fmt_frame(const RepC *__restrict__ reps, int r, const double *__restrict__ xu,
+                                                   const double *__restrict__ yu, const double *__restrict__ zu,
+                                                   const int *__restrict__ type, const int *__restrict__ id,
+                                                   char *out, unsigned char *len, int *bad) {
+    const RepC R = reps[r];
+    const int s = blockIdx.x * BLK + threadIdx.x;
+    if (s >= R.n) return;
+    const int g = R.off + s, i = id[g];
+    char line[80];
+    int b = 0;
+    char *o = fmt5_dev(line, xu[g], &b);
+    *o++ = '\t';
+    o = fmt5_dev(o, yu[g], &b);
+    *o++ = '\t';
+    o = fmt5_dev(o, zu[g], &b);
+    *o++ = '\t';
+    const int t = type[g];
+    if (t < 0) {
+        *o++ = '-';
+        o = fmt_put_u64(o, (unsigned long long)(-(long long)t));
+    } else {
+        o = fmt_put_u64(o, (unsigned long long)t);
+    }
+    *o++ = '\n';
+    const int k = (int)(o - line);
+    char *dst = out + (size_t)i * 80;
+    for (int c = 0; c < k; c++) dst[c] = line[c];
+    len[i] = (unsigned char)k;
+    if (b) atomicOr(bad, 1);
+}
+
+// bytes of each block of BLK lines (reference order)
+__global__ void __launch_bounds__(BLK) k_fmt_bsum(int n, const unsigned char *__restrict__ len,
+                                                  long long *__restrict__ boff) {
+    __shared__ int sum;
+    if (threadIdx.x == 0) sum = 0;
+    __syncthreads();
+    const int i = blockIdx.x * BLK + threadIdx.x;
+    if (i < n) atomicAdd(&sum, (int)len[i]);
+    __syncthreads();
+    if (threadIdx.x == 0) boff[blockIdx.x] = sum;
+}
+
+// exclusive prefix of the nb block sums in place, the total to boff[nb]; one
+// workgroup of FMT_SCAN threads, each scanning a contiguous chunk
+#define FMT_SCAN 1024
+__global__ void __launch_bounds__(FMT_SCAN) k_fmt_scan(int nb, long long *__restrict__ boff) {
+    __shared__ long long part[FMT_SCAN];
+    const int t = threadIdx.x, chunk = (nb + FMT_SCAN - 1) / FMT_SCAN;
+    const int lo = min(nb, t * chunk), hi = min(nb, lo + chunk);
+    long long acc = 0;
+    for (int b = lo; b < hi; b++) acc += boff[b];
+    part[t] = acc;
+    __syncthreads();
+    for (int d = 1; d < FMT_SCAN; d <<= 1) {  // inclusive Hillis-Steele scan
+        const long long v = t >= d ? part[t - d] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    long long run = part[t] - acc;
+    for (int b = lo; b < hi; b++) {
+        const long long v = boff[b];
+        boff[b] = run;
+        run += v;
+    }
+    if (t == FMT_SCAN - 1) boff[nb] = part[t];
+}
+
+// line i from its slot to its place in the joined text
+__global__ void __launch_bounds__(BLK) k_fmt_pack(int n, const char *__restrict__ slots,
+                                                  const unsigned char *__restrict__ len,
+                                                  const long long *__restrict__ boff, char *__restrict__ txt) {
+    __shared__ int sc[BLK];
+    const int t = threadIdx.x, i = blockIdx.x * BLK + t;
+    const int k = i < n ? (int)len[i] : 0;
+    sc[t] = k;
+    __syncthreads();
+    for (int d = 1; d < BLK; d <<= 1) {
+        const int v = t >= d ? sc[t - d] : 0;
+        __syncthreads();
+        sc[t] += v;
+        __syncthreads();
+    }
+    if (i >= n) return;
+    char *dst = txt + boff[blockIdx.x] + (sc[t] - k);
+    const char *src = slots + (size_t)i * 80;
+    for (int c = 0; c < k; c++) dst[c] = src[c];
+}
+
+// data_coor.out text of replica r's current frame (x_u, y_u, z_u, type in
+// reference order), formatted and joined on the device; byte for byte the
+// text of ac4_format_frame on the same values, which formats a frame holding
+// a value the device formatter leaves to the host.
+extern "C" long long ac4_md_format_frame(ac4_md *md, int r, char *out, long long cap) {
+    TRY(check_rep(md, r));
+    const RepC &R = md->reps_h[r];
+    const int n = R.n;
+    AC4_REQUIRE(out && cap >= 80LL * n, AC4_EARG, "format_frame: buffer needs 80 bytes a line");
+    const int nb = (n + BLK - 1) / BLK;
+    if ((size_t)n > md->fmt_cap) {
+        for (void *q : {(void *)md->fmt_dev, (void *)md->fmt_txt, (void *)md->fmt_len_dev, (void *)md->fmt_boff,
+                        (void *)md->fmt_bad})
+            if (q) (void)hipFree(q);
+        md->fmt_dev = md->fmt_txt = nullptr;
+        md->fmt_len_dev = nullptr;
+        md->fmt_boff = nullptr;
+        md->fmt_bad = nullptr;
+        md->fmt_cap = 0;
+        const size_t cn = (size_t)n;
+        AC4_HIP(hipMalloc(&md->fmt_dev, 80 * cn));
+        AC4_HIP(hipMalloc(&md->fmt_txt, 80 * cn));
+        AC4_HIP(hipMalloc(&md->fmt_len_dev, cn));
+        AC4_HIP(hipMalloc(&md->fmt_boff, sizeof(long long) * (nb + 1)));
+        AC4_HIP(hipMalloc(&md->fmt_bad, sizeof(int)));
+        md->fmt_cap = cn;
+    }
+    if (n == 0) return 0;
+    hipStream_t s = md->stream;
+    AC4_HIP(hipMemsetAsync(md->fmt_bad, 0, sizeof(int), s));
+    hipLaunchKernelGGL(k_fmt_frame, dim3(nb), dim3(BLK), 0, s, md->reps, r, md->P.xu, md->P.yu, md->P.zu, md->P.type,
+                       md->P.id, md->fmt_dev, md->fmt_len_dev, md->fmt_bad);
+    AC4_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_fmt_bsum, dim3(nb), dim3(BLK), 0, s, n, md->fmt_len_dev, md->fmt_boff);
+    AC4_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_fmt_scan, dim3(1), dim3(FMT_SCAN), 0, s, nb, md->fmt_boff);
+    AC4_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_fmt_pack, dim3(nb), dim3(BLK), 0, s, n, (const char *)md->fmt_dev,
+                       (const unsigned char *)md->fmt_len_dev, (const long long *)md->fmt_boff, md->fmt_txt);
+    AC4_HIP(hipGetLastError());
+    int bad = 0;
+    long long total = 0;
+    AC4_HIP(hipMemcpyAsync(&bad, md->fmt_bad, sizeof(int), hipMemcpyDeviceToHost, s));
+    AC4_HIP(hipMemcpyAsync(&total, md->fmt_boff + nb, sizeof(long long), hipMemcpyDeviceToHost, s));
+    AC4_HIP(hipStreamSynchronize(s));
+    if (bad) {  // a value the device formatter does not cover: the host formats the frame
+        std::vector<double> x(n), y(n), z(n);
+        std::vector<int> id, ty(n);
+        TRY(get_ids(md, r, id));
+        TRY(fetch_unpermuted(md, r, (const double *)md->P.xu, x.data(), id));
+        TRY(fetch_unpermuted(md, r, (const double *)md->P.yu, y.data(), id));
+        TRY(fetch_unpermuted(md, r, (const double *)md->P.zu, z.data(), id));
+        TRY(fetch_unpermuted(md, r, (const int *)md->P.type, ty.data(), id));
+        return ac4_format_frame(n, x.data(), y.data(), z.data(), ty.data(), out, cap);
+    }
+    AC4_REQUIRE(total > 0 && total <= 80LL * n, AC4_ESTATE, "format_frame: joined text of %lld bytes", total);
+    AC4_HIP(hipMemcpyAsync(out, md->fmt_txt, (size_t)total, hipMemcpyDeviceToHost, s));
+    AC4_HIP(hipStreamSynchronize(s));
+    return total;
+}
+
 extern "C" int ac4_md_fix_drift(ac4_md *md, int r, double *mean3) {
     TRY(check_rep(md, r));
     const RepC &R = md->reps_h[r];

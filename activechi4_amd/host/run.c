@@ -108,12 +108,12 @@ static double *g_bx, *g_by, *g_bz;
 static int *g_bt;
 
 static void print_data(ac4_md *md, state_t *S, const char *file, const char *bin_file, long step) {
-    /* MDcode/print.c:4-16, one formatted block and one write per frame */
-    CK(ac4_md_get_state(md, 0, NULL, NULL, NULL, S->xu, S->yu, S->zu, NULL, NULL, NULL, NULL, NULL, NULL));
+    /* MDcode/print.c:4-16, one formatted block and one write per frame; the
+     * lines are formatted on the device, the host only joins them */
     const long long cap = 80LL * S->n;
     if (!g_txt) g_txt = malloc((size_t)cap);
-    const long long k = ac4_format_frame(S->n, S->xu, S->yu, S->zu, S->type, g_txt, cap);
-    if (k < 0) die("ac4_format_frame", (int)k);
+    const long long k = ac4_md_format_frame(md, 0, g_txt, cap);
+    if (k < 0) die("ac4_md_format_frame", (int)k);
     FILE *f = fopen(file, "ab");
     if (!f) return;
     fwrite(g_txt, 1, (size_t)k, f);

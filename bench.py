@@ -348,11 +348,25 @@ def rows(ac4, n=50000, n_origins=20, gap=100, seed=777):
     xp, yp, zp, tp, used = ac4.parse_frame(txt, m)
     t_parse = time.perf_counter() - t0
     assert used == len(txt) and np.array_equal(xp, xs) and np.array_equal(tp, ts) and np.array_equal(zp, zs)
+    # the same frame as the unwrapped positions of an MD handle, formatted on
+    # the device (ac4_md_format_frame: what bin/run writes per frame)
+    md = ac4.MD([ac4.ReplicaParams(m, 1.2, 0.6, 1.5, -1, 0, 1)])
+    w = [rng.random(m) for _ in range(3)]
+    zero = np.zeros(m)
+    md.set_state(0, w[0], w[1], w[2], x, y, z, zero, zero, zero, t, np.full(m, -1, dtype=np.int32))
+    assert md.format_frame(0) == txt
+    md.sync()
+    t0 = time.perf_counter()
+    dev = md.format_frame(0)
+    t_dev = time.perf_counter() - t0
+    md.close()
+    assert dev == txt
     out["trajectory_io"] = {"frame_bytes": len(txt), "format_s": t_fmt, "sprintf_s": t_printf,
-                            "parse_s": t_parse, "strtod_s": t_strtod,
-                            "format_GBps": len(txt) / t_fmt / 1e9, "parse_GBps": len(txt) / t_parse / 1e9}
-    log("rows: frame text %.1f MB: format %.3f s (sprintf %.3f s), parse %.3f s (strtod %.3f s)"
-        % (len(txt) / 1e6, t_fmt, t_printf, t_parse, t_strtod))
+                            "device_format_s": t_dev, "parse_s": t_parse, "strtod_s": t_strtod,
+                            "format_GBps": len(txt) / t_fmt / 1e9, "device_format_GBps": len(txt) / t_dev / 1e9,
+                            "parse_GBps": len(txt) / t_parse / 1e9}
+    log("rows: frame text %.1f MB: format %.3f s on the host, %.4f s on the device (sprintf %.3f s), "
+        "parse %.3f s (strtod %.3f s)" % (len(txt) / 1e6, t_fmt, t_dev, t_printf, t_parse, t_strtod))
     # trajectory
     frames = trajgen.case_frames(n_origins, gap)
     X, Y, Z, _ = trajgen.make_frames(n, frames, seed, 0.06, 0.5)

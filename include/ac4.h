@@ -220,6 +220,12 @@ int ac4_frames_overlap_cage(ac4_frames *f, int gap, double factor, double a, dou
 int ac4_frames_guu(ac4_frames *f, const int *act_tag, int gap, double factor, double tau, const char *path,
                    double *hist3, int *maxbin, int *frame_num, int *n_origins);
 
+/* The data_coor.out text of replica r's current frame (print_data,
+ * MDcode/print.c:4-16: x_u y_u z_u type in reference order), formatted on the
+ * device: byte for byte the text ac4_format_frame writes for the same values.
+ * out needs 80*N bytes; returns the bytes written or a negative AC4_E* code. */
+long long ac4_md_format_frame(ac4_md *md, int r, char *out, long long cap);
+
 /* fixDrift (MDcode/initialise.c:104-138): removes the mean momentum of
  * replica r (summed in reference particle order, divided by N) from every
  * particle; mean3 (nullable) receives the mean removed.  The reference calls

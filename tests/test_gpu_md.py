@@ -406,3 +406,40 @@ def test_fix_drift_removes_mean_momentum(gpu):
     md.sync()
     assert 0.3 < md.stats(0)["T"] < 1.2
     md.close()
+
+
+def test_format_frame_on_device_matches_host(gpu):
+    """ac4_md_format_frame (print_data, MDcode/print.c:4-16, formatted on the
+    device) is byte for byte the host formatter's text of the same frame: two
+    replicas after steps (slots permuted by the cell sort), then crafted values
+    next to every rounding edge of %.5lf, and values the device leaves to the
+    host (|v| >= 1e12, inf, nan)."""
+    md = make_relaxed_md(gpu, [(1200, 91, 3), (1000, 31, 4)], tau=20, prec=0)
+    md.step(60)
+    md.sync()
+    s0 = [relaxed_state(gpu, n, sd)[0] for n, sd in ((1200, 91), (1000, 31))]
+    for r in range(2):
+        g = md.get_state(r)
+        want = gpu.format_frame(g["xu"], g["yu"], g["zu"], s0[r]["type"])
+        assert md.format_frame(r) == want
+    n = 1200
+    g = md.get_state(0)
+    rng = np.random.default_rng(5)
+    k = rng.integers(-10**9, 10**9, size=n).astype(float)
+    edge = (k + 0.5) / 1e5  # nearest doubles to the rounding boundaries
+    vals = np.concatenate([edge, np.nextafter(edge, np.inf), np.nextafter(edge, -np.inf)])
+    special = np.array([0.0, -0.0, 5e-324, -5e-324, -4e-6, -6e-6, 4.9999999e-6, 5.0000001e-6, 1e-5, 0.99999,
+                        0.999995, 9.99999999999e11, -9.99999999999e11, 2.0 ** -60, 123456.654321])
+    xu = vals[:n].copy()
+    xu[:len(special)] = special
+    yu, zu = vals[n:2 * n].copy(), vals[2 * n:3 * n].copy()
+    md.set_state(0, g["x"], g["y"], g["z"], xu, yu, zu, g["px"], g["py"], g["pz"], s0[0]["type"], s0[0]["act_tag"],
+                 g["kx"], g["ky"], g["kz"])
+    assert md.format_frame(0) == gpu.format_frame(xu, yu, zu, s0[0]["type"])
+    for bad in (1e12, -3.5e13, np.inf, np.nan):
+        xb = xu.copy()
+        xb[7] = bad
+        md.set_state(0, g["x"], g["y"], g["z"], xb, yu, zu, g["px"], g["py"], g["pz"], s0[0]["type"],
+                     s0[0]["act_tag"], g["kx"], g["ky"], g["kz"])
+        assert md.format_frame(0) == gpu.format_frame(xb, yu, zu, s0[0]["type"])
+    md.close()
