@@ -273,7 +273,13 @@ __global__ void __launch_bounds__(BLK) k_first_half(const RepC *__restrict__ rep
     if (threadIdx.x == 0) {
         double m = s_max[0];
         for (int w = 1; w < BLK / 64; w++) m = fmax(m, s_max[w]);
-        atomicMax(&chain_w[r].maxbits, (unsigned long long)__double_as_longlong(m));
+        // The maximum only grows during this kernel (it was cleared by the
+        // previous step's force epilogue), so a block whose maximum is not
+        // above the value it reads needs no atomic: same-address atomics are
+        // serialised across the device, one per block of a large replica.
+        const unsigned long long mb = (unsigned long long)__double_as_longlong(m);
+        if (mb > __hip_atomic_load(&chain_w[r].maxbits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            atomicMax(&chain_w[r].maxbits, mb);
     }
 }
 
