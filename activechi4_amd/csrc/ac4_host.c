@@ -447,6 +447,22 @@ long long ac4_format_frame(int n, const double *x, const double *y, const double
         ac4_set_error("format_frame: bad arguments (buffer needs 80 bytes a line)");
         return AC4_EARG;
     }
+    /* A line of values below 1e12 in magnitude takes at most 72 bytes.  A
+     * frame holding larger or non-finite values (a run that blew up) is
+     * measured first: %.5lf of such a value can take over 300 bytes. */
+    int wide = 0;
+    for (int i = 0; i < n && !wide; i++)
+        wide = !(fabs(x[i]) < 1e12 && fabs(y[i]) < 1e12 && fabs(z[i]) < 1e12);
+    if (wide) {
+        long long need = 0;
+        for (int i = 0; i < n; i++)
+            need += snprintf(NULL, 0, "%.5lf\t%.5lf\t%.5lf\t%d\n", x[i], y[i], z[i], type[i]);
+        if (need > cap) {
+            ac4_set_error("format_frame: the frame holds values of 1e12 or more and needs %lld bytes (buffer %lld)",
+                          need, cap);
+            return AC4_ECAP;
+        }
+    }
     char *o = out;
     for (int i = 0; i < n; i++) {
         o = fmt5(o, x[i]);

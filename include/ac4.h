@@ -222,8 +222,10 @@ int ac4_frames_guu(ac4_frames *f, const int *act_tag, int gap, double factor, do
 
 /* The data_coor.out text of replica r's current frame (print_data,
  * MDcode/print.c:4-16: x_u y_u z_u type in reference order), formatted on the
- * device: byte for byte the text ac4_format_frame writes for the same values.
- * out needs 80*N bytes; returns the bytes written or a negative AC4_E* code. */
+ * device: byte for byte the text ac4_format_frame writes for the same values
+ * (a frame holding values of 1e12 or more is formatted by it, with its
+ * AC4_ECAP).  out needs 80*N bytes; returns the bytes written or a negative
+ * AC4_E* code. */
 long long ac4_md_format_frame(ac4_md *md, int r, char *out, long long cap);
 
 /* fixDrift (MDcode/initialise.c:104-138): removes the mean momentum of
@@ -391,8 +393,10 @@ int ac4_init_lattice(int n, double T, int n_act, uint64_t seed, double *x, doubl
                      double *yu, double *zu, double *px, double *py, double *pz, int *type, int *act_tag, int *kx,
                      int *ky, int *kz);
 /* n lines of data_coor.out, "%.5lf\t%.5lf\t%.5lf\t%d\n" (MDcode/print.c:4-16),
- * byte-identical to glibc's printf; out needs 80*n bytes.  Returns the bytes
- * written or a negative error. */
+ * byte-identical to glibc's printf; out needs 80*n bytes, which holds every
+ * frame whose values are below 1e12 in magnitude.  A frame holding larger or
+ * non-finite values is measured first and gives AC4_ECAP when its text does
+ * not fit cap.  Returns the bytes written or a negative error. */
 long long ac4_format_frame(int n, const double *x, const double *y, const double *z, const int *type, char *out,
                            long long cap);
 /* Reads n such lines (Analysis/read_all_data.c:61-91); every number equals

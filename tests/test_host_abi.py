@@ -129,3 +129,20 @@ def test_frame_parse_is_strtod(ac4):
     assert np.array_equal(x, want[:, 0]) and np.array_equal(y, want[:, 1]) and np.array_equal(z, want[:, 2])
     assert np.array_equal(np.signbit(x), np.signbit(want[:, 0]))
     assert np.array_equal(tt, want[:, 3].astype(np.int32))
+
+
+def test_frame_format_of_huge_values(ac4):
+    """Values of 1e12 or more (a run that blew up) go through glibc's %.5lf;
+    a frame whose text would not fit 80 bytes a line is refused with
+    AC4_ECAP instead of overrunning the buffer."""
+    libc = ctypes.CDLL(None)
+    x = np.array([1e15, -2.5e13, np.inf, -np.inf, np.nan, 0.5])
+    y = np.array([0.25, 1e12, 3.0, -1e-7, 7.0, 123456789012.0])
+    z = np.zeros(6)
+    t = np.array([0, 1, 0, 1, 0, 1], dtype=np.int32)
+    text = ac4.format_frame(x, y, z, t)
+    want = b"".join(_glibc_line(libc, a, b, c, int(d)) for a, b, c, d in zip(x, y, z, t))
+    assert text == want
+    x[1], y[1] = 1e300, -1e300
+    with pytest.raises(ac4.AC4Error, match="1e12 or more"):
+        ac4.format_frame(x, y, z, t)
