@@ -498,7 +498,7 @@ static const char *parse_num(const char *p, double *out) {
     if (nd == 0 || nd > 15 || (*s >= '0' && *s <= '9') || *s == 'e' || *s == 'E') {
         char *e;
         *out = strtod(st, &e);
-        return e;
+        return e == st ? NULL : e;  /* NULL: no number here */
     }
     const double v = (double)d / p10[nf];
     *out = neg ? -v : v;
@@ -506,18 +506,32 @@ static const char *parse_num(const char *p, double *out) {
 }
 
 long long ac4_parse_frame(const char *text, long long len, int n, double *x, double *y, double *z, int *type) {
-    if (!text || n < 0 || !x || !y || !z || !type) { ac4_set_error("parse_frame: bad arguments"); return AC4_EARG; }
+    if (!text || len < 0 || n < 0 || !x || !y || !z || !type) {
+        ac4_set_error("parse_frame: bad arguments");
+        return AC4_EARG;
+    }
+    /* each line is parsed from a NUL-terminated copy, so no conversion reads
+     * past the line or past text + len */
+    char line[1024];
     const char *p = text, *end = text + len;
     for (int i = 0; i < n; i++) {
         if (p >= end) { ac4_set_error("parse_frame: text ends at line %d", i); return AC4_EIO; }
-        p = parse_num(p, &x[i]);
-        p = parse_num(p, &y[i]);
-        p = parse_num(p, &z[i]);
-        char *e;
-        type[i] = (int)strtol(p, &e, 10);
-        p = e;
-        while (p < end && *p != '\n') p++;
-        if (p < end) p++;
+        const char *nl = memchr(p, '\n', (size_t)(end - p));
+        const size_t m = (size_t)((nl ? nl : end) - p);
+        if (m >= sizeof line) { ac4_set_error("parse_frame: line %d is longer than 1023 bytes", i); return AC4_EIO; }
+        memcpy(line, p, m);
+        line[m] = '\0';
+        const char *q = line;
+        if (q) q = parse_num(q, &x[i]);
+        if (q) q = parse_num(q, &y[i]);
+        if (q) q = parse_num(q, &z[i]);
+        char *e = NULL;
+        if (q) {
+            type[i] = (int)strtol(q, &e, 10);
+            if (e == q) q = NULL;
+        }
+        if (!q) { ac4_set_error("parse_frame: line %d is not \"x y z type\"", i); return AC4_EIO; }
+        p = nl ? nl + 1 : end;
     }
     return (long long)(p - text);
 }

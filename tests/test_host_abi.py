@@ -146,3 +146,17 @@ def test_frame_format_of_huge_values(ac4):
     x[1], y[1] = 1e300, -1e300
     with pytest.raises(ac4.AC4Error, match="1e12 or more"):
         ac4.format_frame(x, y, z, t)
+
+
+def test_frame_parse_stops_at_the_text_end(ac4):
+    """ac4_parse_frame reads only the bytes it is given: a last line without
+    its newline is parsed up to the end, and a line that is not four numbers
+    is an error, not a read into the next line."""
+    good = b"0.50000\t-1.25000\t3.00000\t1\n0.1\t0.2\t0.3\t0"
+    x, y, z, t, k = ac4.parse_frame(good, 2)
+    assert k == len(good) and list(x) == [0.5, 0.1] and list(t) == [1, 0]
+    for bad in (b"0.5\t0.5\n1.0\t2.0\t3.0\t1\n", b"0.5\t0.5\t0.5\tx\n", b"\n"):
+        with pytest.raises(ac4.AC4Error, match="not"):
+            ac4.parse_frame(bad, 1)
+    with pytest.raises(ac4.AC4Error, match="ends at line 1"):
+        ac4.parse_frame(b"1\t2\t3\t0\n", 2)
