@@ -33,6 +33,7 @@ static int *fr_read_index(const char *sub_dir, int header_count, int *nframes) {
     }
     if (nf < 2) { fprintf(stderr, "bad frame index file %s\n", path); fclose(f); return NULL; }
     int *fidx = malloc(sizeof(int) * nf);
+    if (!fidx) { fprintf(stderr, "out of memory for %d frame indices\n", nf); fclose(f); return NULL; }
     for (int i = 0; i < nf; i++)
         if (!fgets(buff, sizeof buff, f) || sscanf(buff, "%d", &fidx[i]) != 1) {
             fprintf(stderr, "short index file %s\n", path);
@@ -66,11 +67,19 @@ static int fr_open(frame_reader *r, const char *sub_dir, int n, int bin) {
         if (fread(magic, 1, 8, r->f) != 8 || memcmp(magic, "AC4FRM01", 8) || fread(hd, sizeof hd, 1, r->f) != 1 ||
             hd[0] != n) {
             fprintf(stderr, "%s is not a frame file of N=%d\n", path, n);
+            fclose(r->f);
+            r->f = NULL;
             return -1;
         }
     } else {
         r->cap = (size_t)n * 96 + (1u << 20);
         r->buf = malloc(r->cap);
+        if (!r->buf) {
+            fprintf(stderr, "out of memory for the frame text buffer\n");
+            fclose(r->f);
+            r->f = NULL;
+            return -1;
+        }
     }
     return 0;
 }
@@ -100,7 +109,12 @@ static int fr_next(frame_reader *r, double *x, double *y, double *z, int *ty) {
             memmove(r->buf, r->buf + r->pos, r->have - r->pos);
             r->have -= r->pos;
             r->pos = 0;
-            if (r->have == r->cap) { r->cap *= 2; r->buf = realloc(r->buf, r->cap); }
+            if (r->have == r->cap) { /* a frame longer than the buffer */
+                char *nb = realloc(r->buf, 2 * r->cap);
+                if (!nb) { fprintf(stderr, "out of memory in frame %d\n", r->frame); return -1; }
+                r->buf = nb;
+                r->cap *= 2;
+            }
             const size_t got = fread(r->buf + r->have, 1, r->cap - r->have, r->f);
             if (got == 0) { fprintf(stderr, "frame data ends in frame %d\n", r->frame); return -1; }
             r->have += got;
