@@ -22,6 +22,7 @@
 #include <string.h>
 #include <stdint.h>
 #include "ac4.h"
+#include "xalloc.h"
 #include "frame_reader.h"
 
 static int env_on(const char *k) { return getenv(k) && atoi(getenv(k)); }
@@ -53,8 +54,8 @@ int main(int argc, char **argv) {
     if (getenv("AC4_FS") && atoi(getenv("AC4_FS"))) obs |= AC4_OBS_FS;
     ac4_ana *an = NULL;
     CK(ac4_ana_create(n, L, nframes, fidx, 0, 1.4, 0.3, obs, 0, &an));
-    double *x = malloc(sizeof(double) * n), *y = malloc(sizeof(double) * n), *z = malloc(sizeof(double) * n);
-    int *ty = malloc(sizeof(int) * n);
+    double *x = xmalloc(sizeof(double) * n), *y = xmalloc(sizeof(double) * n), *z = xmalloc(sizeof(double) * n);
+    int *ty = xmalloc(sizeof(int) * n);
     const int guu = env_on("AC4_GUU"), cage = env_on("AC4_CAGE");
     ac4_frames *fs = NULL;
     if (guu || cage) CK(ac4_frames_create(n, L, nframes, &fs));
@@ -79,7 +80,7 @@ int main(int argc, char **argv) {
             CK(ac4_frames_overlap_cage(fs, gap, 1.4, 0.3, 2.5 + 0.35, path, NULL, 0, NULL));
         }
         if (guu) {
-            int *act = calloc(n, sizeof(int)), k; /* read_all_data.c:49-58 */
+            int *act = xcalloc(n, sizeof(int)), k; /* read_all_data.c:49-58 */
             snprintf(path, sizeof path, "%s/ActiveFileIndex.dat", sub_dir);
             FILE *fa = fopen(path, "rb");
             if (!fa) { fprintf(stderr, "analyse: no %s\n", path); return 1; }

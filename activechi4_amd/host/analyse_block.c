@@ -19,6 +19,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include "ac4.h"
+#include "xalloc.h"
 #include "frame_reader.h"
 
 static void die(const char *what, int rc) {
@@ -35,8 +36,8 @@ static int one_job(const char *sub_dir, int n, double L, int bin) {
     if (gap <= 0) die("ac4_origin_gap", gap);
     ac4_frames *fs = NULL;
     CK(ac4_frames_create(n, L, nframes, &fs));
-    double *x = malloc(sizeof(double) * n), *y = malloc(sizeof(double) * n), *z = malloc(sizeof(double) * n);
-    int *ty = malloc(sizeof(int) * n);
+    double *x = xmalloc(sizeof(double) * n), *y = xmalloc(sizeof(double) * n), *z = xmalloc(sizeof(double) * n);
+    int *ty = xmalloc(sizeof(int) * n);
     frame_reader rd;
     if (fr_open(&rd, sub_dir, n, bin)) return 1;
     for (int fr = 0; fr < nframes; fr++) {

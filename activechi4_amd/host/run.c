@@ -29,6 +29,7 @@
 #include <sys/stat.h>
 #include <time.h>
 #include "ac4.h"
+#include "xalloc.h"
 
 #define TIME_STEP 0.005 /* MDcode/global.h:33 */
 #define TAU_PER 200     /* MDcode/global.h:58 */
@@ -111,7 +112,7 @@ static void print_data(ac4_md *md, state_t *S, const char *file, const char *bin
     /* MDcode/print.c:4-16, one formatted block and one write per frame; the
      * lines are formatted on the device, the host only joins them */
     const long long cap = 80LL * S->n;
-    if (!g_txt) g_txt = malloc((size_t)cap);
+    if (!g_txt) g_txt = xmalloc((size_t)cap);
     const long long k = ac4_md_format_frame(md, 0, g_txt, cap);
     if (k < 0) die("ac4_md_format_frame", (int)k);
     FILE *f = fopen(file, "ab");
@@ -120,8 +121,8 @@ static void print_data(ac4_md *md, state_t *S, const char *file, const char *bin
     fclose(f);
     if (!bin_file) return;
     if (!g_bx) {
-        g_bx = malloc(sizeof(double) * S->n); g_by = malloc(sizeof(double) * S->n);
-        g_bz = malloc(sizeof(double) * S->n); g_bt = malloc(sizeof(int) * S->n);
+        g_bx = xmalloc(sizeof(double) * S->n); g_by = xmalloc(sizeof(double) * S->n);
+        g_bz = xmalloc(sizeof(double) * S->n); g_bt = xmalloc(sizeof(int) * S->n);
     }
     if (ac4_parse_frame(g_txt, k, S->n, g_bx, g_by, g_bz, g_bt) != k) die("ac4_parse_frame", AC4_EIO);
     f = fopen(bin_file, "ab");
@@ -145,7 +146,7 @@ static void print_data(ac4_md *md, state_t *S, const char *file, const char *bin
 static void write_exact(ac4_md *md, const char *file, long next, int index) {
     long long len = ac4_md_checkpoint_size(md, 0);
     if (len < 0) die("ac4_md_checkpoint_size", (int)len);
-    char *buf = malloc((size_t)len);
+    char *buf = xmalloc((size_t)len);
     CK(ac4_md_checkpoint(md, 0, buf, len));
     FILE *f = fopen(file, "wb");
     if (!f) { free(buf); return; }
@@ -167,7 +168,13 @@ static int read_exact(ac4_md *md, const char *file, long *next, int *index) {
         fclose(f);
         return -1;
     }
-    char *buf = malloc((size_t)hd[2]);
+    if (hd[2] != ac4_md_checkpoint_size(md, 0)) {  /* written for another N or build */
+        fprintf(stderr, "%s: checkpoint of %lld bytes, this run's is %lld\n", file, (long long)hd[2],
+                ac4_md_checkpoint_size(md, 0));
+        fclose(f);
+        return -1;
+    }
+    char *buf = xmalloc((size_t)hd[2]);
     const int ok = fread(buf, 1, (size_t)hd[2], f) == (size_t)hd[2];
     fclose(f);
     if (!ok) { free(buf); return -1; }
@@ -271,12 +278,12 @@ int main(int argc, char **argv) {
     S.n = n;
     S.nact = (int)((double)n * 0.1000);
     double **d[] = {&S.x, &S.y, &S.z, &S.xu, &S.yu, &S.zu, &S.px, &S.py, &S.pz};
-    for (int k = 0; k < 9; k++) *d[k] = malloc(sizeof(double) * n);
-    S.type = malloc(sizeof(int) * n);
-    S.act = malloc(sizeof(int) * n);
-    S.kx = malloc(sizeof(int) * (S.nact + 1));
-    S.ky = malloc(sizeof(int) * (S.nact + 1));
-    S.kz = malloc(sizeof(int) * (S.nact + 1));
+    for (int k = 0; k < 9; k++) *d[k] = xmalloc(sizeof(double) * n);
+    S.type = xmalloc(sizeof(int) * n);
+    S.act = xmalloc(sizeof(int) * n);
+    S.kx = xmalloc(sizeof(int) * (S.nact + 1));
+    S.ky = xmalloc(sizeof(int) * (S.nact + 1));
+    S.kz = xmalloc(sizeof(int) * (S.nact + 1));
 
     double L = pow((double)n / density, 0.33333333); /* MDcode/main.c:465 */
     uint64_t gs = seed ^ 0x9e3779b97f4a7c15ull;
@@ -298,13 +305,13 @@ int main(int argc, char **argv) {
         fclose(f);
     } else {
         if (read_restart(&S, restart_file, &L)) { fprintf(stderr, "run: cannot read %s\n", restart_file); return 1; }
-        int *idx = malloc(sizeof(int) * S.nact);
+        int *idx = xmalloc(sizeof(int) * S.nact);
         snprintf(path, sizeof path, "%s/ActiveFileIndex.dat", sub_dir);
         if (read_ints(path, idx, S.nact, 1)) { fprintf(stderr, "run: cannot read %s\n", path); return 1; }
         for (int i = 0; i < n; i++) S.act[i] = -1;
         for (int a = 0; a < S.nact; a++) S.act[idx[a]] = a;
         free(idx);
-        int *k3 = malloc(sizeof(int) * 3 * S.nact);
+        int *k3 = xmalloc(sizeof(int) * 3 * S.nact);
         snprintf(path, sizeof path, "%s/%s", sub_dir, restart_run ? "ActiveDirectionRun.dat" : "ActiveDirection.dat");
         if (read_ints(path, k3, S.nact, 3)) { fprintf(stderr, "run: cannot read %s\n", path); return 1; }
         for (int a = 0; a < S.nact; a++) { S.kx[a] = k3[3 * a]; S.ky[a] = k3[3 * a + 1]; S.kz[a] = k3[3 * a + 2]; }
@@ -358,7 +365,7 @@ int main(int argc, char **argv) {
     /* frame schedule, MDcode/main.c:479-485 */
     unsigned long long runLength = (unsigned long long)(prodDuration / TIME_STEP);
     int cap = 200 * (12 + (int)(log((double)runLength / 2.0 + 2.0) / log(1.4))) + 64;
-    uint64_t *next_idx = malloc(sizeof(uint64_t) * cap);
+    uint64_t *next_idx = xmalloc(sizeof(uint64_t) * cap);
     int nfr = ac4_schedule(runLength, 200, 1.4, next_idx, cap);
     if (nfr < 0) die("ac4_schedule", nfr);
     snprintf(path, sizeof path, "%s/time_index_file.dat", sub_dir);
