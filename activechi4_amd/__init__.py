@@ -186,6 +186,13 @@ def _i32(a):
     return np.ascontiguousarray(a, dtype=np.int32)
 
 
+def _len(n, **arrays):
+    """Every array must hold n entries: the C side reads n of each."""
+    for name, a in arrays.items():
+        if a is not None and len(a) != n:
+            raise ValueError("%s holds %d entries, %d expected" % (name, len(a), n))
+
+
 def require_gpu():
     load()
     if lib.ac4_device_count() < 1:
@@ -224,6 +231,10 @@ class MD:
         t = _i32(typ)
         at = None if act_tag is None else _i32(act_tag)
         ks = [None if k is None else _i32(k) for k in (kx, ky, kz)]
+        _len(self.n(r), x=a[0], y=a[1], z=a[2], xu=a[3], yu=a[4], zu=a[5], px=a[6], py=a[7], pz=a[8], type=t,
+             act_tag=at)
+        if all(k is not None for k in ks) and self.nact(r) > 0:
+            _len(self.nact(r), kx=ks[0], ky=ks[1], kz=ks[2])
         _check(lib.ac4_md_set_state(self.h, r, *[_dp(v) for v in a], _ip(t), _ip(at), *[_ip(k) for k in ks]))
 
     def get_state(self, r=0):
@@ -393,6 +404,7 @@ class Analysis:
 
     def push(self, pos, x, y, z, t):
         x, y, z, t = _f64(x), _f64(y), _f64(z), _i32(t)
+        _len(self.n, x=x, y=y, z=z, type=t)
         _check(lib.ac4_ana_push_frame(self.h, pos, _dp(x), _dp(y), _dp(z), _ip(t)))
 
     def push_md(self, pos, md, r=0):
@@ -533,6 +545,7 @@ class Frames:
 
     def push(self, xu, yu, zu, step):
         a = [_f64(v) for v in (xu, yu, zu)]
+        _len(self.n, xu=a[0], yu=a[1], zu=a[2])
         _check(lib.ac4_frames_push(self.h, *[_dp(v) for v in a], int(step)))
 
     def push_md(self, md, r, step):
@@ -576,6 +589,7 @@ class Frames:
     def guu(self, act_tag, gap, tau, path=None, factor=1.4):
         """g_uu(r) (ac4_frames_guu); returns (hist[3][maxbin], frame_num, n_origins)."""
         at = _i32(act_tag)
+        _len(self.n, act_tag=at)
         mb = int(0.5 * self.L / 0.005)
         h = np.zeros(3 * max(mb, 1))
         out = [ctypes.c_int() for _ in range(3)]
@@ -618,6 +632,7 @@ def format_frame(x, y, z, typ):
     """data_coor.out text of one frame (ac4_format_frame), as bytes."""
     load()
     x, y, z, t = _f64(x), _f64(y), _f64(z), _i32(typ)
+    _len(len(x), y=y, z=z, type=t)
     buf = ctypes.create_string_buffer(80 * len(x) + 1)
     k = lib.ac4_format_frame(len(x), _dp(x), _dp(y), _dp(z), _ip(t), buf, 80 * len(x) + 1)
     if k < 0:

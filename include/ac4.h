@@ -214,9 +214,11 @@ int ac4_frames_overlap_cage(ac4_frames *f, int gap, double factor, double a, dou
  * (act_tag[i] == 1, ActiveFileIndex.dat) at the lag nearest tau on the first
  * frame's ladder: replaces Analysis/guu.c:3-300 (the guuTauAlphaAct.dat
  * branch; its call at Analysis/main.c:64 is commented out).  Bins of 0.005
- * up to L/2.  Writes path (nullable) in the reference's format; hist3
- * (nullable, 3*maxbin) receives hist, hist1, hist2; maxbin, frame_num (the
- * lag in steps) and n_origins (origins used) are returned when not NULL. */
+ * up to L/2: maxbin = (int)(0.5*L/0.005) with L the store's box edge
+ * (guu.c:17).  Writes path (nullable) in the reference's format; hist3
+ * (nullable, 3*maxbin doubles) receives hist, hist1, hist2; maxbin,
+ * frame_num (the lag in steps) and n_origins (origins used) are returned
+ * when not NULL. */
 int ac4_frames_guu(ac4_frames *f, const int *act_tag, int gap, double factor, double tau, const char *path,
                    double *hist3, int *maxbin, int *frame_num, int *n_origins);
 
