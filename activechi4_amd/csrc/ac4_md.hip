@@ -653,10 +653,20 @@ __global__ void __launch_bounds__(BLK) k_nlist(const RepC *__restrict__ reps, Ch
 // control flow is uniform within a team (one home particle per team), so
 // every lane of a team is active at each ballot.  FP32 screen with the margin
 // described at k_nlist.
+//
+// The stencil's slot ranges are cut into chunks of NLT candidates, listed by
+// the team in its LDS row (ct, CH_CAP entries), and the team takes CH_PER
+// chunks per round, with CH_PER loads in flight per lane across range
+// boundaries: about half the dependent rounds of a range-by-range walk, the
+// same candidates in the same order.  A stencil of more than CH_CAP chunks
+// (crowded cells, narrow teams) is walked range by range.
+#define CH_CAP 64
+#define CH_PER 4
 template <int NLT>
 __device__ __forceinline__ void nlist_team_one(const RepC &R, int r, int i, ChainS *chain,
                                                const uint4 *__restrict__ posq, const int *__restrict__ cell_of,
-                                               const int2 *__restrict__ ranges, int *nn, int *nl, int cap) {
+                                               const int2 *__restrict__ ranges, int *nn, int *nl, int cap,
+                                               int2 *ct) {
     const int lane = threadIdx.x & 63, t = lane & (NLT - 1);
     const unsigned long long tmask = (NLT == 64 ? ~0ull : ((1ull << (NLT & 63)) - 1ull)) << (lane & ~(NLT - 1));
     const unsigned long long below = (1ull << lane) - 1ull;
@@ -676,18 +686,51 @@ __device__ __forceinline__ void nlist_team_one(const RepC &R, int r, int i, Chai
         }
         count += __popcll(m);
     };
-    // the home cell's stencil as slot ranges (k_cell_sort), in stencil order
-    for (int q = 0; q < NRANGE; q++) {
-        const int2 rg = rgs[q];
-        const int s2 = rg.x, e2 = rg.y;
-        // two candidates per lane, both loaded before either is used
-        // (indices clamped into [s2, e2), validity applied after)
-        for (int k0 = s2; k0 < e2; k0 += 2 * NLT) {
-            const int ka = k0 + t, kb = k0 + NLT + t;
-            const uint4 qa = posq[min(ka, e2 - 1)], qb = posq[min(kb, e2 - 1)];
-            const bool ha = inlist(qa) && ka < e2 && ka != g, hb = inlist(qb) && kb < e2 && kb != g;
-            emit(ha, ka);
-            emit(hb, kb);
+    // the home cell's stencil as slot ranges (k_cell_sort), in stencil order:
+    // lane t holds ranges t and t + NLT (when NLT < NRANGE), and the team
+    // lists their chunks in range order
+    int2 ra = make_int2(0, 0), rb = make_int2(0, 0);
+    if (t < NRANGE) ra = rgs[t];
+    if (NLT < NRANGE && t + NLT < NRANGE) rb = rgs[t + NLT];
+    const int na = (ra.y - ra.x + NLT - 1) / NLT, nb = (rb.y - rb.x + NLT - 1) / NLT;
+    int pa = na, pb = nb;  // inclusive scans over the team
+    for (int o = 1; o < NLT; o <<= 1) {
+        const int ua = __shfl_up(pa, o, NLT), ub = __shfl_up(pb, o, NLT);
+        if (t >= o) { pa += ua; pb += ub; }
+    }
+    const int tot_a = __shfl(pa, NLT - 1, NLT), nch = tot_a + __shfl(pb, NLT - 1, NLT);
+    if (nch <= CH_CAP) {
+        for (int j = 0; j < na; j++) ct[pa - na + j] = make_int2(ra.x + j * NLT, min(NLT, ra.y - ra.x - j * NLT));
+        for (int j = 0; j < nb; j++)
+            ct[tot_a + pb - nb + j] = make_int2(rb.x + j * NLT, min(NLT, rb.y - rb.x - j * NLT));
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // the row is read by the other lanes
+        for (int c0 = 0; c0 < nch; c0 += CH_PER) {
+            int2 ch[CH_PER];
+            uint4 qv[CH_PER];
+#pragma unroll
+            for (int u = 0; u < CH_PER; u++) ch[u] = c0 + u < nch ? ct[c0 + u] : make_int2(0, 0);
+#pragma unroll
+            for (int u = 0; u < CH_PER; u++) qv[u] = posq[t < ch[u].y ? ch[u].x + t : g];
+#pragma unroll
+            for (int u = 0; u < CH_PER; u++) {
+                const int k = ch[u].x + t;
+                emit(t < ch[u].y && k != g && inlist(qv[u]), k);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // reads done before the row is rewritten
+    } else {
+        for (int q = 0; q < NRANGE; q++) {
+            const int2 rg = rgs[q];
+            const int s2 = rg.x, e2 = rg.y;
+            // two candidates per lane, both loaded before either is used
+            // (indices clamped into [s2, e2), validity applied after)
+            for (int k0 = s2; k0 < e2; k0 += 2 * NLT) {
+                const int ka = k0 + t, kb = k0 + NLT + t;
+                const uint4 qa = posq[min(ka, e2 - 1)], qb = posq[min(kb, e2 - 1)];
+                const bool ha = inlist(qa) && ka < e2 && ka != g, hb = inlist(qb) && kb < e2 && kb != g;
+                emit(ha, ka);
+                emit(hb, kb);
+            }
         }
     }
     if (t == 0) {
@@ -706,12 +749,14 @@ __global__ void __launch_bounds__(NLT * LPB) k_nlist_team(const RepC *__restrict
                                                           const uint4 *__restrict__ posq,
                                                           const int *__restrict__ cell_of,
                                                           const int2 *__restrict__ ranges, int *nn, int *nl, int cap) {
+    __shared__ int2 s_ct[LPB][CH_CAP];  // each team's chunk list
     const int nit = wl.n[1];
     for (int it = blockIdx.x; it < nit; it += gridDim.x) {
         const int b = wl.item[wl.off_l + it], r = lblk_rep[b];
         const RepC R = reps[r];
         const int i = lblk_first[b] + (int)threadIdx.x / NLT;
-        if (i < R.n) nlist_team_one<NLT>(R, r, i, chain, posq, cell_of, ranges, nn, nl, cap);  // whole team
+        if (i < R.n)  // whole team
+            nlist_team_one<NLT>(R, r, i, chain, posq, cell_of, ranges, nn, nl, cap, s_ct[threadIdx.x / NLT]);
     }
 }
 
