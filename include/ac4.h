@@ -26,10 +26,11 @@ extern "C" {
 #define AC4_EIO -5       /* file I/O or parse error */
 
 #define AC4_FP64 0       /* plain mode: every force in FP64 */
-#define AC4_FP32 1       /* pair forces in FP32, compensated per-particle sums */
+#define AC4_FP32 1       /* pair terms in FP32, summed per particle in FP64 */
 
 #define AC4_NQ 162       /* lattice q vectors of Analysis/S4Qt.c:297-356 */
 
+/* Text of the last error reported by any entry point of this thread. */
 const char *ac4_last_error(void);
 /* Number of visible HIP devices (0 when none): lets callers fail loudly. */
 int ac4_device_count(void);
@@ -103,6 +104,10 @@ int ac4_md_step(ac4_md *md, int nsteps);
 /* Same as ac4_md_step but records the per-step launch sequence once in a
  * hipGraph and replays it (batched serving path). */
 int ac4_md_step_graph(ac4_md *md, int nsteps);
+/* Neighbour-list rebuild test of every replica (MDcode/time_evolve.c:508):
+ * 0 (default) rebuilds when 4*max|x_u - x_old|^2*L^2 > skin^2 (no two
+ * particles can close the skin before it fires); 1 is the reference's
+ * 3*max|x_u - x_old|^2*L^2 > skin^2 (skin 0.35 there). */
 int ac4_md_set_rebuild_criterion(ac4_md *md, int criterion);
 /* coldstart()/breakxal() (MDcode/initialise.c:166-201): nsteps of velocity
  * Verlet with the momenta rescaled every step to a target temperature ramped
