@@ -428,14 +428,31 @@ __device__ __forceinline__ void xruns(int cx, int nc, int *f0, int *l0, int *f1,
     }
 }
 
+struct PartArrays {  // particle arrays that move with a particle
+    double *x, *y, *z, *xu, *yu, *zu, *px, *py, *pz;
+    int *type, *act, *id, *cell_ref, *cell_of;
+};
+
+// new slot gn <- old slot go of every array that moves with a particle
+__device__ __forceinline__ void gather_one(const PartArrays &src, const PartArrays &dst, int gn, int go) {
+    dst.x[gn] = src.x[go]; dst.y[gn] = src.y[go]; dst.z[gn] = src.z[go];
+    dst.xu[gn] = src.xu[go]; dst.yu[gn] = src.yu[go]; dst.zu[gn] = src.zu[go];
+    dst.px[gn] = src.px[go]; dst.py[gn] = src.py[go]; dst.pz[gn] = src.pz[go];
+    dst.type[gn] = src.type[go]; dst.act[gn] = src.act[go]; dst.id[gn] = src.id[go];
+    dst.cell_ref[gn] = src.cell_ref[go]; dst.cell_of[gn] = src.cell_of[go];
+}
+
 // Members of a cell in increasing original index (the binning order of
 // cell_list+forces.c:71-80): a canonical order, so lists and force sums do not
 // depend on launch timing or on which replicas share the launch.  One wave per
-// cell: keys staged in LDS, each lane ranks its entries by counting.
+// cell: keys staged in LDS, each lane ranks its entries by counting and moves
+// each of its particles to its new slot (start of the cell + rank) in dst,
+// so consecutive ranks write consecutive slots.
 #define SORT_CAP 256
 __global__ void __launch_bounds__(BLK) k_cell_sort(const RepC *__restrict__ reps, WorkList wl,
                                                    const int *__restrict__ cnt, const int *__restrict__ start,
-                                                   const int *__restrict__ id, int *members, int2 *ranges) {
+                                                   const int *__restrict__ id, int *members, int2 *ranges,
+                                                   PartArrays src, PartArrays dst) {
     __shared__ int s_m[BLK / 64][SORT_CAP];
     __shared__ int s_k[BLK / 64][SORT_CAP];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -462,7 +479,8 @@ __global__ void __launch_bounds__(BLK) k_cell_sort(const RepC *__restrict__ reps
             }
             ranges[(size_t)(R.off_cell + c) * NRANGE + lane] = rg;
         }
-        int *m = members + R.off + start[R.off_cell + c];
+        const int first = start[R.off_cell + c];
+        int *m = members + R.off + first;
         const int n = cnt[R.off_cell + c];
         if (n <= SORT_CAP) {
             for (int k = lane; k < n; k += 64) { s_m[w][k] = m[k]; s_k[w][k] = id[R.off + m[k]]; }
@@ -472,41 +490,24 @@ __global__ void __launch_bounds__(BLK) k_cell_sort(const RepC *__restrict__ reps
                 int rank = 0;
                 for (int q = 0; q < n; q++) rank += s_k[w][q] < v;
                 m[rank] = s_m[w][k];
+                gather_one(src, dst, R.off + first + rank, R.off + s_m[w][k]);
             }
             __syncwarp();
-        } else if (lane == 0) {  // crowded cell: serial insertion sort on the key
-            for (int a = 1; a < n; a++) {
-                int v = m[a], kv = id[R.off + v], k = a - 1;
-                while (k >= 0 && id[R.off + m[k]] > kv) { m[k + 1] = m[k]; k--; }
-                m[k + 1] = v;
+        } else {
+            if (lane == 0) {  // crowded cell: serial insertion sort on the key
+                for (int a = 1; a < n; a++) {
+                    int v = m[a], kv = id[R.off + v], k = a - 1;
+                    while (k >= 0 && id[R.off + m[k]] > kv) { m[k + 1] = m[k]; k--; }
+                    m[k + 1] = v;
+                }
             }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");  // lane 0's order, read by every lane
+            __syncwarp();
+            for (int k = lane; k < n; k += 64) gather_one(src, dst, R.off + first + k, R.off + m[k]);
         }
     }
 }
 
-struct PartArrays {  // particle arrays that move with a particle
-    double *x, *y, *z, *xu, *yu, *zu, *px, *py, *pz;
-    int *type, *act, *id, *cell_ref, *cell_of;
-};
-
-// new slot s <- old slot members[s]
-__global__ void __launch_bounds__(BLK) k_gather(const RepC *__restrict__ reps, const int *__restrict__ blk_rep,
-                                                const int *__restrict__ blk_first, WorkList wl,
-                                                const int *__restrict__ members, PartArrays src, PartArrays dst) {
-    const int nit = wl.n[0];
-    for (int it = blockIdx.x; it < nit; it += gridDim.x) {
-        const int b = wl.item[it], r = blk_rep[b];
-        const RepC R = reps[r];
-        const int i = blk_first[b] + threadIdx.x;
-        if (i >= R.n) continue;
-        const int g = R.off + i, o = R.off + members[g];
-        dst.x[g] = src.x[o]; dst.y[g] = src.y[o]; dst.z[g] = src.z[o];
-        dst.xu[g] = src.xu[o]; dst.yu[g] = src.yu[o]; dst.zu[g] = src.zu[o];
-        dst.px[g] = src.px[o]; dst.py[g] = src.py[o]; dst.pz[g] = src.pz[o];
-        dst.type[g] = src.type[o]; dst.act[g] = src.act[o]; dst.id[g] = src.id[o];
-        dst.cell_ref[g] = src.cell_ref[o]; dst.cell_of[g] = src.cell_of[o];
-    }
-}
 
 __global__ void __launch_bounds__(BLK) k_commit(const RepC *__restrict__ reps, const int *__restrict__ blk_rep,
                                                 const int *__restrict__ blk_first, WorkList wl,
@@ -2098,9 +2099,7 @@ static int launch_rebuild(ac4_md *md, bool with_zero = false) {
     hipLaunchKernelGGL(k_scatter, dim3(gb), dim3(BLK), 0, s, md->reps, md->blk_rep, md->blk_first, wl,
                        md->P.cell_of, md->rank, md->start, md->members);
     hipLaunchKernelGGL(k_cell_sort, dim3(gc), dim3(BLK), 0, s, md->reps, wl, md->cnt, md->start, md->P.id,
-                       md->members, md->ranges);
-    hipLaunchKernelGGL(k_gather, dim3(gb), dim3(BLK), 0, s, md->reps, md->blk_rep, md->blk_first, wl, md->members,
-                       md->P, md->A);
+                       md->members, md->ranges, md->P, md->A);
     hipLaunchKernelGGL(k_commit, dim3(gb), dim3(BLK), 0, s, md->reps, md->blk_rep, md->blk_first, wl, md->A, md->P,
                        md->pos4, md->posq, md->xo, md->yo, md->zo);
     if (with_zero)  // update_cell_list: the reference's FP64 test and its 2.85 count
