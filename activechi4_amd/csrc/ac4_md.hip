@@ -300,14 +300,18 @@ __global__ void __launch_bounds__(BLK) k_cell_zero(const RepC *__restrict__ reps
 __device__ __forceinline__ int clampc(int c, int n) { return min(max(c, 0), n - 1); }
 
 // Binning: the reference cell id (bit-exact, for update_cell_list) and the
-// list-grid cell of every particle, with the slot within its cell taken by
+// list-grid cell of every particle, with the place within its cell taken by
 // one atomic per distinct cell in the wave (particles arrive in cell order,
-// so a wave holds a few runs of equal cells).  The last block of a replica to
-// finish turns the cell counts into start offsets (exclusive scan).
+// so a wave holds a few runs of equal cells); the particle is written to
+// that place of its cell's row of CELL_CAP entries (the reference's
+// InCell[cell][MaxPartInCell], MDcode/global.h:20,85; a fuller cell sets the
+// capacity error).  The last block of a replica to finish turns the cell
+// counts into start offsets (exclusive scan).
+#define CELL_CAP 600
 __global__ void __launch_bounds__(BLK) k_bin(const RepC *__restrict__ reps, ChainS *chain,
                                              const int *__restrict__ blk_rep, const int *__restrict__ blk_first,
                                              const double *__restrict__ x, const double *__restrict__ y,
-                                             const double *__restrict__ z, int *cnt, int *cell_of, int *rank,
+                                             const double *__restrict__ z, int *cnt, int *cell_of, int *cell_mem,
                                              int *cell_ref, int *start, WorkList wl) {
     __shared__ int s_last, s_w[BLK / 64], s_base[3];
     const int b = blockIdx.x, r = blk_rep[b];
@@ -342,7 +346,10 @@ __global__ void __launch_bounds__(BLK) k_bin(const RepC *__restrict__ reps, Chai
         if (c == cl) my_rank = base + __popcll(grp & below);
         todo &= ~grp;
     }
-    if (valid) rank[g] = my_rank;
+    if (valid) {
+        if (my_rank < CELL_CAP) cell_mem[(size_t)(R.off_cell + c) * CELL_CAP + my_rank] = i;
+        else atomicOr(&chain[r].err, 1);
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
         __threadfence();
@@ -384,22 +391,6 @@ __global__ void __launch_bounds__(BLK) k_bin(const RepC *__restrict__ reps, Chai
     for (int q = threadIdx.x; q < nb; q += BLK) wl.item[s_base[0] + q] = R.blk_off + q;
     for (int q = threadIdx.x; q < nl; q += BLK) wl.item[wl.off_l + s_base[1] + q] = R.lblk_off + q;
     for (int q = threadIdx.x; q < ncb; q += BLK) wl.item[wl.off_c + s_base[2] + q] = (r << 16) | q;
-}
-
-__global__ void __launch_bounds__(BLK) k_scatter(const RepC *__restrict__ reps, const int *__restrict__ blk_rep,
-                                                 const int *__restrict__ blk_first, WorkList wl,
-                                                 const int *__restrict__ cell_of, const int *__restrict__ rank,
-                                                 const int *__restrict__ start, int *members) {
-    const int nit = wl.n[0];
-    for (int it = blockIdx.x; it < nit; it += gridDim.x) {
-        const int b = wl.item[it], r = blk_rep[b];
-        const RepC R = reps[r];
-        const int i = blk_first[b] + threadIdx.x;
-        if (i < R.n) {
-            const int g = R.off + i;
-            members[R.off + start[R.off_cell + cell_of[g]] + rank[g]] = i;
-        }
-    }
 }
 
 // Distinct neighbour-cell coordinates along one axis (cells are at least one
@@ -451,8 +442,8 @@ __device__ __forceinline__ void gather_one(const PartArrays &src, const PartArra
 #define SORT_CAP 256
 __global__ void __launch_bounds__(BLK) k_cell_sort(const RepC *__restrict__ reps, WorkList wl,
                                                    const int *__restrict__ cnt, const int *__restrict__ start,
-                                                   const int *__restrict__ id, int *members, int2 *ranges,
-                                                   PartArrays src, PartArrays dst) {
+                                                   const int *__restrict__ id, const int *__restrict__ cell_mem,
+                                                   int *members, int2 *ranges, PartArrays src, PartArrays dst) {
     __shared__ int s_m[BLK / 64][SORT_CAP];
     __shared__ int s_k[BLK / 64][SORT_CAP];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -481,9 +472,12 @@ __global__ void __launch_bounds__(BLK) k_cell_sort(const RepC *__restrict__ reps
         }
         const int first = start[R.off_cell + c];
         int *m = members + R.off + first;
-        const int n = cnt[R.off_cell + c];
+        // the cell's members in binning order (a cell beyond CELL_CAP has set
+        // the capacity error; only its first CELL_CAP are moved)
+        const int *row = cell_mem + (size_t)(R.off_cell + c) * CELL_CAP;
+        const int n = min(cnt[R.off_cell + c], CELL_CAP);
         if (n <= SORT_CAP) {
-            for (int k = lane; k < n; k += 64) { s_m[w][k] = m[k]; s_k[w][k] = id[R.off + m[k]]; }
+            for (int k = lane; k < n; k += 64) { const int v = row[k]; s_m[w][k] = v; s_k[w][k] = id[R.off + v]; }
             __syncwarp();
             for (int k = lane; k < n; k += 64) {
                 const int v = s_k[w][k];
@@ -494,6 +488,9 @@ __global__ void __launch_bounds__(BLK) k_cell_sort(const RepC *__restrict__ reps
             }
             __syncwarp();
         } else {
+            for (int k = lane; k < n; k += 64) m[k] = row[k];
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+            __syncwarp();
             if (lane == 0) {  // crowded cell: serial insertion sort on the key
                 for (int a = 1; a < n; a++) {
                     int v = m[a], kv = id[R.off + v], k = a - 1;
@@ -1264,7 +1261,7 @@ struct ac4_md {
     double *fx = nullptr, *fy = nullptr, *fz = nullptr, *xo = nullptr, *yo = nullptr, *zo = nullptr;
     double4 *pos4 = nullptr;
     uint4 *posq = nullptr;
-    int *rank = nullptr, *nn = nullptr, *nn_ref = nullptr, *nl = nullptr, *cnt = nullptr, *start = nullptr,
+    int *cell_mem = nullptr, *nn = nullptr, *nn_ref = nullptr, *nl = nullptr, *cnt = nullptr, *start = nullptr,
         *members = nullptr;
     int8_t *kx = nullptr, *ky = nullptr, *kz = nullptr;
     double *ex = nullptr, *ey = nullptr, *ez = nullptr;  // reference-order export
@@ -1479,7 +1476,7 @@ extern "C" int ac4_md_create_slots(int nslots, const int *cap_n, int precision, 
     ALLOC(fblk_rep, md->fnblk) ALLOC(fblk_first, md->fnblk) ALLOC(lblk_rep, md->lnblk) ALLOC(lblk_first, md->lnblk)
     ALLOC(wl_item, (size_t)md->nblk + md->lnblk + md->ncb) ALLOC(wl_n, 4) ALLOC(ranges, (size_t)off_cell * NRANGE) ALLOC(cblk_rep, off_cell) ALLOC(cblk_cell, off_cell) ALLOC(partial, 4 * (size_t)md->fnblk)
     ALLOC(fx, off) ALLOC(fy, off) ALLOC(fz, off) ALLOC(xo, off) ALLOC(yo, off) ALLOC(zo, off) ALLOC(pos4, off) ALLOC(posq, off)
-    ALLOC(rank, off) ALLOC(nn, off) ALLOC(nn_ref, off) ALLOC(nl, (size_t)md->cap * off) ALLOC(cnt, off_cell) ALLOC(start, off_cell)
+    ALLOC(cell_mem, (size_t)off_cell * CELL_CAP) ALLOC(nn, off) ALLOC(nn_ref, off) ALLOC(nl, (size_t)md->cap * off) ALLOC(cnt, off_cell) ALLOC(start, off_cell)
     ALLOC(members, off) ALLOC(kx, off_act) ALLOC(ky, off_act) ALLOC(kz, off_act)
 #undef ALLOC
     if ((rc = alloc_parts(md->P, off)) != AC4_OK || (rc = alloc_parts(md->A, off)) != AC4_OK) { ac4_md_destroy(md); return rc; }
@@ -1567,7 +1564,7 @@ extern "C" int ac4_md_destroy(ac4_md *md) {
                     md->lblk_rep, md->lblk_first, md->wl_item, md->wl_n, md->ranges,
                     md->cblk_rep, md->cblk_cell, md->ox, md->oy, md->oz, md->ov_q, md->ov_q2,
                     md->ov_step, md->ov_cnt, md->ov_n,
-                    md->partial, md->fx, md->fy, md->fz, md->xo, md->yo, md->zo, md->pos4, md->posq, md->rank, md->nn,
+                    md->partial, md->fx, md->fy, md->fz, md->xo, md->yo, md->zo, md->pos4, md->posq, md->cell_mem, md->nn,
                     md->nn_ref, md->nl,
                     md->cnt, md->start, md->members, md->kx, md->ky, md->kz, md->ex, md->ey, md->ez, md->et};
     for (void *q : ptrs)
@@ -2095,11 +2092,9 @@ static int launch_rebuild(ac4_md *md, bool with_zero = false) {
     if (with_zero)
         hipLaunchKernelGGL(k_cell_zero, dim3(md->R, cy), dim3(BLK), 0, s, md->reps, md->chain, md->cnt, md->wl_n);
     hipLaunchKernelGGL(k_bin, dim3(md->nblk), dim3(BLK), 0, s, md->reps, md->chain, md->blk_rep, md->blk_first,
-                       md->P.x, md->P.y, md->P.z, md->cnt, md->P.cell_of, md->rank, md->P.cell_ref, md->start, wl);
-    hipLaunchKernelGGL(k_scatter, dim3(gb), dim3(BLK), 0, s, md->reps, md->blk_rep, md->blk_first, wl,
-                       md->P.cell_of, md->rank, md->start, md->members);
+                       md->P.x, md->P.y, md->P.z, md->cnt, md->P.cell_of, md->cell_mem, md->P.cell_ref, md->start, wl);
     hipLaunchKernelGGL(k_cell_sort, dim3(gc), dim3(BLK), 0, s, md->reps, wl, md->cnt, md->start, md->P.id,
-                       md->members, md->ranges, md->P, md->A);
+                       md->cell_mem, md->members, md->ranges, md->P, md->A);
     hipLaunchKernelGGL(k_commit, dim3(gb), dim3(BLK), 0, s, md->reps, md->blk_rep, md->blk_first, wl, md->A, md->P,
                        md->pos4, md->posq, md->xo, md->yo, md->zo);
     if (with_zero)  // update_cell_list: the reference's FP64 test and its 2.85 count
