@@ -657,7 +657,7 @@ def set_skin(skin=0.35):
     _check(lib.ac4_set_skin(skin))
 
 
-def set_tuning(tpp=0, nlist_mode=4):
+def set_tuning(tpp=0, nlist_mode=3):
     """Kernel shape for handles created afterwards (ac4_set_tuning): force
     team width tpp (0 = automatic) and dynamics list builder nlist_mode."""
     load()

@@ -424,13 +424,34 @@ struct PartArrays {  // particle arrays that move with a particle
     int *type, *act, *id, *cell_ref, *cell_of;
 };
 
-// new slot gn <- old slot go of every array that moves with a particle
-__device__ __forceinline__ void gather_one(const PartArrays &src, const PartArrays &dst, int gn, int go) {
-    dst.x[gn] = src.x[go]; dst.y[gn] = src.y[go]; dst.z[gn] = src.z[go];
-    dst.xu[gn] = src.xu[go]; dst.yu[gn] = src.yu[go]; dst.zu[gn] = src.zu[go];
+// The particle arrays as a table of 2*PART_FIELDS device pointers: P's
+// x y z xu yu zu px py pz (doubles) type act id cell_ref cell_of (ints), then
+// the permuted copy A's in the same order.
+#define PART_FIELDS 14
+
+// the position copies the list builders and the force read, at a new slot
+struct PosCopies {
+    double4 *pos4;
+    uint4 *posq;
+    double *xo, *yo, *zo;  // x_u at this rebuild (update_cell_list, cell_list+forces.c:141-146)
+};
+
+// new slot gn <- old slot go of every array that moves with a particle, into
+// dst, and the position copies at gn
+__device__ __forceinline__ void gather_one(const PartArrays &src, const PartArrays &dst, const PosCopies &pc, int gn,
+                                           int go) {
+    const double sx = src.x[go], sy = src.y[go], sz = src.z[go], ux = src.xu[go], uy = src.yu[go], uz = src.zu[go];
+    const int t = src.type[go];
+    dst.x[gn] = sx; dst.y[gn] = sy; dst.z[gn] = sz;
+    dst.xu[gn] = ux; dst.yu[gn] = uy; dst.zu[gn] = uz;
     dst.px[gn] = src.px[go]; dst.py[gn] = src.py[go]; dst.pz[gn] = src.pz[go];
-    dst.type[gn] = src.type[go]; dst.act[gn] = src.act[go]; dst.id[gn] = src.id[go];
+    dst.type[gn] = t; dst.act[gn] = src.act[go]; dst.id[gn] = src.id[go];
     dst.cell_ref[gn] = src.cell_ref[go]; dst.cell_of[gn] = src.cell_of[go];
+    pc.pos4[gn] = make_double4(sx, sy, sz, (double)t);
+    pc.posq[gn] = qpos(sx, sy, sz, t);
+    pc.xo[gn] = ux;
+    pc.yo[gn] = uy;
+    pc.zo[gn] = uz;
 }
 
 // Members of a cell in increasing original index (the binning order of
@@ -443,7 +464,8 @@ __device__ __forceinline__ void gather_one(const PartArrays &src, const PartArra
 __global__ void __launch_bounds__(BLK) k_cell_sort(const RepC *__restrict__ reps, WorkList wl,
                                                    const int *__restrict__ cnt, const int *__restrict__ start,
                                                    const int *__restrict__ id, const int *__restrict__ cell_mem,
-                                                   int *members, int2 *ranges, PartArrays src, PartArrays dst) {
+                                                   int *members, int2 *ranges, PartArrays src, PartArrays dst,
+                                                   PosCopies pc) {
     __shared__ int s_m[BLK / 64][SORT_CAP];
     __shared__ int s_k[BLK / 64][SORT_CAP];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -484,7 +506,7 @@ __global__ void __launch_bounds__(BLK) k_cell_sort(const RepC *__restrict__ reps
                 int rank = 0;
                 for (int q = 0; q < n; q++) rank += s_k[w][q] < v;
                 m[rank] = s_m[w][k];
-                gather_one(src, dst, R.off + first + rank, R.off + s_m[w][k]);
+                gather_one(src, dst, pc, R.off + first + rank, R.off + s_m[w][k]);
             }
             __syncwarp();
         } else {
@@ -500,16 +522,17 @@ __global__ void __launch_bounds__(BLK) k_cell_sort(const RepC *__restrict__ reps
             }
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");  // lane 0's order, read by every lane
             __syncwarp();
-            for (int k = lane; k < n; k += 64) gather_one(src, dst, R.off + first + k, R.off + m[k]);
+            for (int k = lane; k < n; k += 64) gather_one(src, dst, pc, R.off + first + k, R.off + m[k]);
         }
     }
 }
 
 
+// the permuted copy back to the primary arrays, to[g] <- from[g] (the list
+// builders other than the teams, which move their own particles back)
 __global__ void __launch_bounds__(BLK) k_commit(const RepC *__restrict__ reps, const int *__restrict__ blk_rep,
-                                                const int *__restrict__ blk_first, WorkList wl,
-                                                PartArrays src, PartArrays dst, double4 *pos4, uint4 *posq,
-                                                double *xo, double *yo, double *zo) {
+                                                const int *__restrict__ blk_first, WorkList wl, PartArrays to,
+                                                PartArrays from) {
     const int nit = wl.n[0];
     for (int it = blockIdx.x; it < nit; it += gridDim.x) {
         const int b = wl.item[it], r = blk_rep[b];
@@ -517,18 +540,11 @@ __global__ void __launch_bounds__(BLK) k_commit(const RepC *__restrict__ reps, c
         const int i = blk_first[b] + threadIdx.x;
         if (i >= R.n) continue;
         const int g = R.off + i;
-        const double sx = src.x[g], sy = src.y[g], sz = src.z[g], ux = src.xu[g], uy = src.yu[g], uz = src.zu[g];
-        const int t = src.type[g];
-        dst.x[g] = sx; dst.y[g] = sy; dst.z[g] = sz;
-        dst.xu[g] = ux; dst.yu[g] = uy; dst.zu[g] = uz;
-        dst.px[g] = src.px[g]; dst.py[g] = src.py[g]; dst.pz[g] = src.pz[g];
-        dst.type[g] = t; dst.act[g] = src.act[g]; dst.id[g] = src.id[g];
-        dst.cell_ref[g] = src.cell_ref[g]; dst.cell_of[g] = src.cell_of[g];
-        pos4[g] = make_double4(sx, sy, sz, (double)t);
-        posq[g] = qpos(sx, sy, sz, t);
-        xo[g] = ux;  // update_cell_list, cell_list+forces.c:141-146
-        yo[g] = uy;
-        zo[g] = uz;
+        to.x[g] = from.x[g]; to.y[g] = from.y[g]; to.z[g] = from.z[g];
+        to.xu[g] = from.xu[g]; to.yu[g] = from.yu[g]; to.zu[g] = from.zu[g];
+        to.px[g] = from.px[g]; to.py[g] = from.py[g]; to.pz[g] = from.pz[g];
+        to.type[g] = from.type[g]; to.act[g] = from.act[g]; to.id[g] = from.id[g];
+        to.cell_ref[g] = from.cell_ref[g]; to.cell_of[g] = from.cell_of[g];
     }
 }
 
@@ -740,21 +756,39 @@ __device__ __forceinline__ void nlist_team_one(const RepC &R, int r, int i, Chai
 // Fixed grid over the step's list blocks (LPB particles each, one team per
 // particle); plain block order spreads the few rebuilding replicas over all
 // XCDs.
-template <int NLT>
+template <int NLT, bool MOVE>
 __global__ void __launch_bounds__(NLT * LPB) k_nlist_team(const RepC *__restrict__ reps, ChainS *chain,
                                                           const int *__restrict__ lblk_rep,
                                                           const int *__restrict__ lblk_first, WorkList wl,
                                                           const uint4 *__restrict__ posq,
                                                           const int *__restrict__ cell_of,
-                                                          const int2 *__restrict__ ranges, int *nn, int *nl, int cap) {
+                                                          const int2 *__restrict__ ranges, int *nn, int *nl, int cap,
+                                                          void *const *__restrict__ ftab) {
     __shared__ int2 s_ct[LPB][CH_CAP];  // each team's chunk list
     const int nit = wl.n[1];
     for (int it = blockIdx.x; it < nit; it += gridDim.x) {
         const int b = wl.item[wl.off_l + it], r = lblk_rep[b];
         const RepC R = reps[r];
+        // MOVE: the block's LPB particles go back from the permuted copy
+        // (second half of ftab) to the primary arrays (first half) here
+        // instead of in k_commit: thread t moves field t / LPB of particle
+        // t % LPB, loaded here and stored after the lists, so each field's LPB
+        // consecutive slots are one coalesced write.  The list build reads
+        // cell_of of the copy, nothing else.  The held value costs a wave of
+        // occupancy, so this pays where k_commit's launch dominates (small
+        // ensembles), not for large boxes.
+        const int cf = (int)threadIdx.x / LPB, cg = R.off + lblk_first[b] + (int)threadIdx.x % LPB;
+        const bool cpy = MOVE && cf < PART_FIELDS && cg < R.off + R.n;
+        long long cv = 0;
+        if (cpy)
+            cv = cf < 9 ? ((const long long *)ftab[PART_FIELDS + cf])[cg] : ((const int *)ftab[PART_FIELDS + cf])[cg];
         const int i = lblk_first[b] + (int)threadIdx.x / NLT;
         if (i < R.n)  // whole team
             nlist_team_one<NLT>(R, r, i, chain, posq, cell_of, ranges, nn, nl, cap, s_ct[threadIdx.x / NLT]);
+        if (cpy) {
+            if (cf < 9) ((long long *)ftab[cf])[cg] = cv;
+            else ((int *)ftab[cf])[cg] = (int)cv;
+        }
     }
 }
 
@@ -1258,6 +1292,7 @@ struct ac4_md {
     int *lblk_rep = nullptr, *lblk_first = nullptr;  // list-build block table (LPB particles per block)
     double *partial = nullptr;
     PartArrays P{}, A{};  // primary and permutation scratch
+    void **ftab = nullptr;  // their pointers on the device (PART_FIELDS of P, then of A)
     double *fx = nullptr, *fy = nullptr, *fz = nullptr, *xo = nullptr, *yo = nullptr, *zo = nullptr;
     double4 *pos4 = nullptr;
     uint4 *posq = nullptr;
@@ -1324,7 +1359,7 @@ static void free_parts(PartArrays &a) {
 // 0 = one wave per cell, 1 = one thread per particle, 2/3/4 = teams of
 // 16/32/64 lanes per particle.
 #define TPP_LANES 65536.0
-static int g_tpp = 0, g_nlist_mode = 4;
+static int g_tpp = 0, g_nlist_mode = 3;
 static double g_skin = K_SKIN;
 
 // Verlet skin of handles created afterwards (0.2 ... 0.6; the reference's is
@@ -1480,6 +1515,21 @@ extern "C" int ac4_md_create_slots(int nslots, const int *cap_n, int precision, 
     ALLOC(members, off) ALLOC(kx, off_act) ALLOC(ky, off_act) ALLOC(kz, off_act)
 #undef ALLOC
     if ((rc = alloc_parts(md->P, off)) != AC4_OK || (rc = alloc_parts(md->A, off)) != AC4_OK) { ac4_md_destroy(md); return rc; }
+    {
+        void *tab[2 * PART_FIELDS];
+        for (int k = 0; k < 2; k++) {
+            const PartArrays &a = k ? md->A : md->P;
+            void *f[PART_FIELDS] = {a.x, a.y, a.z, a.xu, a.yu, a.zu, a.px, a.py, a.pz,
+                                    a.type, a.act, a.id, a.cell_ref, a.cell_of};
+            for (int q = 0; q < PART_FIELDS; q++) tab[k * PART_FIELDS + q] = f[q];
+        }
+        if ((rc = dalloc(&md->ftab, 2 * PART_FIELDS)) != AC4_OK) { ac4_md_destroy(md); return rc; }
+        if (hipMemcpy(md->ftab, tab, sizeof tab, hipMemcpyHostToDevice) != hipSuccess) {
+            ac4_md_destroy(md);
+            ac4_set_error("ac4_md_create: field table upload failed");
+            return AC4_EHIP;
+        }
+    }
     if (hipStreamCreate(&md->stream) != hipSuccess || hipStreamCreate(&md->side) != hipSuccess ||
         hipEventCreateWithFlags(&md->ev_fork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&md->ev_join, hipEventDisableTiming) != hipSuccess) {
@@ -1574,6 +1624,7 @@ extern "C" int ac4_md_destroy(ac4_md *md) {
         if (q) (void)hipFree(q);
     free_parts(md->P);
     free_parts(md->A);
+    if (md->ftab) (void)hipFree(md->ftab);
     if (md->stream) (void)hipStreamDestroy(md->stream);
     if (md->side) (void)hipStreamDestroy(md->side);
     if (md->ev_fork) (void)hipEventDestroy(md->ev_fork);
@@ -2084,6 +2135,7 @@ extern "C" int ac4_md_set_rebuild_criterion(ac4_md *md, int criterion) {
 // (update_cell_list) with_zero clears them first.
 // fixed grid of the work-list kernels: enough workgroups to fill the device
 #define WL_GRID 1024
+#define TEAM_MOVE_N 400000
 static int launch_rebuild(ac4_md *md, bool with_zero = false) {
     hipStream_t s = md->stream;
     const int cy = (md->max_ncell + BLK - 1) / BLK;
@@ -2094,18 +2146,26 @@ static int launch_rebuild(ac4_md *md, bool with_zero = false) {
     hipLaunchKernelGGL(k_bin, dim3(md->nblk), dim3(BLK), 0, s, md->reps, md->chain, md->blk_rep, md->blk_first,
                        md->P.x, md->P.y, md->P.z, md->cnt, md->P.cell_of, md->cell_mem, md->P.cell_ref, md->start, wl);
     hipLaunchKernelGGL(k_cell_sort, dim3(gc), dim3(BLK), 0, s, md->reps, wl, md->cnt, md->start, md->P.id,
-                       md->cell_mem, md->members, md->ranges, md->P, md->A);
-    hipLaunchKernelGGL(k_commit, dim3(gb), dim3(BLK), 0, s, md->reps, md->blk_rep, md->blk_first, wl, md->A, md->P,
-                       md->pos4, md->posq, md->xo, md->yo, md->zo);
+                       md->cell_mem, md->members, md->ranges, md->P, md->A,
+                       PosCopies{md->pos4, md->posq, md->xo, md->yo, md->zo});
+    // 16- and 32-lane list teams of handles up to TEAM_MOVE_N particles move
+    // the permuted arrays back themselves (measured: 64 replicas, 1.5e5
+    // particles, 134.5 -> 130.0 us per step; one box of 1e6, 686 -> 701 us)
+    const bool team_moves = !with_zero && (md->nlist_mode == 2 || md->nlist_mode == 3) && md->ntot <= TEAM_MOVE_N;
+    if (!team_moves)
+        hipLaunchKernelGGL(k_commit, dim3(gb), dim3(BLK), 0, s, md->reps, md->blk_rep, md->blk_first, wl, md->P,
+                           md->A);
     if (with_zero)  // update_cell_list: the reference's FP64 test and its 2.85 count
         hipLaunchKernelGGL(k_nlist<true>, dim3(md->nblk), dim3(BLK), 0, s, md->reps, md->chain, md->blk_rep,
                            md->blk_first, md->pos4, md->posq, md->P.cell_of, md->cnt, md->start, md->nn, md->nn_ref,
                            md->nl, md->stride, md->cap);
     else if (md->nlist_mode >= 2) {
         const int w = md->nlist_mode == 2 ? 16 : (md->nlist_mode == 3 ? 32 : 64);
-        auto kern = w == 16 ? k_nlist_team<16> : (w == 32 ? k_nlist_team<32> : k_nlist_team<64>);
+        auto kern = w == 16 ? (team_moves ? k_nlist_team<16, true> : k_nlist_team<16, false>)
+                            : (w == 32 ? (team_moves ? k_nlist_team<32, true> : k_nlist_team<32, false>)
+                                       : k_nlist_team<64, false>);
         hipLaunchKernelGGL(kern, dim3(gl), dim3(w * LPB), 0, s, md->reps, md->chain, md->lblk_rep, md->lblk_first, wl,
-                           md->posq, md->P.cell_of, md->ranges, md->nn, md->nl, md->cap);
+                           md->posq, md->A.cell_of, md->ranges, md->nn, md->nl, md->cap, md->ftab);
     }
     else if (md->nlist_mode == 0)
         hipLaunchKernelGGL(k_nlist_cell, dim3(md->ncell_tot), dim3(64), 0, s, md->reps, md->chain, md->cblk_rep,

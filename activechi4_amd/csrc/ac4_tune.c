@@ -75,7 +75,7 @@ int ac4_autotune(int n_rep, const ac4_replica_params *p, int precision, int rela
     static const double skins[4] = {0.30, 0.35, 0.45, 0.55};
     rep_state *st = calloc((size_t)n_rep, sizeof *st);
     ac4_md *md = NULL;
-    int rc = ac4_set_tuning(0, 4);
+    int rc = ac4_set_tuning(0, 3);
     if (rc == AC4_OK) rc = ac4_set_skin(0.35);
     if (rc == AC4_OK) rc = ac4_md_create(n_rep, p, precision, &md);
     long long nsum = 0;
@@ -105,12 +105,12 @@ int ac4_autotune(int n_rep, const ac4_replica_params *p, int precision, int rela
     ac4_md_destroy(md);
     if (rc != AC4_OK) {
         free_states(st, n_rep);
-        ac4_set_tuning(0, 4);
+        ac4_set_tuning(0, 3);
         ac4_set_skin(0.35);
         return rc;
     }
     double best = -1.0, bs = 0.35;
-    int bt = 0, bm = 4, nc = 0;
+    int bt = 0, bm = 3, nc = 0;
 #define RECORD(T_, M_, S_, V_)                                                        \
     do {                                                                               \
         if (table && nc < table_cap) {                                                 \
@@ -137,7 +137,7 @@ int ac4_autotune(int n_rep, const ac4_replica_params *p, int precision, int rela
     free_states(st, n_rep);
     if (n_cand) *n_cand = nc;
     if (best <= 0.0) {
-        ac4_set_tuning(0, 4);
+        ac4_set_tuning(0, 3);
         ac4_set_skin(0.35);
         ac4_set_error("ac4_autotune: no candidate ran");
         return AC4_EHIP;

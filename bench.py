@@ -80,7 +80,7 @@ def start(ac4, ps, precision, relax):
 
 def default_shape(ac4):
     """Kernel shape chosen automatically from the handle size, reference skin 0.35."""
-    ac4.set_tuning(0, 4)
+    ac4.set_tuning(0, 3)
     ac4.set_skin(0.35)
 
 

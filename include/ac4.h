@@ -147,7 +147,7 @@ int ac4_philox4x32(const uint32_t *ctr4, const uint32_t *key2, uint32_t *out4);
  * The team width fixes the summation order of each particle's force, so a
  * replica's trajectory is bitwise reproducible for a given handle shape.
  * nlist_mode = dynamics list builder: 0 wave per cell, 1 thread per particle,
- * 2/3/4 teams of 16/32/64 lanes per particle (4 is the default).  All
+ * 2/3/4 teams of 16/32/64 lanes per particle (3 is the default).  All
  * builders produce the same lists in the same order. */
 int ac4_set_tuning(int tpp, int nlist_mode);
 /* Verlet skin of handles created afterwards, in [0.2, 0.6] (default 0.35 =

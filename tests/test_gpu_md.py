@@ -24,13 +24,13 @@ def make_md(ac4, states, f0=1.5, T=0.6, tau=0, prec=0, seeds=None):
 
 
 @contextlib.contextmanager
-def tuning(ac4, tpp, nlist_mode=4):
+def tuning(ac4, tpp, nlist_mode=3):
     """Kernel shape for handles created inside the block (default restored)."""
     ac4.set_tuning(tpp, nlist_mode)
     try:
         yield
     finally:
-        ac4.set_tuning(0, 4)
+        ac4.set_tuning(0, 3)
         ac4.set_skin(0.35)
 
 
