@@ -172,8 +172,9 @@ static inline int xcd_grid(int nb) { return ((nb + N_XCD - 1) / N_XCD) * N_XCD; 
 // global memory.  The shuffle counter is advanced by the force epilogue (or
 // the host for an explicit shuffle), after every component has read it.
 #define SHUF_CH 4096
+#define SHUF_PAR_MIN 16384  // active sets above this take the parallel path (k_shuf_part, k_shuf_apply)
 __global__ void __launch_bounds__(256) k_shuffle(const RepC *__restrict__ reps, const ChainS *__restrict__ chain,
-                                                 int8_t *kx, int8_t *ky, int8_t *kz, int force, int kcap) {
+                                                 int8_t *kx, int8_t *ky, int8_t *kz, int force, int kcap, int skip_large) {
     extern __shared__ unsigned int s_shuf[];  // SHUF_CH partners, then kcap bytes of directions
     unsigned int *s_j = s_shuf;
     int8_t *s_k = (int8_t *)(s_shuf + SHUF_CH);
@@ -183,6 +184,7 @@ __global__ void __launch_bounds__(256) k_shuffle(const RepC *__restrict__ reps, 
     if (!force && (c.mode == 1 || R.tau <= 0 || (c.step % R.tau) != 0)) return;
     const uint64_t sh = (uint64_t)c.shuffles;
     const int na = R.nact;
+    if (skip_large && na > SHUF_PAR_MIN) return;
     int8_t *kg = (comp == 0 ? kx : (comp == 1 ? ky : kz)) + R.off_act;
     const bool lds = na <= kcap;
     int8_t *k = lds ? s_k : kg;
@@ -205,6 +207,103 @@ __global__ void __launch_bounds__(256) k_shuffle(const RepC *__restrict__ reps, 
     __syncthreads();
     if (lds)
         for (int i = threadIdx.x; i < na; i += 256) kg[i] = s_k[i];
+}
+
+// The same swap sequence in parallel for large active sets.  The swaps
+// i = b*SHUF_M ... b*SHUF_M + SHUF_M - 1 of block b move values only among the
+// positions they touch: the block's own SHUF_M positions and the partners j_i
+// outside them.  k_shuf_part finds, for block b, where the value at each
+// touched position comes from (a position of the state before the block),
+// one wave per block replaying the swaps on that map (lane 0 swaps, the wave
+// searches the list of outside partners); k_shuf_apply then applies the
+// blocks in order, each as one gather over its touched positions.  The
+// result is the serial chain's, value for value.
+#define SHUF_M 256
+__global__ void __launch_bounds__(64) k_shuf_part(const RepC *__restrict__ reps, const ChainS *__restrict__ chain,
+                                                  int force, int bmax, int2 *__restrict__ ent, int *__restrict__ ent_n) {
+    __shared__ int s_own[SHUF_M];            // source of own position b0 + q
+    __shared__ int s_fp[SHUF_M], s_fs[SHUF_M];  // outside positions and their sources
+    __shared__ unsigned int s_j[SHUF_M];
+    __shared__ int s_nf;
+    const int blk = blockIdx.x, comp = blockIdx.y, r = blockIdx.z;
+    const RepC R = reps[r];
+    const ChainS &c = chain[r];
+    if (!force && (c.mode == 1 || R.tau <= 0 || (c.step % R.tau) != 0)) return;
+    const int na = R.nact, b0 = blk * SHUF_M;
+    if (na <= SHUF_PAR_MIN || b0 >= na) return;
+    const int m = min(SHUF_M, na - b0), lane = threadIdx.x;
+    const uint64_t sh = (uint64_t)c.shuffles;
+    for (int q = lane; q < m; q += 64) {
+        s_own[q] = b0 + q;
+        s_j[q] = (unsigned int)(int)((double)na * ac4_shuffle_uniform(R.key0, R.key1, sh, comp, b0 + q));
+    }
+    if (lane == 0) s_nf = 0;
+    __syncthreads();
+    for (int q = 0; q < m; q++) {  // MDcode/active.c:58-63, swap(k[i], k[j]) on the source map
+        const int j = (int)s_j[q];
+        int f = -1;  // outside slot of j
+        if (j < b0 || j >= b0 + m) {
+            const int nf = s_nf;
+            for (int k0 = 0; k0 < nf; k0 += 64) {
+                const unsigned long long hit = __ballot(k0 + lane < nf && s_fp[k0 + lane] == j);
+                if (hit) {
+                    f = k0 + __ffsll((long long)hit) - 1;
+                    break;
+                }
+            }
+            if (f < 0) {  // first touch of j: its value is its own
+                f = nf;
+                if (lane == 0) { s_fp[nf] = j; s_fs[nf] = j; s_nf = nf + 1; }
+            }
+        }
+        if (lane == 0) {
+            const int a = s_own[q];
+            if (f < 0) { s_own[q] = s_own[j - b0]; s_own[j - b0] = a; }
+            else { s_own[q] = s_fs[f]; s_fs[f] = a; }
+        }
+        __syncthreads();
+    }
+    const size_t e0 = ((size_t)(r * 3 + comp) * bmax + blk) * (2 * SHUF_M);
+    const int nf = s_nf;
+    for (int q = lane; q < m; q += 64) ent[e0 + q] = make_int2(b0 + q, s_own[q]);
+    for (int q = lane; q < nf; q += 64) ent[e0 + m + q] = make_int2(s_fp[q], s_fs[q]);
+    if (lane == 0) ent_n[(size_t)(r * 3 + comp) * bmax + blk] = m + nf;
+}
+
+__global__ void __launch_bounds__(2 * SHUF_M) k_shuf_apply(const RepC *__restrict__ reps,
+                                                         const ChainS *__restrict__ chain, int8_t *kx, int8_t *ky,
+                                                         int8_t *kz, int force, int kcap, int bmax,
+                                                         const int2 *__restrict__ ent, const int *__restrict__ ent_n) {
+    extern __shared__ unsigned int s_shuf[];  // same layout as k_shuffle: directions after SHUF_CH words
+    int8_t *s_k = (int8_t *)(s_shuf + SHUF_CH);
+    const int r = blockIdx.x, comp = blockIdx.y;
+    const RepC R = reps[r];
+    const ChainS &c = chain[r];
+    if (!force && (c.mode == 1 || R.tau <= 0 || (c.step % R.tau) != 0)) return;
+    const int na = R.nact;
+    if (na <= SHUF_PAR_MIN) return;
+    int8_t *kg = (comp == 0 ? kx : (comp == 1 ? ky : kz)) + R.off_act;
+    const bool lds = na <= kcap;
+    int8_t *k = lds ? s_k : kg;
+    if (lds)
+        for (int i = threadIdx.x; i < na; i += 2 * SHUF_M) s_k[i] = kg[i];
+    __syncthreads();
+    const int nb = (na + SHUF_M - 1) / SHUF_M;
+    for (int blk = 0; blk < nb; blk++) {  // blocks in swap order
+        const size_t cb = (size_t)(r * 3 + comp) * bmax + blk;
+        const int ne = ent_n[cb];
+        int2 e = make_int2(0, 0);
+        int8_t v = 0;
+        if ((int)threadIdx.x < ne) {
+            e = ent[cb * (2 * SHUF_M) + threadIdx.x];
+            v = k[e.y];
+        }
+        __syncthreads();
+        if ((int)threadIdx.x < ne) k[e.x] = v;
+        __syncthreads();
+    }
+    if (lds)
+        for (int i = threadIdx.x; i < na; i += 2 * SHUF_M) kg[i] = s_k[i];
 }
 
 __global__ void __launch_bounds__(BLK) k_first_half(const RepC *__restrict__ reps, const ChainS *chain,
@@ -1286,6 +1385,9 @@ struct ac4_md {
     int *cblk_rep = nullptr, *cblk_cell = nullptr;
     int fnblk = 0, lnblk = 0, ncb = 0;
     int shuf_kcap = 0;     // largest Nact whose directions k_shuffle keeps in LDS
+    int shuf_bmax = 0;     // swap blocks of the largest slot, 0 without the parallel shuffle path
+    int2 *shuf_ent = nullptr;  // its per-block source maps
+    int *shuf_ent_n = nullptr;
     size_t shuf_lds = 0;   // dynamic LDS of k_shuffle
     int *wl_item = nullptr, *wl_n = nullptr;  // rebuild work list (WorkList)
     int2 *ranges = nullptr;                    // per cell: stencil slot ranges (NRANGE)
@@ -1498,15 +1600,22 @@ extern "C" int ac4_md_create_slots(int nslots, const int *cap_n, int precision, 
         md->shuf_kcap = std::max(0, std::min(room, (cap_max + 15) & ~15));
         md->shuf_lds = SHUF_CH * sizeof(unsigned int) + (size_t)md->shuf_kcap;
         if (md->shuf_lds > 65536 &&
-            hipFuncSetAttribute((const void *)k_shuffle, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)md->shuf_lds) != hipSuccess) {
+            (hipFuncSetAttribute((const void *)k_shuffle, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)md->shuf_lds) != hipSuccess ||
+             hipFuncSetAttribute((const void *)k_shuf_apply, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)md->shuf_lds) != hipSuccess)) {
             (void)hipGetLastError();
             md->shuf_kcap = 65536 - (int)(SHUF_CH * sizeof(unsigned int));
             md->shuf_lds = 65536;
         }
+        md->shuf_bmax = cap_max > SHUF_PAR_MIN ? (cap_max + SHUF_M - 1) / SHUF_M : 0;
     }
     int rc = AC4_OK;
 #define ALLOC(ptr, n) if ((rc = dalloc(&md->ptr, (size_t)(n))) != AC4_OK) { ac4_md_destroy(md); return rc; }
+    if (md->shuf_bmax > 0) {
+        ALLOC(shuf_ent, (size_t)nslots * 3 * md->shuf_bmax * 2 * SHUF_M)
+        ALLOC(shuf_ent_n, (size_t)nslots * 3 * md->shuf_bmax)
+    }
     ALLOC(reps, nslots) ALLOC(chain, nslots) ALLOC(blk_rep, md->nblk) ALLOC(blk_first, md->nblk) ALLOC(blk_idx, md->nblk)
     ALLOC(fblk_rep, md->fnblk) ALLOC(fblk_first, md->fnblk) ALLOC(lblk_rep, md->lnblk) ALLOC(lblk_first, md->lnblk)
     ALLOC(wl_item, (size_t)md->nblk + md->lnblk + md->ncb) ALLOC(wl_n, 4) ALLOC(ranges, (size_t)off_cell * NRANGE) ALLOC(cblk_rep, off_cell) ALLOC(cblk_cell, off_cell) ALLOC(partial, 4 * (size_t)md->fnblk)
@@ -1614,7 +1723,7 @@ extern "C" int ac4_md_destroy(ac4_md *md) {
                     md->lblk_rep, md->lblk_first, md->wl_item, md->wl_n, md->ranges,
                     md->cblk_rep, md->cblk_cell, md->ox, md->oy, md->oz, md->ov_q, md->ov_q2,
                     md->ov_step, md->ov_cnt, md->ov_n,
-                    md->partial, md->fx, md->fy, md->fz, md->xo, md->yo, md->zo, md->pos4, md->posq, md->cell_mem, md->nn,
+                    md->partial, md->fx, md->fy, md->fz, md->xo, md->yo, md->zo, md->pos4, md->posq, md->cell_mem, md->shuf_ent, md->shuf_ent_n, md->nn,
                     md->nn_ref, md->nl,
                     md->cnt, md->start, md->members, md->kx, md->ky, md->kz, md->ex, md->ey, md->ez, md->et};
     for (void *q : ptrs)
@@ -2197,7 +2306,14 @@ static int launch_force(ac4_md *md) {
 
 static void launch_shuffle(ac4_md *md, int force, hipStream_t s) {
     hipLaunchKernelGGL(k_shuffle, dim3(md->R, 3), dim3(256), md->shuf_lds, s, md->reps, md->chain, md->kx, md->ky,
-                       md->kz, force, md->shuf_kcap);
+                       md->kz, force, md->shuf_kcap, md->shuf_bmax > 0 ? 1 : 0);
+    if (md->shuf_bmax > 0) {  // slots that can hold more than SHUF_PAR_MIN active particles
+        hipLaunchKernelGGL(k_shuf_part, dim3(md->shuf_bmax, 3, md->R), dim3(64), 0, s, md->reps, md->chain, force,
+                           md->shuf_bmax, md->shuf_ent, md->shuf_ent_n);
+        hipLaunchKernelGGL(k_shuf_apply, dim3(md->R, 3), dim3(2 * SHUF_M), md->shuf_lds, s, md->reps, md->chain,
+                           md->kx, md->ky, md->kz, force, md->shuf_kcap, md->shuf_bmax, md->shuf_ent,
+                           md->shuf_ent_n);
+    }
 }
 static void launch_first_half(ac4_md *md) {
     hipLaunchKernelGGL(k_first_half, dim3(md->nblk), dim3(BLK), 0, md->stream, md->reps, md->chain, md->blk_rep,

@@ -443,3 +443,29 @@ def test_format_frame_on_device_matches_host(gpu):
                      s0[0]["act_tag"], g["kx"], g["ky"], g["kz"])
         assert md.format_frame(0) == gpu.format_frame(xb, yu, zu, s0[0]["type"])
     md.close()
+
+
+def test_parallel_shuffle_follows_reference_swap_rule(gpu):
+    """Active sets above 16384 take the parallel shuffle (per-block source
+    maps of the swap sequence, applied in block order): the directions equal
+    the oracle's serial swap chain (MDcode/active.c:58-75) on the same
+    uniforms, for a shuffle forced now and for the one a step makes."""
+    n = 200000  # 20000 active particles
+    s, L = lattice_state(n, seed=11, jitter=0.0)
+    md = make_md(gpu, [(s, L)], tau=1)
+    for it in range(2):
+        before = md.get_state(0)
+        u = md.shuffle_uniforms(0)
+        if it == 0:
+            md.shuffle()
+        else:
+            md.step(1)  # tau = 1: the step shuffles first
+            md.sync()
+        after = md.get_state(0)
+        for c, k in enumerate(("kx", "ky", "kz")):
+            ref = np.array(before[k], dtype=np.int32)
+            OL.load().or_shuffle(len(ref), OL.ip(ref), OL.dp(np.ascontiguousarray(u[c])))
+            assert np.array_equal(after[k], ref), (it, k)
+            assert after[k].sum() == 0
+        assert not np.array_equal(after["kx"], before["kx"])
+    md.close()
