@@ -469,3 +469,41 @@ def test_parallel_shuffle_follows_reference_swap_rule(gpu):
             assert after[k].sum() == 0
         assert not np.array_equal(after["kx"], before["kx"])
     md.close()
+
+
+def test_serial_and_parallel_shuffles_in_one_handle(gpu):
+    """A handle whose replicas take both shuffle paths (200 and 17000 active
+    particles): each replica's directions equal the oracle's swap chain."""
+    states = [lattice_state(n, seed=12 + k, jitter=0.0) for k, n in enumerate((2000, 170000))]
+    md = make_md(gpu, states, tau=200)
+    before = [md.get_state(r) for r in range(2)]
+    u = [md.shuffle_uniforms(r) for r in range(2)]
+    md.shuffle()
+    for r in range(2):
+        after = md.get_state(r)
+        for c, k in enumerate(("kx", "ky", "kz")):
+            ref = np.array(before[r][k], dtype=np.int32)
+            OL.load().or_shuffle(len(ref), OL.ip(ref), OL.dp(np.ascontiguousarray(u[r][c])))
+            assert np.array_equal(after[k], ref), (r, k)
+    md.close()
+
+
+def test_list_teams_on_a_large_handle(gpu):
+    """Above 4e5 particles the 32-lane teams leave the copy back to k_commit:
+    their run equals the 64-lane teams' bit for bit (same lists, same
+    order)."""
+    out = []
+    for mode in (3, 4):
+        with tuning(gpu, 0, mode):
+            s, L = lattice_state(420000, seed=13, jitter=0.0)
+            md = make_md(gpu, [(s, L)], tau=50, prec=1)
+            md.relax(100, 1.5, 1.0)
+            md.init_chain(0, seed=13)
+        md.step(60, graph=True)
+        md.sync()
+        g = md.get_state(0)
+        out.append((g, md.rebuilds(0)))
+        md.close()
+    assert out[0][1] == out[1][1] > 0
+    for k in ("xu", "yu", "zu", "px"):
+        assert np.array_equal(out[0][0][k], out[1][0][k]), k
