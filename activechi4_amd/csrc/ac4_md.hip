@@ -1417,8 +1417,7 @@ struct ac4_md {
     unsigned int *ov_cnt = nullptr;
     int *ov_n = nullptr;
     int *et = nullptr;
-    hipStream_t stream = nullptr, side = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipStream_t stream = nullptr;
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
     bool have_forces = false;
@@ -1639,11 +1638,9 @@ extern "C" int ac4_md_create_slots(int nslots, const int *cap_n, int precision, 
             return AC4_EHIP;
         }
     }
-    if (hipStreamCreate(&md->stream) != hipSuccess || hipStreamCreate(&md->side) != hipSuccess ||
-        hipEventCreateWithFlags(&md->ev_fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&md->ev_join, hipEventDisableTiming) != hipSuccess) {
+    if (hipStreamCreate(&md->stream) != hipSuccess) {
         ac4_md_destroy(md);
-        ac4_set_error("hipStreamCreate/hipEventCreate failed");
+        ac4_set_error("hipStreamCreate failed");
         return AC4_EHIP;
     }
     AC4_HIP(hipMemcpy(md->reps, md->reps_h.data(), sizeof(RepC) * nslots, hipMemcpyHostToDevice));
@@ -1735,9 +1732,6 @@ extern "C" int ac4_md_destroy(ac4_md *md) {
     free_parts(md->A);
     if (md->ftab) (void)hipFree(md->ftab);
     if (md->stream) (void)hipStreamDestroy(md->stream);
-    if (md->side) (void)hipStreamDestroy(md->side);
-    if (md->ev_fork) (void)hipEventDestroy(md->ev_fork);
-    if (md->ev_join) (void)hipEventDestroy(md->ev_join);
     delete md;
     return AC4_OK;
 }
@@ -2322,22 +2316,15 @@ static void launch_first_half(ac4_md *md) {
                        md->prec == AC4_FP32 ? 1 : 0, md->chain, md->blk_idx, md->cnt, md->wl_n);
 }
 
-// One MD step.  With fork (graph capture), the shuffle, which only the force
-// kernel depends on, runs on a second branch beside the half kick, drift and
-// list rebuild.
-static int launch_step(ac4_md *md, bool fork = false) {
-    if (fork) {
-        AC4_HIP(hipEventRecord(md->ev_fork, md->stream));
-        AC4_HIP(hipStreamWaitEvent(md->side, md->ev_fork, 0));
-        launch_shuffle(md, 0, md->side);
-        AC4_HIP(hipEventRecord(md->ev_join, md->side));
-    } else {
-        launch_shuffle(md, 0, md->stream);
-    }
+// One MD step, one chain of launches on the handle's stream (also when
+// captured in a graph).  The shuffle, which only the force kernel needs, once
+// ran on a second graph branch beside the half kick and the rebuild: the
+// fork and join cost more than the overlap gave (about 9 us per step).
+static int launch_step(ac4_md *md) {
+    launch_shuffle(md, 0, md->stream);
     launch_first_half(md);
     AC4_HIP(hipGetLastError());
     TRY(launch_rebuild(md));
-    if (fork) AC4_HIP(hipStreamWaitEvent(md->stream, md->ev_join, 0));
     return launch_force<true>(md);
 }
 
@@ -2401,7 +2388,7 @@ extern "C" int ac4_md_step_graph(ac4_md *md, int nsteps) {
     if (!md->have_forces) TRY(ac4_md_calculate_forces(md, nullptr));
     if (!md->gexec) {
         AC4_HIP(hipStreamBeginCapture(md->stream, hipStreamCaptureModeThreadLocal));
-        int rc = launch_step(md, true);
+        int rc = launch_step(md);
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(md->stream, &g);
         if (rc != AC4_OK) return rc;
