@@ -88,7 +88,6 @@ _SIGS = {
     "ac4_ana_write": (ctypes.c_int, [_VP, ctypes.c_char_p]),
     "ac4_schedule": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int, ctypes.c_double, _U64, ctypes.c_int]),
     "ac4_origin_gap": (ctypes.c_int, [_I, ctypes.c_int]),
-    "ac4_chi4_ensemble": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _D, _I, _D]),
     "ac4_set_tuning": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
     "ac4_frames_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.POINTER(_VP)]),
     "ac4_frames_destroy": (ctypes.c_int, [_VP]),
@@ -131,9 +130,25 @@ _SIGS = {
                                              ctypes.c_double]),
     "ac4_md_overlap_sample": (ctypes.c_int, [_VP]),
     "ac4_md_overlap_get": (ctypes.c_int, [_VP, ctypes.c_int, _D, _D, _I]),
+    "ac4_md_overlap_chi4": (ctypes.c_int, [_VP, ctypes.c_int, _I, _D]),
+    "ac4_md_overlap_reset": (ctypes.c_int, [_VP, ctypes.c_int]),
     "ac4_server_create": (ctypes.c_int, [ctypes.c_int, _I, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                          ctypes.POINTER(_VP)]),
     "ac4_server_destroy": (ctypes.c_int, [_VP]),
+    "ac4_server_plan": (ctypes.c_int, [ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _I,
+                                       ctypes.c_int]),
+    "ac4_server_create_budget": (ctypes.c_int, [ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                ctypes.c_double, ctypes.POINTER(_VP)]),
+    "ac4_server_enable_overlap": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double]),
+    "ac4_server_slots": (ctypes.c_int, [_VP, _I]),
+    "ac4_server_job_error": (ctypes.c_int, [_VP, ctypes.c_int]),
+    "ac4_server_job_overlap": (ctypes.c_int, [_VP, ctypes.c_int, _D, _D, _I]),
+    "ac4_md_slot_bytes": (ctypes.c_longlong, [ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "ac4_md_errors": (ctypes.c_int, [_VP, _I]),
+    "ac4_md_raise_error": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int]),
+    "ac4_server_handle": (_VP, [_VP]),
+    "ac4_server_job_slot": (ctypes.c_int, [_VP, ctypes.c_int]),
     "ac4_server_submit": (ctypes.c_int, [_VP, ctypes.POINTER(Job)]),
     "ac4_server_tick": (ctypes.c_int, [_VP, ctypes.c_double]),
     "ac4_server_pending": (ctypes.c_int, [_VP]),
@@ -282,6 +297,18 @@ class MD:
     def step(self, nsteps, graph=False):
         _check((lib.ac4_md_step_graph if graph else lib.ac4_md_step)(self.h, nsteps))
 
+    def export_xu(self):
+        """x_u of every replica in reference order (ac4_md_export_all): a list
+        of (3, N_r) arrays."""
+        nt = lib.ac4_md_total_particles(self.h)
+        a = [np.empty(nt) for _ in range(3)]
+        _check(lib.ac4_md_export_all(self.h, *[_dp(v) for v in a]))
+        out = []
+        for r in range(self.R):
+            o, n = lib.ac4_md_slot_offset(self.h, r), self.n(r)
+            out.append(np.stack([v[o:o + n] for v in a]))
+        return out
+
     def relax(self, nsteps, f_start=1.5, f_end=1.0):
         """Velocity-rescale ramp from f_start*T to f_end*T (ac4_md_relax)."""
         _check(lib.ac4_md_relax(self.h, nsteps, f_start, f_end))
@@ -366,6 +393,24 @@ class MD:
         c = np.empty(nl, dtype=np.int32)
         _check(lib.ac4_md_overlap_get(self.h, r, _dp(q), _dp(q2), _ip(c)))
         return q, q2, c
+
+    def chi4(self, reps):
+        """Pooled chi4(l), <Q>(l) and origin counts over the replicas `reps`
+        (ac4_md_overlap_chi4, FP64 on the device), l = 0..n_lags."""
+        nl = self.ov[1] + 1
+        lst = np.ascontiguousarray(reps, dtype=np.int32)
+        o = np.empty(3 * nl)
+        _check(lib.ac4_md_overlap_chi4(self.h, len(lst), _ip(lst), _dp(o)))
+        o = o.reshape(nl, 3)
+        return o[:, 0].copy(), o[:, 1].copy(), o[:, 2].astype(np.int64)
+
+    def errors(self):
+        e = np.empty(self.R, dtype=np.int32)
+        _check(lib.ac4_md_errors(self.h, _ip(e)))
+        return e
+
+    def overlap_reset(self, r):
+        _check(lib.ac4_md_overlap_reset(self.h, r))
 
     def kernel_times(self, nsteps):
         """Average ms per step of {first_half, rebuild, force, shuffle} (HIP events)."""
@@ -463,13 +508,55 @@ class Analysis:
 class Server:
     """Request loop over device-resident slots (ac4_server_* of include/ac4.h)."""
 
-    def __init__(self, slot_caps, precision=FP32, chunk=100, relax=300):
+    def __init__(self, slot_caps=None, precision=FP32, chunk=100, relax=300, budget=None, n_range=(500, 20000),
+                 overlap=None):
+        """Slots of the capacities slot_caps, or planned from an HBM byte
+        budget for requests with N in n_range (ac4_server_create_budget).
+        overlap = (origin_every, n_origin, n_lags, a): every job's Q(t)
+        accumulated on the device from its frames."""
         load()
-        caps = _i32(slot_caps)
         h = _VP()
-        _check(lib.ac4_server_create(len(caps), _ip(caps), precision, chunk, relax, ctypes.byref(h)))
+        if budget is not None:
+            oe, no, nl, a = overlap if overlap else (0, 0, 0, 0.3)
+            _check(lib.ac4_server_create_budget(float(budget), int(n_range[0]), int(n_range[1]), precision, chunk,
+                                                relax, oe, no, nl, a, ctypes.byref(h)))
+        else:
+            caps = _i32(slot_caps)
+            _check(lib.ac4_server_create(len(caps), _ip(caps), precision, chunk, relax, ctypes.byref(h)))
         self.h = h
         self.n_of = {}
+        self.n_lags = None
+        if overlap:
+            self.n_lags = overlap[2]
+            if budget is None:
+                _check(lib.ac4_server_enable_overlap(self.h, *overlap))
+
+    def slots(self):
+        ns = lib.ac4_server_slots(self.h, None)
+        caps = np.empty(ns, dtype=np.int32)
+        lib.ac4_server_slots(self.h, _ip(caps))
+        return caps
+
+    def raise_error(self, jid, bits=2):
+        """Sets error bits on the replica of running job jid, as a kernel
+        would on an overflow (ac4_md_raise_error); the next tick fails it."""
+        slot = lib.ac4_server_job_slot(self.h, jid)
+        if slot < 0:
+            raise AC4Error("job %d is not running" % jid)
+        _check(lib.ac4_md_raise_error(lib.ac4_server_handle(self.h), slot, bits))
+
+    def job_error(self, jid):
+        rc = lib.ac4_server_job_error(self.h, jid)
+        if rc < 0:
+            _check(rc)
+        return rc
+
+    def job_overlap(self, jid):
+        nl = self.n_lags + 1
+        q, q2 = np.empty(nl), np.empty(nl)
+        c = np.empty(nl, dtype=np.int32)
+        _check(lib.ac4_server_job_overlap(self.h, jid, _dp(q), _dp(q2), _ip(c)))
+        return q, q2, c
 
     def close(self):
         if self.h:
@@ -604,6 +691,56 @@ def shell_directions(ndir, seed):
     n3 = np.empty((ndir, 3))
     _check(lib.ac4_shell_directions(ndir, int(seed) & (2**64 - 1), _dp(n3)))
     return n3
+
+
+def chi4_peak(md, reps, dt_lag, first=1):
+    """The chi4(t) peak of the replicas `reps` of an MD handle with overlap
+    accumulators: chi4(l) pooled over the replicas and their origins
+    (MD.chi4, FP64 on the device), its maximum over lags >= first, and
+    jackknife standard errors over the replicas (each replica left out in
+    turn; replicas are independent runs).  Returns a dict with the curve, the
+    peak height, time (lag * dt_lag) and lag, and their errors."""
+    chi, q, c = md.chi4(reps)
+    lags = np.arange(len(chi))
+    ok = (c > 0) & (lags >= first)
+    if not ok.any():
+        return None
+    k = int(np.flatnonzero(ok)[np.argmax(chi[ok])])
+    out = {"chi4": chi, "Q": q, "origins": c, "peak": float(chi[k]), "peak_lag": k, "peak_time": k * dt_lag,
+           "Q_at_peak": float(q[k])}
+    m = len(reps)
+    if m >= 3:
+        hs, ts = [], []
+        for r in range(m):
+            sub = [x for i, x in enumerate(reps) if i != r]
+            cj, _, cc = md.chi4(sub)
+            okj = (cc > 0) & (lags >= first)
+            kj = int(np.flatnonzero(okj)[np.argmax(cj[okj])])
+            hs.append(cj[kj])
+            ts.append(kj * dt_lag)
+        hs, ts = np.array(hs), np.array(ts)
+        f = (m - 1) / m
+        out["peak_err"] = float(np.sqrt(f * ((hs - hs.mean()) ** 2).sum()))
+        out["peak_time_err"] = float(np.sqrt(f * ((ts - ts.mean()) ** 2).sum()))
+    return out
+
+
+def server_plan(budget, n_min, n_max, n_origin=0, n_lags=0):
+    """Slot capacities ac4_server_plan derives from an HBM byte budget."""
+    load()
+    caps = np.empty(65536, dtype=np.int32)
+    ns = lib.ac4_server_plan(float(budget), n_min, n_max, n_origin, n_lags, _ip(caps), len(caps))
+    if ns < 0:
+        _check(ns)
+    return caps[:ns].copy()
+
+
+def slot_bytes(cap_n, n_origin=0, n_lags=0):
+    load()
+    b = lib.ac4_md_slot_bytes(cap_n, n_origin, n_lags)
+    if b < 0:
+        _check(int(b))
+    return int(b)
 
 
 def chain_init(n, T, seed):

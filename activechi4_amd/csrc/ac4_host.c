@@ -153,28 +153,6 @@ void ac4_walk_free(ac4_walk_t *w) {
     memset(w, 0, sizeof *w);
 }
 
-int ac4_chi4_ensemble(int nrep, int nor, const double *q, const int *n_r, double *out2) {
-    if (nrep < 1 || nor < 1 || !q || !n_r || !out2) { ac4_set_error("ac4_chi4_ensemble: bad arguments"); return AC4_EARG; }
-    double s = 0.0, s2 = 0.0, nbar = 0.0, mean_r = 0.0;
-    for (int r = 0; r < nrep; r++) {
-        double a = 0.0, a2 = 0.0;
-        for (int o = 0; o < nor; o++) {
-            double v = q[(size_t)r * nor + o];
-            a += v;
-            a2 += v * v;
-        }
-        s += a;
-        s2 += a2;
-        nbar += (double)n_r[r];
-        mean_r += (double)n_r[r] * (a2 / nor - (a / nor) * (a / nor));
-    }
-    double m = (double)nrep * nor;
-    nbar /= (double)nrep;
-    out2[0] = nbar * (s2 / m - (s / m) * (s / m));
-    out2[1] = mean_r / (double)nrep;
-    return AC4_OK;
-}
-
 /* Degeneracy of a q vector, Analysis/S4Qt.c:72-76 */
 int ac4_degen(int x, int y, int z) {
     if (y != z && y != x && z != x) return 6;

@@ -1066,6 +1066,54 @@ __global__ void __launch_bounds__(BLK) k_ov_origin(const RepC *__restrict__ reps
     if (i == 0) ostep[(size_t)r * cfg.n_origin + o] = step;
 }
 
+// Ensemble chi4 from the per-lag accumulators, FP64 on the device: one thread
+// per lag sums the listed replicas in list order (so the result does not
+// depend on the launch), then Q = s/c and X4 = s2/c - s*s/(c*c) in the
+// expression order of Analysis/overlap.c:114-118, chi4 = Nbar * X4 with Nbar
+// the mean N of the listed replicas.  out[l] = {chi4, Q, origins}.
+__global__ void k_ov_chi4(const RepC *__restrict__ reps, OvCfg cfg, const int *__restrict__ list, int nlist,
+                          const double *__restrict__ sq, const double *__restrict__ sq2, const int *__restrict__ scnt,
+                          double *out) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l > cfg.n_lags) return;
+    double s = 0.0, s2 = 0.0, nbar = 0.0;
+    long long c = 0;
+    for (int k = 0; k < nlist; k++) {
+        const int r = list[k];
+        const size_t a = (size_t)r * (cfg.n_lags + 1) + l;
+        s += sq[a];
+        s2 += sq2[a];
+        c += scnt[a];
+        nbar += (double)reps[r].n;
+    }
+    nbar /= (double)nlist;
+    double q = 0.0, x4 = 0.0;
+    if (c > 0) {
+        const double dc = (double)c;
+        q = s / dc;
+        x4 = s2 / dc - s * s / (dc * dc);
+    }
+    out[3 * l] = nbar * x4;
+    out[3 * l + 1] = q;
+    out[3 * l + 2] = (double)c;
+}
+
+// Clears replica r's accumulators and origins (a slot taking a new job).
+__global__ void k_ov_reset(OvCfg cfg, int r, long long *ostep, unsigned int *ocnt, double *sq, double *sq2,
+                           int *scnt) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < cfg.n_origin) {
+        ostep[(size_t)r * cfg.n_origin + t] = -1;
+        ocnt[(size_t)r * cfg.n_origin + t] = 0u;
+    }
+    if (t <= cfg.n_lags) {
+        const size_t a = (size_t)r * (cfg.n_lags + 1) + t;
+        sq[a] = 0.0;
+        sq2[a] = 0.0;
+        scnt[a] = 0;
+    }
+}
+
 // Reference-order export of x_u and type for in-situ analysis.
 __global__ void __launch_bounds__(BLK) k_export(const RepC *__restrict__ reps, const int *__restrict__ blk_rep,
                                                 const int *__restrict__ blk_first, const int *__restrict__ id,
@@ -1406,6 +1454,8 @@ struct ac4_md {
     OvCfg ov{0, 0, 0, 0, 0.0};
     double *ox = nullptr, *oy = nullptr, *oz = nullptr, *ov_q = nullptr, *ov_q2 = nullptr;
     long long *ov_step = nullptr;
+    int *ov_list = nullptr;     // replica list of ac4_md_overlap_chi4
+    double *ov_out = nullptr;   // its per-lag results
     // frame text on the device (ac4_md_format_frame): 80-byte line slots in
     // reference order, line lengths, the joined text, per-block byte offsets
     // (the total after the last block) and a flag
@@ -1719,7 +1769,7 @@ extern "C" int ac4_md_destroy(ac4_md *md) {
     void *ptrs[] = {md->reps, md->chain, md->blk_rep, md->blk_first, md->blk_idx, md->fblk_rep, md->fblk_first,
                     md->lblk_rep, md->lblk_first, md->wl_item, md->wl_n, md->ranges,
                     md->cblk_rep, md->cblk_cell, md->ox, md->oy, md->oz, md->ov_q, md->ov_q2,
-                    md->ov_step, md->ov_cnt, md->ov_n,
+                    md->ov_step, md->ov_cnt, md->ov_n, md->ov_list, md->ov_out,
                     md->partial, md->fx, md->fy, md->fz, md->xo, md->yo, md->zo, md->pos4, md->posq, md->cell_mem, md->shuf_ent, md->shuf_ent_n, md->nn,
                     md->nn_ref, md->nl,
                     md->cnt, md->start, md->members, md->kx, md->ky, md->kz, md->ex, md->ey, md->ez, md->et};
@@ -1737,6 +1787,31 @@ extern "C" int ac4_md_destroy(ac4_md *md) {
 }
 
 extern "C" int ac4_md_total_particles(const ac4_md *md) { return md ? md->ntot : 0; }
+
+// Device bytes a slot of capacity cap_n adds to a handle, from the sizes
+// ac4_md_create_slots and ac4_md_overlap_enable allocate (the reference-order
+// export buffers included; the parallel-shuffle tables counted at their
+// largest, as if this slot were the largest of its handle).
+extern "C" long long ac4_md_slot_bytes(int cap_n, int n_origin, int n_lags) {
+    if (cap_n < 2 || n_origin < 0 || n_lags < 0) return AC4_EARG;
+    const long long n = cap_n;
+    const long long cells = (long long)pow((int)(pow((double)cap_n / 0.5, 1.0 / 3.0) / K_CELL_CUT) + 1, 3);
+    const long long nblk = (n + BLK - 1) / BLK, lnblk = (n + LPB - 1) / LPB;
+    const long long fnb = (n * 16 + BLK - 1) / BLK;  // force blocks at the widest team (16 lanes a particle)
+    const long long ncb = (cells + BLK / 64 - 1) / (BLK / 64);
+    long long b = 0;
+    b += n * (6 * 8 + 32 + 16 + 2 * 4 + 192LL * 4 + 4 + 3);         // forces, xold, pos copies, counts, list, members, k
+    b += 2 * n * (9 * 8 + 5 * 4);                                    // particle arrays and their scratch copy
+    b += n * (3 * 8 + 4);                                            // reference-order export
+    b += cells * ((long long)NRANGE * 8 + 2 * 4 + (long long)CELL_CAP * 4 + 2 * 4);
+    b += nblk * 3 * 4 + fnb * 2 * 4 + lnblk * 2 * 4;                // block tables
+    b += (nblk + lnblk + ncb) * 4 + fnb * 4 * 8;                     // work list, force partials
+    b += (long long)sizeof(RepC) + (long long)sizeof(ChainS);
+    if (cap_n > SHUF_PAR_MIN) b += 3LL * 2 * ((n + SHUF_M - 1) / SHUF_M) * SHUF_M * 4 + 3LL * ((n + SHUF_M - 1) / SHUF_M) * 4;
+    if (n_origin > 0) b += n * 24LL * n_origin + (long long)n_origin * 12 + (long long)(n_lags + 1) * 20;
+    return b;
+}
+
 
 static int check_rep(const ac4_md *md, int r) {
     AC4_REQUIRE(md && r >= 0 && r < md->R, AC4_EARG, "bad handle or replica index %d", r);
@@ -1773,6 +1848,28 @@ static int get_chains(ac4_md *md, std::vector<ChainS> &c) {
 static int put_chains(ac4_md *md, const std::vector<ChainS> &c) {
     TRY(h2d(md->chain, c.data(), md->R, md->stream));
     AC4_HIP(hipStreamSynchronize(md->stream));
+    return AC4_OK;
+}
+
+// Sets error bits on replica r as the kernels do when a list or cell
+// overflows or a coordinate runs away, without any of those happening: the
+// way tests exercise the failure handling of the callers.
+extern "C" int ac4_md_raise_error(ac4_md *md, int r, int bits) {
+    TRY(check_rep(md, r));
+    AC4_REQUIRE(bits > 0 && bits < 4, AC4_EARG, "raise_error: bits must be 1, 2 or 3");
+    ChainS c;
+    TRY(get_chain(md, r, &c));
+    c.err |= bits;
+    return put_chain(md, r, &c);
+}
+
+// Error bits of every replica (bit 0: neighbour list or cell over capacity,
+// bit 1: non-finite or runaway coordinate); err[r] for r < R.
+extern "C" int ac4_md_errors(ac4_md *md, int *err) {
+    AC4_REQUIRE(md && err, AC4_EARG, "errors: bad arguments");
+    std::vector<ChainS> c;
+    TRY(get_chains(md, c));
+    for (int r = 0; r < md->R; r++) err[r] = md->reps_h[r].n > 0 ? c[r].err : 0;
     return AC4_OK;
 }
 
@@ -2391,10 +2488,19 @@ extern "C" int ac4_md_step_graph(ac4_md *md, int nsteps) {
         int rc = launch_step(md);
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(md->stream, &g);
-        if (rc != AC4_OK) return rc;
-        AC4_HIP(e);
+        if (rc != AC4_OK || e != hipSuccess) {  // nothing of a failed capture is kept
+            if (g) (void)hipGraphDestroy(g);
+            if (rc != AC4_OK) return rc;
+            AC4_HIP(e);
+        }
+        hipGraphExec_t x = nullptr;
+        e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            (void)hipGraphDestroy(g);
+            AC4_HIP(e);
+        }
         md->graph = g;
-        AC4_HIP(hipGraphInstantiate(&md->gexec, g, nullptr, nullptr, 0));
+        md->gexec = x;
     }
     for (int k = 0; k < nsteps; k++) AC4_HIP(hipGraphLaunch(md->gexec, md->stream));
     return AC4_OK;
@@ -2527,6 +2633,8 @@ extern "C" int ac4_md_overlap_enable(ac4_md *md, int sample_every, int origin_ev
     TRY(dalloc(&md->ov_q, acc)); TRY(dalloc(&md->ov_q2, acc)); TRY(dalloc(&md->ov_n, acc));
     TRY(dalloc(&md->ov_cnt, (size_t)md->R * n_origin));
     TRY(dalloc(&md->ov_step, (size_t)md->R * n_origin));
+    TRY(dalloc(&md->ov_list, (size_t)md->R));
+    TRY(dalloc(&md->ov_out, 3 * (size_t)(n_lags + 1)));
     std::vector<long long> none((size_t)md->R * n_origin, -1);
     AC4_HIP(hipMemcpy(md->ov_step, none.data(), sizeof(long long) * none.size(), hipMemcpyHostToDevice));
     return AC4_OK;
@@ -2559,6 +2667,33 @@ extern "C" int ac4_md_overlap_get(ac4_md *md, int r, double *sq, double *sq2, in
     if (sq2) TRY(d2h(sq2, (const double *)md->ov_q2 + off, nl, md->stream));
     if (cnt) TRY(d2h(cnt, (const int *)md->ov_n + off, nl, md->stream));
     AC4_HIP(hipStreamSynchronize(md->stream));
+    return AC4_OK;
+}
+
+// chi4(l) pooled over the listed replicas and their origins (k_ov_chi4):
+// out3[3*l + {0,1,2}] = {Nbar*(<Q^2>-<Q>^2), <Q>, origins}, l = 0..n_lags.
+extern "C" int ac4_md_overlap_chi4(ac4_md *md, int nlist, const int *list, double *out3) {
+    AC4_REQUIRE(md && md->ox && nlist > 0 && list && out3, AC4_EARG, "overlap_chi4: bad arguments");
+    for (int k = 0; k < nlist; k++)
+        AC4_REQUIRE(list[k] >= 0 && list[k] < md->R && md->reps_h[list[k]].n > 0, AC4_EARG,
+                    "overlap_chi4: entry %d names replica %d, which is not a running replica", k, list[k]);
+    const int nl = md->ov.n_lags + 1;
+    AC4_HIP(hipMemcpyAsync(md->ov_list, list, sizeof(int) * nlist, hipMemcpyHostToDevice, md->stream));
+    hipLaunchKernelGGL(k_ov_chi4, dim3((nl + 255) / 256), dim3(256), 0, md->stream, md->reps, md->ov, md->ov_list,
+                       nlist, md->ov_q, md->ov_q2, md->ov_n, md->ov_out);
+    AC4_HIP(hipGetLastError());
+    TRY(d2h(out3, (const double *)md->ov_out, 3 * (size_t)nl, md->stream));
+    AC4_HIP(hipStreamSynchronize(md->stream));
+    return AC4_OK;
+}
+
+// Starts replica r's accumulators afresh (no origins, zero sums).
+extern "C" int ac4_md_overlap_reset(ac4_md *md, int r) {
+    AC4_REQUIRE(md && md->ox && r >= 0 && r < md->R, AC4_EARG, "overlap_reset: bad arguments");
+    const int nt = std::max(md->ov.n_origin, md->ov.n_lags + 1);
+    hipLaunchKernelGGL(k_ov_reset, dim3((nt + 255) / 256), dim3(256), 0, md->stream, md->ov, r, md->ov_step,
+                       md->ov_cnt, md->ov_q, md->ov_q2, md->ov_n);
+    AC4_HIP(hipGetLastError());
     return AC4_OK;
 }
 
@@ -2603,14 +2738,23 @@ struct CkHead {
     double L, skin, T, f0;
     int32_t tau, pad;
     uint64_t seed, chain_bytes, total;
+    // in-situ overlap state (ac4_md_overlap_enable) when the handle has it:
+    // the origin snapshots in reference order, their steps, the per-lag sums
+    int32_t ov_sample, ov_origin_every, ov_n_origin, ov_n_lags;
+    double ov_a2;
 };
 static const char k_ck_magic[8] = {'A', 'C', '4', 'C', 'K', 'P', 'T', '1'};
 
+static size_t ck_ov_bytes(const ac4_md *md, const RepC &R) {
+    if (!md->ox) return 0;
+    const size_t no = md->ov.n_origin, nl = md->ov.n_lags + 1;
+    return no * 3 * (size_t)R.n * sizeof(double) + no * sizeof(long long) + nl * (2 * sizeof(double) + sizeof(int));
+}
 static size_t ck_bytes(const ac4_md *md, const RepC &R) {
     const size_t n = R.n, nc = R.ncell;
     return sizeof(CkHead) + sizeof(ChainS) + n * (15 * sizeof(double) + sizeof(double4) + sizeof(uint4)) +
            n * 7 * sizeof(int) + 3 * (size_t)R.nact + nc * 2 * sizeof(int) + n * sizeof(int) +
-           nc * NRANGE * sizeof(int2) + (size_t)md->cap * n * sizeof(int);
+           nc * NRANGE * sizeof(int2) + (size_t)md->cap * n * sizeof(int) + ck_ov_bytes(md, R);
 }
 
 extern "C" long long ac4_md_checkpoint_size(ac4_md *md, int r) {
@@ -2628,7 +2772,8 @@ static int ck_io(bool save, T *dev, size_t n, char *&p, hipStream_t s) {
     return AC4_OK;
 }
 
-static int ck_arrays(ac4_md *md, const RepC &R, bool save, char *p) {
+static int ck_arrays(ac4_md *md, int r, bool save, char *p) {
+    const RepC &R = md->reps_h[r];
     hipStream_t s = md->stream;
     const size_t g = R.off, n = R.n, c0 = R.off_cell, nc = R.ncell;
     double *dv[15] = {md->P.x, md->P.y, md->P.z, md->P.xu, md->P.yu, md->P.zu, md->P.px, md->P.py, md->P.pz,
@@ -2648,6 +2793,19 @@ static int ck_arrays(ac4_md *md, const RepC &R, bool save, char *p) {
     TRY(ck_io(save, md->ranges + c0 * NRANGE, nc * NRANGE, p, s));
     int *nl = (int *)p;
     TRY(ck_io(save, md->nl + g * md->cap, (size_t)md->cap * n, p, s));
+    if (md->ox) {  // origin snapshots of this replica, its origin steps, its per-lag sums
+        const size_t no = md->ov.n_origin, nl1 = md->ov.n_lags + 1;
+        for (size_t o = 0; o < no; o++) {
+            const size_t base = g * no + o * (size_t)R.cap_n;
+            TRY(ck_io(save, md->ox + base, n, p, s));
+            TRY(ck_io(save, md->oy + base, n, p, s));
+            TRY(ck_io(save, md->oz + base, n, p, s));
+        }
+        TRY(ck_io(save, md->ov_step + (size_t)r * no, no, p, s));
+        TRY(ck_io(save, md->ov_q + (size_t)r * nl1, nl1, p, s));
+        TRY(ck_io(save, md->ov_q2 + (size_t)r * nl1, nl1, p, s));
+        TRY(ck_io(save, md->ov_n + (size_t)r * nl1, nl1, p, s));
+    }
     AC4_HIP(hipStreamSynchronize(s));
     if (save) {  // replica-local slot indices
         for (size_t k = 0; k < nc * NRANGE; k++)
@@ -2667,11 +2825,15 @@ extern "C" int ac4_md_checkpoint(ac4_md *md, int r, void *buf, long long cap) {
     CkHead h;
     memset(&h, 0, sizeof h);
     memcpy(h.magic, k_ck_magic, 8);
-    h.version = 1; h.n = R.n; h.nact = R.nact; h.ncell = R.ncell; h.cap = md->cap; h.lc = R.lc; h.tpp = R.tpp;
+    h.version = 2; h.n = R.n; h.nact = R.nact; h.ncell = R.ncell; h.cap = md->cap; h.lc = R.lc; h.tpp = R.tpp;
     h.prec = md->prec; h.L = R.L; h.skin = R.skin; h.T = R.T; h.f0 = R.f0; h.tau = R.tau;
     h.seed = (uint64_t)R.key0 | ((uint64_t)R.key1 << 32);
     h.chain_bytes = sizeof(ChainS);
     h.total = ck_bytes(md, R);
+    if (md->ox) {
+        h.ov_sample = md->ov.sample_every; h.ov_origin_every = md->ov.origin_every;
+        h.ov_n_origin = md->ov.n_origin; h.ov_n_lags = md->ov.n_lags; h.ov_a2 = md->ov.a2;
+    }
     char *p = (char *)buf;
     memcpy(p, &h, sizeof h);
     p += sizeof h;
@@ -2679,7 +2841,7 @@ extern "C" int ac4_md_checkpoint(ac4_md *md, int r, void *buf, long long cap) {
     TRY(get_chain(md, r, &c));
     memcpy(p, &c, sizeof c);
     p += sizeof c;
-    return ck_arrays(md, R, true, p);
+    return ck_arrays(md, r, true, p);
 }
 
 extern "C" int ac4_md_restore(ac4_md *md, int r, const void *buf, long long len) {
@@ -2688,13 +2850,22 @@ extern "C" int ac4_md_restore(ac4_md *md, int r, const void *buf, long long len)
     AC4_REQUIRE(buf && len >= (long long)sizeof(CkHead), AC4_EARG, "restore: no checkpoint");
     CkHead h;
     memcpy(&h, buf, sizeof h);
-    AC4_REQUIRE(!memcmp(h.magic, k_ck_magic, 8) && h.version == 1 && h.chain_bytes == sizeof(ChainS), AC4_EARG,
+    AC4_REQUIRE(!memcmp(h.magic, k_ck_magic, 8) && h.version == 2 && h.chain_bytes == sizeof(ChainS), AC4_EARG,
                 "restore: not a checkpoint of this library version");
     AC4_REQUIRE((long long)h.total == len && h.total == ck_bytes(md, R), AC4_EARG,
                 "restore: checkpoint of %lld bytes does not fit slot %d", len, r);
     AC4_REQUIRE(h.n == R.n && h.nact == R.nact && h.ncell == R.ncell && h.cap == md->cap && h.lc == R.lc &&
                     h.tpp == R.tpp && h.prec == md->prec && h.L == R.L && h.skin == R.skin,
                 AC4_EARG, "restore: the slot's replica or handle shape differs from the checkpoint's");
+    const bool ov_same = md->ox ? (h.ov_sample == md->ov.sample_every && h.ov_origin_every == md->ov.origin_every &&
+                                   h.ov_n_origin == md->ov.n_origin && h.ov_n_lags == md->ov.n_lags &&
+                                   h.ov_a2 == md->ov.a2)
+                                : h.ov_n_origin == 0;
+    AC4_REQUIRE(ov_same, AC4_EARG, "restore: the overlap accumulators of the checkpoint and the handle differ");
+    const uint64_t seed = (uint64_t)R.key0 | ((uint64_t)R.key1 << 32);
+    AC4_REQUIRE(h.seed == seed && h.T == R.T && h.f0 == R.f0 && h.tau == R.tau, AC4_EARG,
+                "restore: slot %d runs seed %llu, T %g, f0 %g, tau %d; the checkpoint seed %llu, T %g, f0 %g, tau %d",
+                r, (unsigned long long)seed, R.T, R.f0, R.tau, (unsigned long long)h.seed, h.T, h.f0, h.tau);
     std::vector<char> tmp((const char *)buf, (const char *)buf + len);  // local copy: indices are rebased
     char *p = tmp.data() + sizeof h;
     ChainS c;
@@ -2709,7 +2880,7 @@ extern "C" int ac4_md_restore(ac4_md *md, int r, const void *buf, long long len)
         if (rg[k].y > rg[k].x) { rg[k].x += (int)g; rg[k].y += (int)g; }
     int *nl = (int *)(q + nc * NRANGE * sizeof(int2));
     for (size_t k = 0; k < (size_t)md->cap * n; k++) nl[k] += (int)g;
-    TRY(ck_arrays(md, R, false, p));
+    TRY(ck_arrays(md, r, false, p));
     c.done_blocks = 0u;
     c.bin_done = 0u;
     return put_chain(md, r, &c);

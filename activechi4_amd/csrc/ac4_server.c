@@ -2,12 +2,23 @@
  * ac4_server.c -- request loop for MD jobs on one GPU (continuous batching).
  * No reference counterpart: the reference runs one job per mpirun
  * (MDcode/readme:1-11).  Jobs (N, T, f0, tau_p, steps, seed) arrive over
- * time; the server keeps a fixed set of device-resident slots (capacities
- * chosen by the caller from an HBM budget), admits queued jobs into a free
- * slot of their size class (FIFO order; a later job may take a free slot of
- * its own class while an earlier one waits), advances every slot together in chunks
- * of `chunk` steps with one replayed hipGraph, copies every running job's
- * frame (x_u) to the host after each chunk, and retires finished jobs.
+ * time; the server keeps a fixed set of device-resident slots, admits queued
+ * jobs into a free slot of their size class (FIFO order; a later job may take
+ * a free slot of its own class while an earlier one waits), advances every
+ * slot together in chunks of `chunk` steps with one replayed hipGraph, and
+ * retires finished jobs.
+ *
+ * - Slots come from an HBM byte budget (ac4_server_plan): size classes a
+ *   factor 2 apart over the request range [n_min, n_max], slot counts per
+ *   class in proportion to the share of a log-uniform N distribution each
+ *   class takes, as many as the budget holds at ac4_md_slot_bytes per slot.
+ * - Frames (one every chunk) stay on the device: each job's overlap Q(t) is
+ *   accumulated in place (ac4_md_overlap_*, keyed on the job's own step
+ *   counter) and copied out with the final x_u when the job finishes.  No
+ *   per-chunk copy of the slots' positions.
+ * - A replica whose error flag is set (list or cell over capacity, runaway
+ *   coordinate) fails alone: its job is marked failed and its slot freed and
+ *   cleared, the other jobs keep stepping.
  * Admission never changes a running replica's arithmetic (slot-local loads),
  * and a job's results do not depend on the other jobs in the batch.
  */
@@ -24,25 +35,78 @@ int ac4_md_load_slot(ac4_md *md, int r, const double *x, const double *y, const 
 
 typedef struct {
     ac4_job job;
-    int state; /* 0 queued, 1 running, 2 done */
-    int slot, frames, preemptions;
+    int state; /* 0 queued, 1 running, 2 done, 3 failed */
+    int slot, frames, preemptions, err;
     long steps_done, target;
     double t_start, t_done;
     double *result;  /* final x_u,y_u,z_u (3n) */
+    double *ov;      /* per-lag {sum Q, sum Q^2, origins} of the job (3*(n_lags+1)) */
     char *ckpt;      /* exact checkpoint of a preempted job, resumed on admission */
     long long ckpt_len;
 } job_rec;
 
 struct ac4_server {
     ac4_md *md;
-    int nslots, chunk, relax;
-    int *cap, *slot_job;
+    int nslots, chunk, relax; /* relax: ramp steps rounded up to whole chunks */
+    int *cap, *slot_job, *err;
     job_rec *jobs;
     int njobs, jcap;
-    double *hx, *hy, *hz; /* export buffers (ntot each) */
-    int ntot;
+    int ov_on, ov_nl;         /* in-situ overlap: on, lags + 1 */
     long long particle_steps;
 };
+
+/* Slot classes a factor 2 apart from the first power of two >= n_min up to
+ * n_max (the last class n_max rounded up to 256).  Class k holds the N in
+ * (c_{k-1}, c_k] and gets a slot count in proportion to
+ * ln(min(c_k, n_max) / max(c_{k-1}, n_min)), the share of a log-uniform N
+ * over [n_min, n_max] it takes (at least one slot); the total is the largest
+ * that fits hbm_bytes at ac4_md_slot_bytes per slot. */
+int ac4_server_plan(double hbm_bytes, int n_min, int n_max, int n_origin, int n_lags, int *caps, int caps_cap) {
+    if (!(hbm_bytes > 0) || n_min < 2 || n_max < n_min || n_origin < 0 || n_lags < 0 || !caps || caps_cap < 1) {
+        ac4_set_error("ac4_server_plan: bad arguments");
+        return AC4_EARG;
+    }
+    int cls[40], nc = 0;
+    long long c = 256;
+    while (c < n_min) c *= 2;
+    while (c < n_max && nc < 39) { cls[nc++] = (int)c; c *= 2; }
+    cls[nc++] = ((n_max + 255) / 256) * 256;
+    double share[40], bytes[40], tot_share = 0.0;
+    for (int k = 0; k < nc; k++) {
+        const double lo = k ? fmax((double)cls[k - 1], (double)n_min) : (double)n_min;
+        const double hi = fmin((double)cls[k], (double)n_max);
+        share[k] = hi > lo ? log(hi / lo) : 0.0;
+        tot_share += share[k];
+        long long b = ac4_md_slot_bytes(cls[k], n_origin, n_lags);
+        if (b < 0) return (int)b;
+        bytes[k] = (double)b;
+    }
+    /* largest total M whose per-class counts max(1, round(M*share)) fit */
+    int best[40], have = 0;
+    for (int m = 1; m <= 100000; m++) {
+        int cnt[40];
+        double need = 0.0;
+        int slots = 0;
+        for (int k = 0; k < nc; k++) {
+            cnt[k] = (int)floor((double)m * share[k] / tot_share + 0.5);
+            if (cnt[k] < 1) cnt[k] = 1;
+            need += cnt[k] * bytes[k];
+            slots += cnt[k];
+        }
+        if (need > hbm_bytes || slots > caps_cap) break;
+        memcpy(best, cnt, sizeof(int) * nc);
+        have = 1;
+    }
+    if (!have) {
+        ac4_set_error("ac4_server_plan: %.3g bytes hold no slot of every class (a %d-particle slot takes %.3g)",
+                      hbm_bytes, cls[nc - 1], bytes[nc - 1]);
+        return AC4_ECAP;
+    }
+    int ns = 0;
+    for (int k = 0; k < nc; k++)
+        for (int q = 0; q < best[k]; q++) caps[ns++] = cls[k];
+    return ns;
+}
 
 int ac4_server_create(int nslots, const int *slot_cap, int precision, int chunk, int relax_steps,
                       ac4_server **out) {
@@ -51,27 +115,68 @@ int ac4_server_create(int nslots, const int *slot_cap, int precision, int chunk,
         return AC4_EARG;
     }
     ac4_server *s = calloc(1, sizeof *s);
+    if (!s) { ac4_set_error("ac4_server_create: out of host memory"); return AC4_ECAP; }
     int rc = ac4_md_create_slots(nslots, slot_cap, precision, &s->md);
     if (rc != AC4_OK) { free(s); return rc; }
     s->nslots = nslots;
     s->chunk = chunk;
-    s->relax = relax_steps;
+    /* the ramp does not advance a replica's step counter: whole chunks keep
+     * every production step count a multiple of the chunk at a tick */
+    s->relax = ((relax_steps + chunk - 1) / chunk) * chunk;
     s->cap = malloc(sizeof(int) * nslots);
     s->slot_job = malloc(sizeof(int) * nslots);
+    s->err = malloc(sizeof(int) * nslots);
+    if (!s->cap || !s->slot_job || !s->err) {
+        ac4_server_destroy(s);
+        ac4_set_error("ac4_server_create: out of host memory");
+        return AC4_ECAP;
+    }
     for (int k = 0; k < nslots; k++) { s->cap[k] = slot_cap[k]; s->slot_job[k] = -1; }
-    s->ntot = ac4_md_total_particles(s->md);
-    s->hx = malloc(sizeof(double) * s->ntot);
-    s->hy = malloc(sizeof(double) * s->ntot);
-    s->hz = malloc(sizeof(double) * s->ntot);
     *out = s;
     return AC4_OK;
 }
 
+int ac4_server_create_budget(double hbm_bytes, int n_min, int n_max, int precision, int chunk, int relax_steps,
+                             int origin_every, int n_origin, int n_lags, double a, ac4_server **out) {
+    const int ov = n_origin > 0;
+    int caps_cap = 65536;
+    int *caps = malloc(sizeof(int) * caps_cap);
+    if (!caps) { ac4_set_error("ac4_server_create_budget: out of host memory"); return AC4_ECAP; }
+    int ns = ac4_server_plan(hbm_bytes, n_min, n_max, ov ? n_origin : 0, ov ? n_lags : 0, caps, caps_cap);
+    if (ns < 0) { free(caps); return ns; }
+    int rc = ac4_server_create(ns, caps, precision, chunk, relax_steps, out);
+    free(caps);
+    if (rc != AC4_OK) return rc;
+    if (ov && (rc = ac4_server_enable_overlap(*out, origin_every, n_origin, n_lags, a)) != AC4_OK) {
+        ac4_server_destroy(*out);
+        *out = NULL;
+    }
+    return rc;
+}
+
+int ac4_server_enable_overlap(ac4_server *s, int origin_every, int n_origin, int n_lags, double a) {
+    if (!s || s->njobs > 0 || s->ov_on) {
+        ac4_set_error("enable_overlap: call once, before the first job is submitted");
+        return AC4_ESTATE;
+    }
+    int rc = ac4_md_overlap_enable(s->md, s->chunk, origin_every, n_origin, n_lags, a);
+    if (rc != AC4_OK) return rc;
+    s->ov_on = 1;
+    s->ov_nl = n_lags + 1;
+    return AC4_OK;
+}
+
+int ac4_server_slots(const ac4_server *s, int *caps) {
+    if (!s) return AC4_EARG;
+    if (caps) memcpy(caps, s->cap, sizeof(int) * s->nslots);
+    return s->nslots;
+}
+
 int ac4_server_destroy(ac4_server *s) {
     if (!s) return AC4_OK;
-    ac4_md_destroy(s->md);
-    for (int j = 0; j < s->njobs; j++) { free(s->jobs[j].result); free(s->jobs[j].ckpt); }
-    free(s->jobs); free(s->cap); free(s->slot_job); free(s->hx); free(s->hy); free(s->hz);
+    if (s->md) ac4_md_destroy(s->md);
+    for (int j = 0; j < s->njobs; j++) { free(s->jobs[j].result); free(s->jobs[j].ov); free(s->jobs[j].ckpt); }
+    free(s->jobs); free(s->cap); free(s->slot_job); free(s->err);
     free(s);
     return AC4_OK;
 }
@@ -82,8 +187,11 @@ int ac4_server_submit(ac4_server *s, const ac4_job *job) {
     for (int k = 0; k < s->nslots; k++) fits |= s->cap[k] >= job->n;
     if (!fits) { ac4_set_error("submit: no slot can hold N=%d", job->n); return AC4_ECAP; }
     if (s->njobs == s->jcap) {
-        s->jcap = s->jcap ? 2 * s->jcap : 64;
-        s->jobs = realloc(s->jobs, sizeof(job_rec) * s->jcap);
+        int nc = s->jcap ? 2 * s->jcap : 64;
+        job_rec *nj = realloc(s->jobs, sizeof(job_rec) * nc);
+        if (!nj) { ac4_set_error("submit: out of host memory"); return AC4_ECAP; }
+        s->jobs = nj;
+        s->jcap = nc;
     }
     job_rec *r = &s->jobs[s->njobs];
     memset(r, 0, sizeof *r);
@@ -91,7 +199,7 @@ int ac4_server_submit(ac4_server *s, const ac4_job *job) {
     r->slot = -1;
     /* production steps rounded up to whole chunks, plus the relaxation ramp */
     long prod = ((job->steps + s->chunk - 1) / s->chunk) * s->chunk;
-    r->target = prod + ((s->relax + s->chunk - 1) / s->chunk) * s->chunk;
+    r->target = prod + s->relax;
     return s->njobs++;
 }
 
@@ -103,7 +211,7 @@ static int admit(ac4_server *s, int j, int slot, double now) {
     ac4_replica_params p = {q->n, 1.2, q->T, q->f0, -1, tau, q->seed};
     int rc = ac4_md_assign(s->md, slot, &p);
     if (rc != AC4_OK) return rc;
-    if (r->ckpt) { /* a preempted job continues exactly where it stopped */
+    if (r->ckpt) { /* a preempted job continues exactly where it stopped (its Q(t) origins and sums included) */
         rc = ac4_md_restore(s->md, slot, r->ckpt, r->ckpt_len);
         if (rc != AC4_OK) return rc;
         free(r->ckpt);
@@ -113,9 +221,16 @@ static int admit(ac4_server *s, int j, int slot, double now) {
         s->slot_job[slot] = j;
         return AC4_OK;
     }
+    if (s->ov_on && (rc = ac4_md_overlap_reset(s->md, slot)) != AC4_OK) return rc;
     int n = q->n, nact = (int)((double)n * 0.1000);
     double *buf = malloc(sizeof(double) * 9 * (size_t)n);
     int *ib = malloc(sizeof(int) * (2 * (size_t)n + 3 * (size_t)(nact + 1)));
+    if (!buf || !ib) {
+        free(buf);
+        free(ib);
+        ac4_set_error("admit: out of host memory");
+        return AC4_ECAP;
+    }
     double *x = buf, *y = x + n, *z = y + n, *xu = z + n, *yu = xu + n, *zu = yu + n, *px = zu + n, *py = px + n,
            *pz = py + n;
     int *type = ib, *act = type + n, *kx = act + n, *ky = kx + nact + 1, *kz = ky + nact + 1;
@@ -135,6 +250,27 @@ static int admit(ac4_server *s, int j, int slot, double now) {
     r->slot = slot;
     r->t_start = now;
     s->slot_job[slot] = j;
+    return AC4_OK;
+}
+
+/* A finished job: final x_u in reference order and its Q(t) sums. */
+static int retire(ac4_server *s, int k, job_rec *r) {
+    const int n = r->job.n;
+    r->result = malloc(sizeof(double) * 3 * (size_t)n);
+    if (!r->result) { ac4_set_error("retire: out of host memory"); return AC4_ECAP; }
+    int rc = ac4_md_get_state(s->md, k, NULL, NULL, NULL, r->result, r->result + n, r->result + 2 * n, NULL, NULL,
+                              NULL, NULL, NULL, NULL);
+    if (rc != AC4_OK) return rc;
+    if (s->ov_on) {
+        const int nl = s->ov_nl;
+        r->ov = malloc(sizeof(double) * 3 * (size_t)nl);
+        int *cnt = malloc(sizeof(int) * (size_t)nl);
+        if (!r->ov || !cnt) { free(cnt); ac4_set_error("retire: out of host memory"); return AC4_ECAP; }
+        rc = ac4_md_overlap_get(s->md, k, r->ov, r->ov + nl, cnt);
+        for (int l = 0; l < nl; l++) r->ov[2 * nl + l] = (double)cnt[l];
+        free(cnt);
+        if (rc != AC4_OK) return rc;
+    }
     return AC4_OK;
 }
 
@@ -167,21 +303,26 @@ int ac4_server_tick(ac4_server *s, double now) {
     if (!running) return 0;
     int rc = ac4_md_step_graph(s->md, s->chunk);
     if (rc != AC4_OK) return rc;
-    /* frame of every running job after the chunk (chunk = frame interval) */
-    if ((rc = ac4_md_export_all(s->md, s->hx, s->hy, s->hz)) != AC4_OK) return rc;
+    /* every running job's frame of this chunk, counted on the device */
+    if (s->ov_on && (rc = ac4_md_overlap_sample(s->md)) != AC4_OK) return rc;
+    if ((rc = ac4_md_errors(s->md, s->err)) != AC4_OK) return rc;
     for (int k = 0; k < s->nslots; k++) {
         int j = s->slot_job[k];
         if (j < 0) continue;
         job_rec *r = &s->jobs[j];
+        if (s->err[k]) { /* this job alone fails; its slot is cleared for the next one */
+            r->state = 3;
+            r->err = s->err[k];
+            r->t_done = now;
+            s->slot_job[k] = -1;
+            if ((rc = ac4_md_release(s->md, k)) != AC4_OK) return rc;
+            continue;
+        }
         r->steps_done += s->chunk;
         r->frames++;
         s->particle_steps += (long long)r->job.n * s->chunk;
         if (r->steps_done >= r->target) {
-            int off = ac4_md_slot_offset(s->md, k), n = r->job.n;
-            r->result = malloc(sizeof(double) * 3 * (size_t)n);
-            memcpy(r->result, s->hx + off, sizeof(double) * n);
-            memcpy(r->result + n, s->hy + off, sizeof(double) * n);
-            memcpy(r->result + 2 * n, s->hz + off, sizeof(double) * n);
+            if ((rc = retire(s, k, r)) != AC4_OK) return rc;
             r->state = 2;
             r->t_done = now;
             s->slot_job[k] = -1;
@@ -212,7 +353,7 @@ int ac4_server_preempt(ac4_server *s, int id) {
 
 int ac4_server_pending(const ac4_server *s) {
     int p = 0;
-    for (int j = 0; j < s->njobs; j++) p += s->jobs[j].state != 2;
+    for (int j = 0; j < s->njobs; j++) p += s->jobs[j].state < 2;
     return p;
 }
 
@@ -228,6 +369,11 @@ int ac4_server_job(const ac4_server *s, int id, double *t_start, double *t_done,
     return AC4_OK;
 }
 
+int ac4_server_job_error(const ac4_server *s, int id) {
+    if (!s || id < 0 || id >= s->njobs) { ac4_set_error("job_error: bad id"); return AC4_EARG; }
+    return s->jobs[id].err;
+}
+
 int ac4_server_result(const ac4_server *s, int id, double *xu, double *yu, double *zu) {
     if (!s || id < 0 || id >= s->njobs || s->jobs[id].state != 2) {
         ac4_set_error("result: job %d not finished", id);
@@ -241,4 +387,26 @@ int ac4_server_result(const ac4_server *s, int id, double *xu, double *yu, doubl
     return AC4_OK;
 }
 
+int ac4_server_job_overlap(const ac4_server *s, int id, double *sq, double *sq2, int *cnt) {
+    if (!s || id < 0 || id >= s->njobs || s->jobs[id].state != 2 || !s->ov_on) {
+        ac4_set_error("job_overlap: job %d not finished or the server keeps no overlap", id);
+        return AC4_ESTATE;
+    }
+    const job_rec *r = &s->jobs[id];
+    const int nl = s->ov_nl;
+    for (int l = 0; l < nl; l++) {
+        if (sq) sq[l] = r->ov[l];
+        if (sq2) sq2[l] = r->ov[nl + l];
+        if (cnt) cnt[l] = (int)r->ov[2 * nl + l];
+    }
+    return AC4_OK;
+}
+
 long long ac4_server_particle_steps(const ac4_server *s) { return s ? s->particle_steps : 0; }
+
+ac4_md *ac4_server_handle(ac4_server *s) { return s ? s->md : NULL; }
+
+int ac4_server_job_slot(const ac4_server *s, int id) {
+    if (!s || id < 0 || id >= s->njobs) { ac4_set_error("job_slot: bad id"); return AC4_EARG; }
+    return s->jobs[id].slot;
+}

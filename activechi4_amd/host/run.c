@@ -69,11 +69,14 @@ static double gauss(uint64_t *s) { /* xorshift64* + Box-Muller for p_eta draws *
     return sqrt(-2.0 * log(1.0 - u[0])) * cos(2.0 * 3.14159265358979 * u[1]);
 }
 
-/* InitialiseNoseHoover, MDcode/time_evolve.c:476-493 */
-static void init_chain(ac4_md *md, int n, double T, uint64_t *s) {
-    double c[9], tau = .20;
-    for (int i = 0; i < 3; i++) { c[6 + i] = T * tau * tau; c[i] = 0.0; }
-    c[6] = (3.0 * (double)n - 3.0) * T * tau * tau;
+/* InitialiseNoseHoover, MDcode/time_evolve.c:476-493: eta = 0, p_eta drawn
+ * with variance T*Q_k.  The masses Q_k are the ones the library gives every
+ * new replica (Q0 = DOF*T*tau^2, Q1 = Q2 = T*tau^2, tau = 0.2); they do not
+ * change during a run. */
+static void init_chain(ac4_md *md, double T, uint64_t *s) {
+    double c[9];
+    CK(ac4_md_get_chain(md, 0, c));
+    for (int i = 0; i < 3; i++) c[i] = 0.0;
     for (int i = 0; i < 3; i++) c[3 + i] = sqrt(T * c[6 + i]) * gauss(s);
     CK(ac4_md_set_chain(md, 0, c));
 }
@@ -337,7 +340,7 @@ int main(int argc, char **argv) {
     if (!restart_run) { /* equilibrate, MDcode/main.c:123-206 */
         fix_drift(&S);
         push(md, &S);
-        init_chain(md, n, T, &gs);
+        init_chain(md, T, &gs);
         long steps = (long)(eqDuration / TIME_STEP);
         snprintf(path, sizeof path, "%s/ActiveDirection.dat", sub_dir);
         { /* the reference's 1000 timing steps before equilibration, MDcode/main.c:171-177 */
@@ -385,7 +388,7 @@ int main(int argc, char **argv) {
     } else {
         fix_drift(&S);
         push(md, &S);
-        init_chain(md, n, T, &gs);
+        init_chain(md, T, &gs);
     }
     snprintf(path, sizeof path, "%s/ActiveDirectionRun.dat", sub_dir);
     long steps = (long)(prodDuration / TIME_STEP);

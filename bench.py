@@ -7,20 +7,25 @@ f0 in {0, 0.5, 1, 1.5, 2} x tau_p in {0.1, 1, 10} (tau_per 20/200/2000 steps),
 pair forces in FP32 with FP64 positions/momenta/thermostat, all replicas packed
 into one launch sequence per step and replayed as a hipGraph, kernel shape
 chosen by the autotuner (ac4_autotune), 5e4 steps with the overlap Q(t)
-(a = 0.3) of every replica accumulated on the device every 100 steps, chi4(t)
-from those accumulators.  A "step" is one MD step of every replica;
-value = sum_r N_r x steps / seconds of the timed region.
+(a = 0.3) of every replica accumulated on the device every 100 steps.  A
+"step" is one MD step of every replica; value = sum_r N_r x steps / seconds of
+the timed region.
 
 Beside it:
+- chi4(t) of every (f0, tau_p) point from an untimed fixed-length run of the
+  same ensemble (5e4 steps), once with FP32 pair forces and once in FP64: the
+  chi4 peak height and time of each point with jackknife errors over its
+  replicas, and their FP32-vs-FP64 deviation;
 - the plain single run: the same 64 replicas run one after another, FP64
   everywhere, eager launches (one launch sequence per step);
 - a serving stream: jobs with N logU[500, 20000], T U[0.45, 0.8], f0 U[0, 2],
   tau_p logU[0.1, 10], steps U[1e4, 1e5] arriving as a seeded Poisson process,
-  frames every 100 steps, p50/p99 time to completion, against the same jobs
-  run one after another with the plain kernel;
+  frames every 100 steps counted into each job's Q(t) on the device, slots
+  planned from an HBM budget, p50/p99 time to completion, against the same
+  jobs run one after another with the plain kernel (each one measured);
 - per-kernel times (HIP events on the library's stream), the roofline of the
   force kernel, and the oracle (the CPU restatement) on a bounded sample on
-  one host core.
+  the host's cores.
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--replicas R] [--serve-jobs J]
 """
@@ -88,44 +93,60 @@ def log(msg):
     print("bench: " + msg, file=sys.stderr, flush=True)
 
 
-def chi4_report(md, ps, sample_every):
-    """chi4(t) = N (<Q^2> - <Q>^2) from the device accumulators (lags >= 1;
-    Q(0) = 1): per replica over its time origins (Analysis/overlap.c:116-118
-    times N), averaged over replicas; and for every (f0, tau_p) point of the
-    grid pooled over its replicas and their origins (ensemble fluctuation)."""
-    acc = [md.overlap(r) for r in range(len(ps))]
-    # lags >= 1 that every replica has reached from at least one origin
-    nl = 1
-    while nl < len(acc[0][2]) and all(a[2][nl] > 0 for a in acc):
-        nl += 1
-    acc = [tuple(v[:nl] for v in a) for a in acc]
-    if nl < 3:
-        return None
-    lags = np.arange(1, nl) * sample_every * 0.005
-    per = []
-    for r, (q, q2, c) in enumerate(acc):
-        mq, mq2 = q[1:] / c[1:], q2[1:] / c[1:]
-        per.append(ps[r].n * (mq2 - mq * mq))
-    mean_per = np.mean(per, axis=0)
+CHI4_STEPS = 50000                      # the batched ensemble's run length
+CHI4_OV = (100, 1000, 26, 250, 0.3)     # sample every, origin every, origins kept, lags, a
+
+
+def chi4_leg(ac4, ps, precision, relax, steps=CHI4_STEPS):
+    """chi4(t) of every (f0, tau_p) point of the ensemble from a fixed-length
+    run (untimed): overlap Q(t) of every replica accumulated on the device
+    every 100 steps from origins every 1000 steps, lags up to 2.5e4 steps;
+    chi4 = Nbar (<Q^2> - <Q>^2) pooled over the point's replicas and origins
+    (FP64 on the device), its peak and jackknife errors over the replicas."""
+    se, oe, no, nl, a = CHI4_OV
+    md = start(ac4, ps, precision, relax)
+    md.overlap_enable(se, oe, no, nl, a)
+    t0 = time.perf_counter()
+    md.overlap_sample()  # step 0: the first origin
+    for _ in range(steps // se):
+        md.step(se, graph=True)
+        md.overlap_sample()
+    md.sync()
+    wall = time.perf_counter() - t0
     groups = {}
     for r, p in enumerate(ps):
         groups.setdefault((p.f0, p.tau_per), []).append(r)
-    by_param = []
-    for (f0, tau), rs in sorted(groups.items()):
-        q = sum(acc[r][0][1:] for r in rs)
-        q2 = sum(acc[r][1][1:] for r in rs)
-        c = sum(acc[r][2][1:] for r in rs)
-        nbar = float(np.mean([ps[r].n for r in rs]))
-        chi = nbar * (q2 / c - (q / c) ** 2)
-        k = int(np.argmax(chi))
-        by_param.append({"f0": f0, "tau_p": tau * 0.005, "replicas": len(rs), "chi4_peak": float(chi[k]),
-                         "peak_time": float(lags[k]), "Q_at_peak": float(q[k] / c[k])})
-    qall = sum(a[0][1:] for a in acc) / sum(a[2][1:] for a in acc)
-    k = int(np.argmax(mean_per))
-    pick = [0, len(lags) // 4, len(lags) // 2, len(lags) - 1]
-    return {"a": 0.3, "lag_times": [float(lags[i]) for i in pick], "Q_mean_at_those_lags": [float(qall[i]) for i in pick],
-            "chi4_origins_peak": float(mean_per[k]), "chi4_origins_peak_time": float(lags[k]),
-            "chi4_ensemble_by_param": by_param, "max_lag_time": float(lags[-1])}
+    res = {key: ac4.chi4_peak(md, rs, se * 0.005) for key, rs in sorted(groups.items())}
+    md.close()
+    return res, groups, wall
+
+
+def chi4_compare(ac4, ps, relax):
+    """The batched FP32-force ensemble against the same replicas in FP64 (the
+    plain form: a replica's results do not depend on the batch): chi4 peak
+    height and time per (f0, tau_p) point, with jackknife errors."""
+    r32, groups, w32 = chi4_leg(ac4, ps, ac4.FP32, relax)
+    r64, _, w64 = chi4_leg(ac4, ps, ac4.FP64, relax)
+    rows = []
+    for key in sorted(groups):
+        a, b = r32[key], r64[key]
+        row = {"f0": key[0], "tau_p": key[1] * 0.005, "replicas": len(groups[key])}
+        for tag, v in (("fp32", a), ("fp64", b)):
+            row[tag] = {"peak": v["peak"], "peak_err": v.get("peak_err"), "peak_time": v["peak_time"],
+                        "peak_time_err": v.get("peak_time_err"), "Q_at_peak": v["Q_at_peak"]}
+        row["peak_rel_dev"] = (a["peak"] - b["peak"]) / b["peak"]
+        if a.get("peak_err") is not None and b.get("peak_err") is not None:
+            row["peak_rel_sigma"] = math.hypot(a["peak_err"], b["peak_err"]) / b["peak"]
+        row["peak_time_dev"] = a["peak_time"] - b["peak_time"]
+        row["resolved"] = b["peak_lag"] < CHI4_OV[3]  # the peak lies inside the lag window
+        rows.append(row)
+    dev = [abs(r["peak_rel_dev"]) for r in rows]
+    zs = [abs(r["peak_rel_dev"]) / r["peak_rel_sigma"] for r in rows if r.get("peak_rel_sigma")]
+    return {"a": CHI4_OV[4], "steps": CHI4_STEPS, "sample_every": CHI4_OV[0], "origin_every": CHI4_OV[1],
+            "max_lag_time": CHI4_OV[0] * CHI4_OV[3] * 0.005, "wall_s": {"fp32": w32, "fp64": w64},
+            "by_param": rows, "max_abs_peak_rel_dev": max(dev), "median_abs_peak_rel_dev": float(np.median(dev)),
+            "points_within_2pct": int(sum(d <= 0.02 for d in dev)),
+            "points_within_2sigma": int(sum(z <= 2.0 for z in zs)), "points": len(rows)}
 
 
 def plain_sequential(ac4, ps, relax, steps):
@@ -145,21 +166,13 @@ def plain_sequential(ac4, ps, relax, steps):
     return tot_ps / tot_t, tot_t / steps * 1e3
 
 
-def plain_rate(ac4, n, T, f0, tau, seed, relax=150, steps=200):
-    default_shape(ac4)
-    md = start(ac4, [ac4.ReplicaParams(n, 1.2, T, f0, -1, tau, seed)], ac4.FP64, relax)
-    md.step(30)
-    md.sync()
-    t0 = time.perf_counter()
-    md.step(steps)
-    md.sync()
-    dt = time.perf_counter() - t0
-    md.close()
-    return n * steps / dt
+SERVE_BUDGET = 512e6          # HBM bytes the server may use for slots
+SERVE_OVERLAP = (1000, 10, 100, 0.3)  # per-job Q(t): origins every 1000 steps, 10 kept, lags to 1e4 steps
 
 
-def serving(ac4, njobs, rate_per_s, seed=4242):
-    """Seeded Poisson stream of MD jobs through the continuous-batching server."""
+def serving_jobs(njobs, rate_per_s, seed=4242):
+    """The serving stream's requests: N logU[500, 20000], T U[0.45, 0.8],
+    f0 U[0, 2], tau_p logU[0.1, 10], steps U[1e4, 1e5], Poisson arrivals."""
     rng = np.random.default_rng(seed)
     jobs = []
     t = 0.0
@@ -169,12 +182,41 @@ def serving(ac4, njobs, rate_per_s, seed=4242):
         jobs.append(dict(n=n, T=float(rng.uniform(0.45, 0.8)), f0=float(rng.uniform(0.0, 2.0)),
                          tau_p=float(math.exp(rng.uniform(math.log(0.1), math.log(10.0)))),
                          steps=int(rng.integers(10000, 100001)), seed=1000 + k, t=t))
-    # slot classes sized from an HBM budget: a class's slot count follows the
-    # share of jobs it takes under the N distribution (about 1.2 KB per
-    # particle of capacity incl. the 192-entry neighbour list)
-    caps = [1024] * 8 + [2048] * 10 + [4096] * 10 + [8192] * 8 + [20480] * 4
+    return jobs
+
+
+def plain_job(ac4, j, relax=300):
+    """One job with the plain kernel (FP64, eager launches): seeded lattice,
+    the relaxation ramp, then its production steps; seconds of GPU work."""
     default_shape(ac4)
-    srv = ac4.Server(caps, precision=ac4.FP32, chunk=100, relax=300)
+    tau = max(1, int(round(j["tau_p"] / 0.005)))
+    md = ac4.MD([ac4.ReplicaParams(j["n"], 1.2, j["T"], j["f0"], -1, tau, j["seed"])], precision=ac4.FP64)
+    load_states(ac4, md, [ac4.ReplicaParams(j["n"], 1.2, j["T"], j["f0"], -1, tau, j["seed"])])
+    md.sync()
+    t0 = time.perf_counter()
+    md.relax(relax, 1.5, 1.0)
+    md.init_chain(0, seed=j["seed"])
+    left = ((j["steps"] + 99) // 100) * 100
+    while left > 0:
+        k = min(left, 5000)
+        md.step(k)
+        left -= k
+    md.sync()
+    dt = time.perf_counter() - t0
+    md.close()
+    return dt
+
+
+def serving(ac4, njobs, rate_per_s, seed=4242):
+    """Seeded Poisson stream of MD jobs through the continuous-batching
+    server (slots planned from an HBM budget, every job's Q(t) kept on the
+    device), against the same jobs run one after another with the plain
+    kernel, measured job by job."""
+    jobs = serving_jobs(njobs, rate_per_s, seed)
+    default_shape(ac4)
+    srv = ac4.Server(precision=ac4.FP32, chunk=100, relax=300, budget=SERVE_BUDGET, n_range=(500, 20000),
+                     overlap=SERVE_OVERLAP)
+    caps = srv.slots()
     ids = [srv.submit(j["n"], j["T"], j["f0"], j["tau_p"], j["steps"], j["seed"], j["t"]) for j in jobs]
     t0 = time.perf_counter()
     while srv.pending():
@@ -184,40 +226,56 @@ def serving(ac4, njobs, rate_per_s, seed=4242):
     lat = []
     for j, i in zip(jobs, ids):
         st = srv.job(i)
-        assert st["state"] == 2
+        assert st["state"] == 2, (i, st)
         lat.append(st["t_done"] - j["t"])
+    q1 = [srv.job_overlap(i) for i in ids[:1]]
     srv.close()
     work = sum(j["n"] * j["steps"] for j in jobs)
     serve_rate = work / (wall - jobs[0]["t"])
-    # the same jobs run one after another with the plain kernel (rate measured
-    # per job on a short run, extrapolated to the job's steps)
-    t_plain = [j["n"] * j["steps"] / plain_rate(ac4, j["n"], j["T"], j["f0"],
-                                                 max(1, int(round(j["tau_p"] / 0.005))), j["seed"])
-               for j in jobs]
+    log("serving: %d jobs in %.1f s on %d slots; plain sequential next" % (njobs, wall, len(caps)))
+    t_plain = [plain_job(ac4, j) for j in jobs]
     t_seq = sum(t_plain)
     lat = np.array(lat)
     # the same arrivals served first-come first-served, one job at a time,
-    # with the plain kernel
+    # with the plain kernel (each job's measured time)
     free, seq_lat = 0.0, []
     for j, tj in zip(jobs, t_plain):
         free = max(free, j["t"]) + tj
         seq_lat.append(free - j["t"])
-    return {"jobs": njobs, "arrival_rate_per_s": rate_per_s, "slots": len(caps),
-            "slot_capacity_particles": int(sum(caps)), "p50_latency_s": float(np.percentile(lat, 50)),
+    qq, _, qc = q1[0]
+    return {"jobs": njobs, "arrival_rate_per_s": rate_per_s, "hbm_budget_bytes": SERVE_BUDGET,
+            "slots": len(caps), "slot_classes": {str(c): int((caps == c).sum()) for c in np.unique(caps)},
+            "slot_capacity_particles": int(caps.sum()), "p50_latency_s": float(np.percentile(lat, 50)),
             "p99_latency_s": float(np.percentile(lat, 99)), "value": serve_rate, "unit": "particle-steps/s",
             "plain_sequential_value": work / t_seq, "plain_sequential_makespan_s": t_seq,
+            "plain_sequential_measured": True,
             "plain_sequential_p50_latency_s": float(np.percentile(seq_lat, 50)),
             "plain_sequential_p99_latency_s": float(np.percentile(seq_lat, 99)),
             "speedup_vs_plain_sequential": serve_rate * t_seq / work, "wall_s": wall,
-            "production_particle_steps": work}
+            "production_particle_steps": work,
+            "job0_Q_at_lag_1e4_steps": float(qq[-1] / qc[-1]) if qc[-1] else None}
 
 
-def cpu_baseline(ac4, n=2485, steps=1500):
-    """Oracle (single-threaded C restatement of the reference step) on one core,
-    from a state brought to temperature on the GPU (the oracle has no ramp)."""
+def _oracle_worker(args):
+    """One host core: the oracle's NHC step on one replica (spawned process)."""
+    s, L, chain, steps = args
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as OL
-    lib = OL.load()
+    t0 = time.perf_counter()
+    _, _, out = OL.nhc_steps(s, L, 0.5, 1.0, steps, chain)
+    return time.perf_counter() - t0, out
+
+
+def cpu_baseline(ac4, n=2485, steps=2000, max_workers=16):
+    """The oracle (single-threaded C restatement of the reference step) on
+    every host core the box gives this GPU (at most 16): one replica per
+    core, in spawned processes, from a state brought to temperature on the
+    GPU (the oracle has no ramp).  Replicas are independent, so the host's
+    rate is the sum over cores."""
+    import multiprocessing as mp
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as OL
+    L = OL.load().or_md_box_length(n, 1.2)
     p = ac4.ReplicaParams(n, 1.2, 0.5, 1.0, -1, 0, 99)
     default_shape(ac4)
     md = start(ac4, [p], ac4.FP64, 300)
@@ -226,13 +284,23 @@ def cpu_baseline(ac4, n=2485, steps=1500):
     md.close()
     s = ac4.init_lattice(n, 0.5, 99)
     s.update({k: g[k] for k in ("x", "y", "z", "xu", "yu", "zu", "px", "py", "pz")})
-    t0 = time.perf_counter()
-    _, _, out = OL.nhc_steps(s, lib.or_md_box_length(n, 1.2), 0.5, 1.0, steps, chain)
-    dt = time.perf_counter() - t0
-    assert np.isfinite(out[4]) and 0.2 < out[3] < 1.0, out
-    return {"value": n * steps / dt, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-            "sample": "oracle NHC step (FP64, half Verlet list, f0=1.0) on one N=%d replica, %d steps, %.1f s"
-                      % (n, steps, dt)}
+    nproc = os.cpu_count() or 1
+    try:
+        nproc = len(os.sched_getaffinity(0))
+    except AttributeError:
+        pass
+    w = max(1, min(nproc, max_workers))
+    with mp.get_context("spawn").Pool(w) as pool:
+        pool.map(_oracle_worker, [(s, L, chain, 5)] * w)  # interpreter and library start-up, untimed
+        t0 = time.perf_counter()
+        res = pool.map(_oracle_worker, [(s, L, chain, steps)] * w)
+        wall = time.perf_counter() - t0
+    for _, out in res:
+        assert np.isfinite(out[4]) and 0.2 < out[3] < 1.0, out
+    return {"value": w * n * steps / wall, "unit": "particle-steps/s", "cores": w, "host_cpus": nproc,
+            "kind": "port", "per_core_value": float(np.mean([n * steps / t for t, _ in res])),
+            "sample": "oracle NHC step (FP64, half Verlet list, f0=1.0): one N=%d replica per core on %d cores, "
+                      "%d steps each, %.1f s wall" % (n, w, steps, wall)}
 
 
 def cfg5(ac4, n=1_000_000, steps=20000, nframes=500, relax=500, ndir=64, seed=5150):
@@ -548,6 +616,7 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=2.0, help="job arrivals per second")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-chi4", action="store_true", help="skip the untimed fixed-length chi4 leg")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="short eager run + 256 MiB copies, for rocprofv3 --pmc (prints no bench line)")
     ap.add_argument("--cfg5", action="store_true",
@@ -582,8 +651,8 @@ def main():
                 "candidates": [[int(t), int(m), float(sk), float(v)] for t, m, sk, v in table]}
         log("autotune: tpp=%d nlist_mode=%d skin=%.2f" % best)
     md = start(ac4, ps, ac4.FP32, a.relax)
-    sample_every = 100
-    md.overlap_enable(sample_every, 1000, 21, 200, 0.3)
+    sample_every = CHI4_OV[0]
+    md.overlap_enable(*CHI4_OV)
     log("%d replicas, %d particles; relaxed %d steps, warmup %d steps" % (a.replicas, ntot, a.relax, a.warmup))
     md.step(a.warmup, graph=True)  # chain steps before the first origin
     md.sync()
@@ -606,7 +675,6 @@ def main():
     t_batched = time.perf_counter() - t0
     value = ntot * a.steps / t_batched
     log("batched %.3f s for %d steps" % (t_batched, a.steps))
-    chi4 = chi4_report(md, ps, sample_every)
     kt = md.kernel_times(200)
     rebuilds = sum(md.rebuilds(r) for r in range(a.replicas))
     md.close()
@@ -615,6 +683,11 @@ def main():
         print(json.dumps({"metric": "particle-steps/s (batched only)", "value": value, "unit": "particle-steps/s",
                           "steps": a.steps, "kernel_ms_per_step": kt}), flush=True)
         return
+    chi4 = None
+    if not a.no_chi4:
+        chi4 = chi4_compare(ac4, ps, a.relax)
+        log("chi4: max |peak dev| FP32 vs FP64 %.3f, %d/%d points within 2 sigma"
+            % (chi4["max_abs_peak_rel_dev"], chi4["points_within_2sigma"], chi4["points"]))
     plain_value, plain_ms = plain_sequential(ac4, ps, a.relax, a.plain_steps)
     log("plain sequential %.3e particle-steps/s" % plain_value)
     serve = None

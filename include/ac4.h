@@ -277,6 +277,16 @@ int ac4_md_load_slot(ac4_md *md, int r, const double *x, const double *y,
                      const int *kx, const int *ky, const int *kz);
 /* ac4_md_relax for one slot, applied during its next nsteps steps. */
 int ac4_md_relax_slot(ac4_md *md, int r, int nsteps, double f_start, double f_end);
+/* Device bytes one slot of capacity cap_n adds to a handle (with in-situ
+ * overlap of n_origin origins and n_lags lags; 0, 0 without). */
+long long ac4_md_slot_bytes(int cap_n, int n_origin, int n_lags);
+/* Error bits of every slot: err[r] bit 0 list or cell over capacity, bit 1
+ * non-finite or runaway coordinate (0 for idle slots). */
+int ac4_md_errors(ac4_md *md, int *err);
+/* Sets error bits on replica r as the kernels do on an overflow or a
+ * runaway coordinate, with none of those happening (tests of the callers'
+ * failure handling). */
+int ac4_md_raise_error(ac4_md *md, int r, int bits);
 /* x_u of every slot in reference order (arrays of ac4_md_total_particles). */
 int ac4_md_export_all(ac4_md *md, double *xu, double *yu, double *zu);
 
@@ -291,6 +301,17 @@ int ac4_md_overlap_enable(ac4_md *md, int sample_every, int origin_every, int n_
 int ac4_md_overlap_sample(ac4_md *md);
 /* sq[l] = sum over origins of Q, sq2[l] of Q^2, cnt[l] origins (n_lags+1). */
 int ac4_md_overlap_get(ac4_md *md, int r, double *sq, double *sq2, int *cnt);
+/* Ensemble chi4 on the device in FP64, pooled over the nlist replicas of
+ * `list` (summed in list order) and their origins: for l = 0..n_lags,
+ * out3[3l] = Nbar*(<Q^2> - <Q>^2), out3[3l+1] = <Q>, out3[3l+2] = origins,
+ * with <Q^2> - <Q>^2 formed as Analysis/overlap.c:114-118 forms X4 (which
+ * has no factor N) and Nbar the mean N of the listed replicas.  chi4 as an
+ * ensemble fluctuation over independent replicas has no reference
+ * counterpart; the reference's commented-out ensemble mean is
+ * Analysis/avg_and_print.c:30-41. */
+int ac4_md_overlap_chi4(ac4_md *md, int nlist, const int *list, double *out3);
+/* Clears replica r's accumulators and origins (a slot starting a new job). */
+int ac4_md_overlap_reset(ac4_md *md, int r);
 
 /* Request loop (continuous batching of MD jobs on one GPU). */
 typedef struct {
@@ -302,17 +323,37 @@ typedef struct {
     double t_arrive; /* arrival time (seconds, caller's clock) */
 } ac4_job;
 typedef struct ac4_server ac4_server;
-/* Slots of the given capacities (the caller sizes them from an HBM budget);
- * chunk = steps between frames; relax_steps = velocity-rescale ramp before
- * each job's production steps. */
+/* Slots of the given capacities; chunk = steps between frames;
+ * relax_steps = velocity-rescale ramp before each job's production steps
+ * (rounded up to whole chunks). */
 int ac4_server_create(int nslots, const int *slot_cap, int precision, int chunk,
                       int relax_steps, ac4_server **out);
+/* Slot capacities from an HBM byte budget for requests with N in
+ * [n_min, n_max]: classes a factor 2 apart, slot counts per class in
+ * proportion to the share of a log-uniform N each class takes (at least one),
+ * as many as hbm_bytes holds at ac4_md_slot_bytes(cap, n_origin, n_lags) per
+ * slot.  Writes at most caps_cap capacities; returns the slot count. */
+int ac4_server_plan(double hbm_bytes, int n_min, int n_max, int n_origin, int n_lags,
+                    int *caps, int caps_cap);
+/* ac4_server_plan + ac4_server_create (+ ac4_server_enable_overlap when
+ * n_origin > 0). */
+int ac4_server_create_budget(double hbm_bytes, int n_min, int n_max, int precision,
+                             int chunk, int relax_steps, int origin_every, int n_origin,
+                             int n_lags, double a, ac4_server **out);
+/* Every job's overlap Q(t) accumulated on the device from its frames (one
+ * per chunk; ac4_md_overlap_enable with sample_every = chunk), from its own
+ * step counter; call before the first submit. */
+int ac4_server_enable_overlap(ac4_server *s, int origin_every, int n_origin, int n_lags,
+                              double a);
+/* Slot count; caps (may be NULL) receives the capacities. */
+int ac4_server_slots(const ac4_server *s, int *caps);
 int ac4_server_destroy(ac4_server *s);
 /* Queues a job; returns its id (>= 0) or an error. */
 int ac4_server_submit(ac4_server *s, const ac4_job *job);
 /* Admits arrived jobs (t_arrive <= now) into free slots, advances all slots
- * one chunk, copies each running job's frame, retires finished jobs.
- * Returns the number of jobs that ran in the chunk. */
+ * one chunk, counts each running job's frame into its Q(t) accumulators,
+ * fails the jobs whose replica raised an error flag (the others go on) and
+ * retires finished jobs.  Returns the number of jobs that ran in the chunk. */
 int ac4_server_tick(ac4_server *s, double now);
 int ac4_server_pending(const ac4_server *s);
 /* Preemption (no reference counterpart): takes running job `id` off the GPU
@@ -320,12 +361,20 @@ int ac4_server_pending(const ac4_server *s);
  * its slot; the job is queued again behind the jobs that have not started
  * and, when admitted, continues bitwise as if it had never been interrupted. */
 int ac4_server_preempt(ac4_server *s, int id);
-/* state: 0 queued, 1 running, 2 done. */
+/* state: 0 queued, 1 running, 2 done, 3 failed (ac4_server_job_error gives
+ * its error bits: 1 list or cell over capacity, 2 runaway coordinate). */
 int ac4_server_job(const ac4_server *s, int id, double *t_start, double *t_done,
                    long *steps_done, int *frames, int *state);
+int ac4_server_job_error(const ac4_server *s, int id);
 /* Final unwrapped positions (reference order) of a finished job. */
 int ac4_server_result(const ac4_server *s, int id, double *xu, double *yu, double *zu);
+/* Per-lag overlap sums of a finished job (as ac4_md_overlap_get). */
+int ac4_server_job_overlap(const ac4_server *s, int id, double *sq, double *sq2, int *cnt);
 long long ac4_server_particle_steps(const ac4_server *s);
+/* The server's MD handle (kernel timings, error injection in tests) and the
+ * slot a running job occupies (-1 when it is not running). */
+ac4_md *ac4_server_handle(ac4_server *s);
+int ac4_server_job_slot(const ac4_server *s, int id);
 
 /* ------------------------------------------------------------ Analysis --
  * Streaming per-frame analysis over all time origins (Analysis/main.c:53-62).
@@ -423,10 +472,6 @@ void ac4_chain_init(int n, double T, uint64_t seed, double *chain9);
  * NULL. */
 int ac4_autotune(int n_rep, const ac4_replica_params *p, int precision, int relax_steps, int steps,
                  double *best3, double *table, int table_cap, int *n_cand);
-/* Ensemble chi4 over replicas (no reference counterpart; chi4 as an
- * ensemble fluctuation over independent replicas): q[r*nor+o]; out2 = {pooled Nbar*(<Q^2>-<Q>^2), mean_r N_r*Var_o Q_r}. */
-int ac4_chi4_ensemble(int nrep, int nor, const double *q, const int *n_r,
-                      double *out2);
 
 #ifdef __cplusplus
 }

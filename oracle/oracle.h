@@ -28,9 +28,12 @@
  *   or_getq, or_write_* ..................... byte-identical output files vs
  *                                             the reference Analysis program
  *                                             (golden fixtures) + closed forms
- *   or_chi4_ensemble ........................ closed forms (replica
- *                                             permutation invariance, zero
- *                                             variance for identical replicas)
+ *   or_chi4_ensemble ........................ closed forms from per-origin
+ *                                             samples (numpy population
+ *                                             variance x Nbar, the
+ *                                             reference's one-run X4 for one
+ *                                             replica, identical replicas,
+ *                                             zero for constant samples)
  */
 #ifndef AC4_ORACLE_H
 #define AC4_ORACLE_H
@@ -180,11 +183,14 @@ int or_write_fsqt(const char *path, double totpart, int nslots,
 int or_analyse_dir(const char *sub_dir, int N, int do_msd, int do_overlap,
                    int do_s4, int do_fs);
 
-/* New capability (no reference counterpart): ensemble chi4 from per-replica
- * per-origin Q samples.  q[r*nor+o], n_r[r].  out2 = {pooled, mean_of_r}:
- * pooled = Nbar*(<Q^2>-<Q>^2) over all (r,o); mean_of_r = mean_r N_r*Var_o(Q_r). */
-void or_chi4_ensemble(int nrep, int nor, const double *q, const int *n_r,
-                      double *out2);
+/* Ensemble chi4 (no reference counterpart) from the per-lag sums each
+ * replica accumulates, in the form Analysis/overlap.c:89-94,114-118 keeps
+ * them: sq[r*nlag+l] = sum over origins of Q, sq2 of Q^2, cnt the origins.
+ * Pooled over the replicas (summed in index order) and their origins:
+ * out3[3l] = Nbar*(s2/c - s*s/(c*c)), out3[3l+1] = s/c, out3[3l+2] = c, with
+ * Nbar the mean of n_r; zeros where c = 0. */
+void or_chi4_ensemble(int nrep, int nlag, const double *sq, const double *sq2,
+                      const int *cnt, const int *n_r, double *out3);
 /* Analysis_Block/{msd_pos,overlap,vanHove}.c for one block edge cell_cut:
  * writes MSDLb, RelaxLb, RelaxTotLb and vanHove files under
  * sub_dir/BlockAnalysis/ (the directories must exist).  param = a*a. */

@@ -539,22 +539,27 @@ int or_analyse_dir(const char *sub_dir, int N, int do_msd, int do_overlap,
     return rc;
 }
 
-void or_chi4_ensemble(int nrep, int nor, const double *q, const int *n_r, double *out2) {
-    double s = 0.0, s2 = 0.0, nbar = 0.0, mean_r = 0.0;
-    for (int r = 0; r < nrep; r++) {
-        double a = 0.0, a2 = 0.0;
-        for (int o = 0; o < nor; o++) {
-            double v = q[(size_t)r * nor + o];
-            a += v;
-            a2 += v * v;
-        }
-        s += a;
-        s2 += a2;
-        nbar += (double)n_r[r];
-        mean_r += (double)n_r[r] * (a2 / nor - (a / nor) * (a / nor));
-    }
-    double m = (double)nrep * nor;
+void or_chi4_ensemble(int nrep, int nlag, const double *sq, const double *sq2, const int *cnt, const int *n_r,
+                      double *out3) {
+    double nbar = 0.0;
+    for (int r = 0; r < nrep; r++) nbar += (double)n_r[r];
     nbar /= (double)nrep;
-    out2[0] = nbar * (s2 / m - (s / m) * (s / m));
-    out2[1] = mean_r / (double)nrep;
+    for (int l = 0; l < nlag; l++) {
+        double s = 0.0, s2 = 0.0;
+        long long c = 0;
+        for (int r = 0; r < nrep; r++) {
+            s += sq[(size_t)r * nlag + l];
+            s2 += sq2[(size_t)r * nlag + l];
+            c += cnt[(size_t)r * nlag + l];
+        }
+        double q = 0.0, x4 = 0.0;
+        if (c > 0) { /* overlap.c:116-118 */
+            double dc = (double)c;
+            q = s / dc;
+            x4 = s2 / dc - s * s / (dc * dc);
+        }
+        out3[3 * l] = nbar * x4;
+        out3[3 * l + 1] = q;
+        out3[3 * l + 2] = (double)c;
+    }
 }

@@ -54,7 +54,7 @@ SIGS = {
                                ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, D, D, D, D, I, D, D, D]),
     "or_analyse_dir": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                       ctypes.c_int]),
-    "or_chi4_ensemble": (None, [ctypes.c_int, ctypes.c_int, D, I, D]),
+    "or_chi4_ensemble": (None, [ctypes.c_int, ctypes.c_int, D, D, I, I, D]),
     "or_block_scale": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_int, I, ctypes.c_int, D, D, D,
                                       ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                       ctypes.c_char_p]),
@@ -289,3 +289,16 @@ def forces_sample(s, L, f0, idx):
     load().or_forces_sample(len(g["x"]), dp(g["x"]), dp(g["y"]), dp(g["z"]), ip(g["type"]), L, ip(g["act_tag"]),
                             ip(g["kx"]), ip(g["ky"]), ip(g["kz"]), f0, len(idx), ip(idx), dp(out))
     return out
+
+
+def chi4_ensemble(sq, sq2, cnt, n_r):
+    """or_chi4_ensemble over per-replica per-lag sums (arrays [nrep, nlag]);
+    returns (chi4, Q, origins) per lag."""
+    sq, sq2 = (np.ascontiguousarray(a, dtype=np.float64) for a in (sq, sq2))
+    cnt = np.ascontiguousarray(cnt, dtype=np.int32)
+    n_r = np.ascontiguousarray(n_r, dtype=np.int32)
+    nrep, nlag = sq.shape
+    out = np.zeros(3 * nlag)
+    load().or_chi4_ensemble(nrep, nlag, dp(sq), dp(sq2), ip(cnt), ip(n_r), dp(out))
+    out = out.reshape(nlag, 3)
+    return out[:, 0].copy(), out[:, 1].copy(), out[:, 2].astype(np.int64)

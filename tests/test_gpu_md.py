@@ -156,6 +156,20 @@ def _chain0(n, T):
     return np.concatenate([np.zeros(3), np.sqrt(T * q) * np.array([0.4, -0.3, 0.2]), q])
 
 
+def test_fresh_handle_chain_masses_known_answers(gpu):
+    """The chain masses a new replica starts with (InitialiseNoseHoover,
+    time_evolve.c:477-482) against hand-worked values; the trajectory test
+    below passes the oracle's masses in and would not see a mass error."""
+    from test_oracle_md import CHAIN_MASS_KAT
+    cases = [(n, T, want) for n, T, want in CHAIN_MASS_KAT if n >= 500]
+    md = gpu.MD([gpu.ReplicaParams(n, 1.2, T, 1.0, -1, 200, 5 + k) for k, (n, T, _) in enumerate(cases)])
+    for k, (n, T, want) in enumerate(cases):
+        c = md.get_chain(k)
+        assert c[6:9] == pytest.approx(want, rel=1e-14, abs=0)
+        assert np.all(c[0:6] == 0.0)
+    md.close()
+
+
 def test_nhc_trajectory_matches_oracle(gpu):
     n, T, f0 = 1000, 0.6, 1.5
     s, L = lattice_state(n, T=T, jitter=0.02, seed=21)
@@ -242,6 +256,7 @@ def test_checkpoint_restore_is_exact(gpu, prec):
         a = make_relaxed_md(gpu, [(1500, 81, 3), (2200, 82, 4)], tau=20, prec=prec)
         b = make_relaxed_md(gpu, [(2200, 82, 4)], tau=20, prec=prec)
         c = make_relaxed_md(gpu, [(2000, 83, 5)], tau=20, prec=prec)
+        d = make_relaxed_md(gpu, [(2200, 82, 9)], tau=20, prec=prec)  # same shape, another seed
     a.step(150, graph=True)
     blob = a.checkpoint(1)
     a.step(170, graph=True)
@@ -258,6 +273,8 @@ def test_checkpoint_restore_is_exact(gpu, prec):
     assert a.rebuilds(1) == b.rebuilds(0) > 0
     with pytest.raises(gpu.AC4Error):
         c.restore(0, blob)
+    with pytest.raises(gpu.AC4Error, match="seed"):
+        d.restore(0, blob)
 
 
 def test_graph_replay_equals_eager(gpu):
@@ -287,12 +304,19 @@ def test_stats_consistent_with_state(gpu):
     assert st["steps"] == 30
 
 
-def _run_server(ac4, caps, jobs, chunk=100):
-    srv = ac4.Server(caps, precision=ac4.FP32, chunk=chunk, relax=200)
+def _run_server(ac4, caps, jobs, chunk=100, overlap=None, budget=None, fail=None):
+    """Runs the jobs to completion, one tick per time unit; fail = (job
+    index, tick) raises an error flag on that job's replica after the tick."""
+    if budget is not None:
+        srv = ac4.Server(precision=ac4.FP32, chunk=chunk, relax=200, budget=budget, n_range=caps, overlap=overlap)
+    else:
+        srv = ac4.Server(caps, precision=ac4.FP32, chunk=chunk, relax=200, overlap=overlap)
     ids = [srv.submit(*j) for j in jobs]
     t = 0
     while srv.pending() and t < 10000:
         srv.tick(float(t))
+        if fail and t == fail[1]:
+            srv.raise_error(ids[fail[0]], 2)
         t += 1
     return srv, ids
 
@@ -315,16 +339,82 @@ def test_server_job_alone_equals_in_mix(gpu):
     assert mix.particle_steps() > 0
 
 
+def test_server_failed_job_is_isolated(gpu):
+    """A job whose replica raises an error flag (set from the host, as a
+    kernel sets it on a list overflow or a runaway coordinate) fails alone:
+    its slot is cleared and serves a later job, and every other job finishes
+    bitwise as when it runs alone."""
+    caps = [2048, 2048, 4096]
+    ov = (200, 4, 8, 0.3)
+    jobs = [(1500, 0.5, 1.0, 1.0, 800, 31, 0), (1800, 0.6, 0.5, 0.1, 800, 32, 0), (3000, 0.5, 2.0, 10.0, 600, 33, 0),
+            (1900, 0.7, 1.5, 1.0, 500, 34, 6)]
+    mix, ids = _run_server(gpu, caps, jobs, overlap=ov, fail=(1, 3))
+    st = [mix.job(i) for i in ids]
+    assert st[1]["state"] == 3 and mix.job_error(ids[1]) == 2
+    assert st[1]["steps_done"] == 400  # four chunks ran, the fifth saw the flag
+    with pytest.raises(gpu.AC4Error):
+        mix.result(ids[1])
+    assert [st[k]["state"] for k in (0, 2, 3)] == [2, 2, 2]
+    assert st[3]["t_start"] >= 6
+    for k in (0, 2, 3):
+        alone, aid = _run_server(gpu, caps, [jobs[k][:6] + (0,)], overlap=ov)
+        a, m = alone.result(aid[0]), mix.result(ids[k])
+        for u in range(3):
+            assert np.array_equal(a[u], m[u]), (k, u)
+        for u, (va, vm) in enumerate(zip(alone.job_overlap(aid[0]), mix.job_overlap(ids[k]))):
+            assert np.array_equal(va, vm), (k, u)
+        alone.close()
+    mix.close()
+
+
+def test_server_cfg4_stream(gpu):
+    """The serving configuration's request distributions (N logU[500, 20000],
+    T U[0.45, 0.8], f0 U[0, 2], tau_p logU[0.1, 10]; steps cut to 300-1200 and
+    a job below N = 875 added) on slots planned from an HBM budget, with every
+    job's Q(t) kept on the device: all jobs finish, and the smallest and the
+    largest job give bitwise the results (final x_u, Q sums) they give alone."""
+    rng = np.random.default_rng(404)
+    jobs = []
+    for k in range(10):
+        n = int(np.exp(rng.uniform(np.log(500), np.log(20000))))
+        jobs.append((n, float(rng.uniform(0.45, 0.8)), float(rng.uniform(0.0, 2.0)),
+                     float(np.exp(rng.uniform(np.log(0.1), np.log(10.0)))), int(rng.integers(300, 1201)), 500 + k,
+                     float(k // 2)))
+    jobs.append((600, 0.6, 1.2, 1.0, 700, 599, 1.0))
+    ov = (500, 6, 12, 0.3)
+    budget = 160e6
+    caps = gpu.server_plan(budget, 500, 20000, 6, 12)
+    assert sum(gpu.slot_bytes(int(c), 6, 12) for c in caps) <= budget
+    assert min(caps) >= 512 and max(caps) >= 20000
+    mix, ids = _run_server(gpu, (500, 20000), jobs, overlap=ov, budget=budget)
+    assert np.array_equal(mix.slots(), caps)
+    assert all(mix.job(i)["state"] == 2 for i in ids)
+    order = np.argsort([j[0] for j in jobs])
+    for k in (int(order[0]), int(order[-1])):
+        alone, aid = _run_server(gpu, (500, 20000), [jobs[k][:6] + (0.0,)], overlap=ov, budget=budget)
+        a, m = alone.result(aid[0]), mix.result(ids[k])
+        for u in range(3):
+            assert np.array_equal(a[u], m[u]), (k, u)
+        qa, qm = alone.job_overlap(aid[0]), mix.job_overlap(ids[k])
+        for u in range(3):
+            assert np.array_equal(qa[u], qm[u]), (k, u)
+        assert qm[2][1] > 0 and 0.0 < qm[0][1] / qm[2][1] <= 1.0  # Q at the first lag
+        alone.close()
+    mix.close()
+
+
 def test_server_preempted_job_resumes_exactly(gpu):
-    """A job taken off the GPU mid-run (exact checkpoint in host memory),
-    whose slot then serves another job, finishes bitwise as when it runs
-    uninterrupted."""
+    """A job taken off the GPU mid-run (exact checkpoint in host memory, its
+    Q(t) origins and sums included), whose slot then serves another job,
+    finishes bitwise as when it runs uninterrupted."""
     caps = [2048, 4096]
+    ov = (200, 4, 6, 0.3)
     a_job = (1500, 0.5, 1.0, 1.0, 700, 21, 0)
-    alone, aid = _run_server(gpu, caps, [a_job])
+    alone, aid = _run_server(gpu, caps, [a_job], overlap=ov)
     ref = alone.result(aid[0])
+    ref_q = alone.job_overlap(aid[0])
     alone.close()
-    srv = gpu.Server(caps, precision=gpu.FP32, chunk=100, relax=200)
+    srv = gpu.Server(caps, precision=gpu.FP32, chunk=100, relax=200, overlap=ov)
     ia = srv.submit(*a_job)
     srv.tick(0.0)
     srv.tick(1.0)
@@ -341,6 +431,8 @@ def test_server_preempted_job_resumes_exactly(gpu):
     got = srv.result(ia)
     for u in range(3):
         assert np.array_equal(got[u], ref[u]), u
+    for u, (a, b) in enumerate(zip(srv.job_overlap(ia), ref_q)):
+        assert np.array_equal(a, b), u
     srv.close()
 
 

@@ -68,6 +68,17 @@ def test_product_walk_matches_oracle(ac4, oracle):
         assert ac4.origin_gap(f) == gap == oracle.or_origin_gap(OL.ip(f), len(f))
 
 
+@pytest.mark.parametrize("n,T,want", __import__("test_oracle_md").CHAIN_MASS_KAT)
+def test_chain_init_masses_known_answers(ac4, n, T, want):
+    # ac4_chain_init: the masses of InitialiseNoseHoover (time_evolve.c:477-482),
+    # eta = 0, p_eta finite and seed-determined
+    c = ac4.chain_init(n, T, 12345)
+    assert c[6:9] == pytest.approx(want, rel=1e-14, abs=0)
+    assert np.all(c[0:3] == 0.0) and np.all(np.isfinite(c[3:6]))
+    assert np.array_equal(c, ac4.chain_init(n, T, 12345))
+    assert not np.array_equal(c[3:6], ac4.chain_init(n, T, 12346)[3:6])
+
+
 def test_init_lattice_invariants(ac4):
     n, T = 1000, 0.5
     s = ac4.init_lattice(n, T, 42)

@@ -158,18 +158,33 @@ def test_getq_table(oracle):
 
 
 def test_chi4_ensemble_closed_forms(oracle):
+    """or_chi4_ensemble from per-lag sums against the variance of the
+    per-origin samples the sums were made of (numpy), the reference's one-run
+    X4 (overlap.c:116-118, no factor N) for a single replica, and zero for
+    identical samples."""
     rng = np.random.default_rng(5)
-    q = rng.random((6, 20))
+    nrep, nor, nlag = 6, 20, 4
+    q = rng.random((nrep, nor, nlag))
     nr = np.array([1000, 2000, 1500, 1200, 3000, 2500], dtype=np.int32)
-    out = np.zeros(2)
-    oracle.or_chi4_ensemble(6, 20, OL.dp(np.ascontiguousarray(q)), OL.ip(nr), OL.dp(out))
-    assert out[0] == pytest.approx(nr.mean() * q.var(), rel=1e-12)
-    assert out[1] == pytest.approx(np.mean(nr * q.var(1)), rel=1e-12)
-    perm = rng.permutation(6)
-    out2 = np.zeros(2)
-    oracle.or_chi4_ensemble(6, 20, OL.dp(np.ascontiguousarray(q[perm])), OL.ip(np.ascontiguousarray(nr[perm])),
-                            OL.dp(out2))
-    assert np.allclose(out, out2, rtol=1e-12)
-    same = np.repeat(q[:1], 6, 0)
-    oracle.or_chi4_ensemble(6, 20, OL.dp(np.ascontiguousarray(same)), OL.ip(nr), OL.dp(out))
-    assert out[1] >= 0
+    cnt = np.full((nrep, nlag), nor, dtype=np.int32)
+    chi, qm, c = OL.chi4_ensemble(q.sum(1), (q * q).sum(1), cnt, nr)
+    for l in range(nlag):
+        assert chi[l] == pytest.approx(nr.mean() * q[:, :, l].var(), rel=1e-12)
+        assert qm[l] == pytest.approx(q[:, :, l].mean(), rel=1e-14)
+        assert c[l] == nrep * nor
+    # one replica: N * X4 of Analysis/overlap.c:116-118
+    x4 = (q[0] * q[0]).sum(0) / nor - q[0].sum(0) * q[0].sum(0) / (nor * nor)
+    chi1, _, _ = OL.chi4_ensemble(q[:1].sum(1), (q[:1] * q[:1]).sum(1), cnt[:1], nr[:1])
+    assert np.array_equal(chi1, nr[0] * x4)
+    # uneven origin counts: lags with fewer origins pool only what exists
+    cnt2 = cnt.copy()
+    cnt2[:, 3] = 0
+    chi2, qm2, c2 = OL.chi4_ensemble(q.sum(1), (q * q).sum(1), cnt2, nr)
+    assert chi2[3] == 0.0 and qm2[3] == 0.0 and c2[3] == 0
+    same = np.repeat(q[:1], nrep, 0)
+    chi3, _, _ = OL.chi4_ensemble(same.sum(1), (same * same).sum(1), cnt, nr)
+    assert np.all(np.abs(chi3 - nr.mean() * q[0].var(0)) <= 1e-12 * nr.mean())
+    flat = np.full((nrep, nor, nlag), 0.625)  # exactly representable: no fluctuation at all
+    chi4z, _, _ = OL.chi4_ensemble(flat.sum(1), (flat * flat).sum(1), cnt, nr)
+    assert np.all(chi4z == 0.0)
+

@@ -166,6 +166,25 @@ def test_potential_species_scaling(oracle):
         assert e_bb == pytest.approx(0.5 * e_aa, rel=1e-12)
 
 
+# InitialiseNoseHoover (MDcode/time_evolve.c:477-482) with tau = 0.2 and
+# DOF = 3N-3 (MDcode/global.h:35), worked by hand: Q0 = DOF*T*tau^2,
+# Q1 = Q2 = T*tau^2.  N=1000, T=0.6: DOF=2997, Q0 = 2997*0.6*0.04 = 71.928,
+# Q1 = 0.6*0.04 = 0.024.  N=2, T=1: Q0 = 3*0.04 = 0.12, Q1 = 0.04.
+# N=4096, T=0.45: Q0 = 12285*0.45*0.04 = 221.13, Q1 = 0.018.
+CHAIN_MASS_KAT = [
+    (1000, 0.6, (71.928, 0.024, 0.024)),
+    (2, 1.0, (0.12, 0.04, 0.04)),
+    (4096, 0.45, (221.13, 0.018, 0.018)),
+]
+
+
+@pytest.mark.parametrize("n,T,want", CHAIN_MASS_KAT)
+def test_nhc_masses_known_answers(oracle, n, T, want):
+    q = np.zeros(3)
+    oracle.or_nhc_masses(n, T, OL.dp(q))
+    assert q == pytest.approx(want, rel=1e-14, abs=0)
+
+
 def test_nhc_conserves_heff(oracle):
     n, T = 1000, 0.6
     s, L = lattice_state(n, T=T, jitter=0.02, seed=11)
