@@ -16,7 +16,8 @@ __all__ = ["lib", "load", "MD", "Analysis", "ReplicaParams", "AC4Error", "FP64",
            "OBS_MSD", "OBS_OVERLAP", "OBS_S4", "OBS_FS", "NQ", "LIB_PATH"]
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libac4.so")
+# AC4_LIB names another build of the library (the sanitizer build of the host C, say)
+LIB_PATH = os.environ.get("AC4_LIB") or os.path.join(HERE, "libac4.so")
 FP64, FP32 = 0, 1
 OBS_MSD, OBS_OVERLAP, OBS_S4, OBS_FS = 1, 2, 4, 8
 NQ = 162

@@ -8,7 +8,7 @@ import subprocess
 import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-PATH = os.path.join(ROOT, "oracle", "liboracle.so")
+PATH = os.environ.get("AC4_ORACLE_LIB") or os.path.join(ROOT, "oracle", "liboracle.so")
 D = ctypes.POINTER(ctypes.c_double)
 I = ctypes.POINTER(ctypes.c_int)
 U64 = ctypes.POINTER(ctypes.c_uint64)
