@@ -109,6 +109,9 @@ _SIGS = {
     "ac4_frames_s4_shells": (ctypes.c_int, [_VP, ctypes.c_int, _I, _I, ctypes.c_double, ctypes.c_double,
                                             ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, _D, _D,
                                             _D]),
+    "ac4_frames_s4_shells_noalloc": (ctypes.c_int, [_VP, ctypes.c_int, _I, _I, ctypes.c_double, ctypes.c_double,
+                                                    ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, _D,
+                                                    _D, _D]),
     "ac4_md_set_nve": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int]),
     "ac4_chain_init": (None, [ctypes.c_int, ctypes.c_double, ctypes.c_uint64, _D]),
     "ac4_autotune": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ReplicaParams), ctypes.c_int, ctypes.c_int,
@@ -650,14 +653,15 @@ class Frames:
         _check(lib.ac4_frames_overlap(self.h, i0, i1, a, ctypes.byref(c)))
         return c.value
 
-    def s4_shells(self, orig, lagf, q0, dq, nshell, ndir, seed, a=0.3, phases=False):
-        """Returns (s4_ref, s4_dir[, phases[npairs][ndir][nshell][2]])."""
+    def s4_shells(self, orig, lagf, q0, dq, nshell, ndir, seed, a=0.3, phases=False, noalloc=False):
+        """Returns (s4_ref, s4_dir[, phases[npairs][ndir][nshell][2]]);
+        noalloc=True calls ac4_frames_s4_shells_noalloc."""
         o, lf = _i32(orig), _i32(lagf)
         s4r, s4d = np.empty(nshell), np.empty(nshell)
-        ph = np.empty((len(o), ndir, nshell, 2)) if phases else None
-        _check(lib.ac4_frames_s4_shells(self.h, len(o), _ip(o), _ip(lf), a, q0, dq, nshell, ndir,
-                                        int(seed) & (2**64 - 1), _dp(s4r), _dp(s4d),
-                                        _dp(ph) if phases else None))
+        ph = np.empty((len(o), ndir, nshell, 2)) if (phases or noalloc) else None
+        fn = lib.ac4_frames_s4_shells_noalloc if noalloc else lib.ac4_frames_s4_shells
+        _check(fn(self.h, len(o), _ip(o), _ip(lf), a, q0, dq, nshell, ndir, int(seed) & (2**64 - 1), _dp(s4r),
+                  _dp(s4d), _dp(ph) if ph is not None else None))
         return (s4r, s4d, ph) if phases else (s4r, s4d)
 
     def block(self, gap, cell_cut, sub_dir, factor=1.4, a=0.3, vh_time=1500.0):

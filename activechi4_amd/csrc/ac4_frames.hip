@@ -277,9 +277,42 @@ extern "C" int ac4_shell_directions(int ndir, uint64_t seed, double *n3) {
     return AC4_OK;
 }
 
-extern "C" int ac4_frames_s4_shells(ac4_frames *f, int npairs, const int *orig, const int *lagf, double a, double q0,
-                                    double dq, int nshell, int ndir, uint64_t seed, double *s4_ref, double *s4_dir,
-                                    double *phases) {
+// Variance reductions of S4 on shells from the per-pair phase sums
+// ph[p][d][k][2] (host memory), as described at the top of this file.
+static void s4_reduce_host(const double *ph, int npairs, int ndir, int nshell, double N, double *s4_ref,
+                           double *s4_dir) {
+    const size_t ne = (size_t)ndir * nshell * 2;
+    const double inv_p = 1.0 / npairs;
+    for (int k = 0; k < nshell; k++) {
+        double mc = 0, mc2 = 0, ms = 0, ms2 = 0, dir_sum = 0;
+        for (int p = 0; p < npairs; p++) {
+            double c = 0, s = 0;
+            for (int d = 0; d < ndir; d++) {
+                const double *e = ph + (size_t)p * ne + ((size_t)d * nshell + k) * 2;
+                c += e[0];
+                s += e[1];
+            }
+            c /= ndir;
+            s /= ndir;
+            mc += c; mc2 += c * c; ms += s; ms2 += s * s;
+        }
+        mc *= inv_p; mc2 *= inv_p; ms *= inv_p; ms2 *= inv_p;
+        for (int d = 0; d < ndir; d++) {
+            double c1 = 0, c2 = 0, s1 = 0, s2 = 0;
+            for (int p = 0; p < npairs; p++) {
+                const double *e = ph + (size_t)p * ne + ((size_t)d * nshell + k) * 2;
+                c1 += e[0]; c2 += e[0] * e[0]; s1 += e[1]; s2 += e[1] * e[1];
+            }
+            c1 *= inv_p; c2 *= inv_p; s1 *= inv_p; s2 *= inv_p;
+            dir_sum += (c2 - c1 * c1) + (s2 - s1 * s1);
+        }
+        if (s4_ref) s4_ref[k] = ndir * ((mc2 - mc * mc) + (ms2 - ms * ms)) / N;
+        if (s4_dir) s4_dir[k] = dir_sum / ndir / N;
+    }
+}
+
+static int s4_check_args(ac4_frames *f, int npairs, const int *orig, const int *lagf, double a, int nshell, int ndir,
+                         double dq) {
     AC4_REQUIRE(f && npairs > 0 && orig && lagf && a > 0 && nshell > 0 && ndir > 0 && dq >= 0, AC4_EARG,
                 "frames_s4_shells: bad arguments");
     const int ngroup = (nshell + SHELL_RUN - 1) / SHELL_RUN;
@@ -288,6 +321,14 @@ extern "C" int ac4_frames_s4_shells(ac4_frames *f, int npairs, const int *orig, 
     for (int p = 0; p < npairs; p++)
         AC4_REQUIRE(orig[p] >= 0 && orig[p] < f->count && lagf[p] >= 0 && lagf[p] < f->count, AC4_EARG,
                     "frames_s4_shells: pair %d refers to a frame not in the store", p);
+    return AC4_OK;
+}
+
+extern "C" int ac4_frames_s4_shells(ac4_frames *f, int npairs, const int *orig, const int *lagf, double a, double q0,
+                                    double dq, int nshell, int ndir, uint64_t seed, double *s4_ref, double *s4_dir,
+                                    double *phases) {
+    TRY(s4_check_args(f, npairs, orig, lagf, a, nshell, ndir, dq));
+    const int ngroup = (nshell + SHELL_RUN - 1) / SHELL_RUN;
     const size_t ne = (size_t)ndir * nshell * 2;
     if (f->part_n < FR_BLOCKS * ne) {
         if (f->part) (void)hipFree(f->part);
@@ -321,32 +362,66 @@ extern "C" int ac4_frames_s4_shells(ac4_frames *f, int npairs, const int *orig, 
     }
     AC4_HIP(hipStreamSynchronize(f->s));
     if (phases) memcpy(phases, ph.data(), sizeof(double) * ph.size());
-    const double N = (double)f->n, inv_p = 1.0 / npairs;
-    for (int k = 0; k < nshell; k++) {
-        double mc = 0, mc2 = 0, ms = 0, ms2 = 0, dir_sum = 0;
-        for (int p = 0; p < npairs; p++) {
-            double c = 0, s = 0;
-            for (int d = 0; d < ndir; d++) {
-                const double *e = ph.data() + (size_t)p * ne + ((size_t)d * nshell + k) * 2;
-                c += e[0];
-                s += e[1];
-            }
-            c /= ndir;
-            s /= ndir;
-            mc += c; mc2 += c * c; ms += s; ms2 += s * s;
-        }
-        mc *= inv_p; mc2 *= inv_p; ms *= inv_p; ms2 *= inv_p;
-        for (int d = 0; d < ndir; d++) {
-            double c1 = 0, c2 = 0, s1 = 0, s2 = 0;
-            for (int p = 0; p < npairs; p++) {
-                const double *e = ph.data() + (size_t)p * ne + ((size_t)d * nshell + k) * 2;
-                c1 += e[0]; c2 += e[0] * e[0]; s1 += e[1]; s2 += e[1] * e[1];
-            }
-            c1 *= inv_p; c2 *= inv_p; s1 *= inv_p; s2 *= inv_p;
-            dir_sum += (c2 - c1 * c1) + (s2 - s1 * s1);
-        }
-        if (s4_ref) s4_ref[k] = ndir * ((mc2 - mc * mc) + (ms2 - ms * ms)) / N;
-        if (s4_dir) s4_dir[k] = dir_sum / ndir / N;
+    s4_reduce_host(ph.data(), npairs, ndir, nshell, (double)f->n, s4_ref, s4_dir);
+    return AC4_OK;
+}
+
+// ---------------------------------------------- S4 on shells, allocating nothing
+// The per-block partial sums, the per-pair sums and the directions live in
+// one file-scope device array instead of buffers of the frame store, so a
+// call makes no device or host allocation.  Each GPU holds its own copy of
+// the array until the process ends; its address on the current device is
+// looked up once (hipGetSymbolAddress) and kept per device.  Calls that
+// overlap on one GPU share the array and race.
+// ndir * ceil(nshell / SHELL_RUN) <= FR_THREADS bounds ndir * nshell by
+// FR_THREADS * SHELL_RUN = 6912, so ne = 2 * ndir * nshell <= 13824.
+#define S4_NOALLOC_NE_MAX (2 * FR_THREADS * SHELL_RUN)
+#define S4_NOALLOC_DIR_MAX FR_THREADS
+#define S4_SCRATCH_DOUBLES ((size_t)FR_BLOCKS * S4_NOALLOC_NE_MAX + S4_NOALLOC_NE_MAX + 3 * S4_NOALLOC_DIR_MAX)
+static __device__ __align__(256) double scratch[S4_SCRATCH_DOUBLES];  // 26.7 MB per GPU
+#define S4_MAX_DEV 64
+static double *g_scratch_ptr[S4_MAX_DEV];
+
+static int scratch_ptr(double **out) {
+    int dev = 0;
+    AC4_HIP(hipGetDevice(&dev));
+    AC4_REQUIRE(dev >= 0 && dev < S4_MAX_DEV, AC4_EARG, "s4_shells_noalloc: device %d beyond %d", dev, S4_MAX_DEV);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        AC4_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        AC4_REQUIRE(p != nullptr, AC4_EHIP, "s4_shells_noalloc: no device address for the scratch array");
+        g_scratch_ptr[dev] = (double *)p;
     }
+    *out = g_scratch_ptr[dev];
+    return AC4_OK;
+}
+
+extern "C" int ac4_frames_s4_shells_noalloc(ac4_frames *f, int npairs, const int *orig, const int *lagf, double a,
+                                            double q0, double dq, int nshell, int ndir, uint64_t seed,
+                                            double *s4_ref, double *s4_dir, double *phases) {
+    TRY(s4_check_args(f, npairs, orig, lagf, a, nshell, ndir, dq));
+    AC4_REQUIRE(phases, AC4_EARG, "s4_shells_noalloc: phases (npairs*ndir*nshell*2 doubles) is required");
+    const size_t ne = (size_t)ndir * nshell * 2;
+    AC4_REQUIRE(ne <= S4_NOALLOC_NE_MAX, AC4_ECAP,
+                "s4_shells_noalloc: ndir * nshell = %d exceeds the scratch limit of %d", ndir * nshell,
+                S4_NOALLOC_NE_MAX / 2);
+    const int ngroup = (nshell + SHELL_RUN - 1) / SHELL_RUN;
+    double *sp = nullptr;
+    TRY(scratch_ptr(&sp));
+    double *part = sp, *cs = sp + (size_t)FR_BLOCKS * S4_NOALLOC_NE_MAX, *dirs = cs + S4_NOALLOC_NE_MAX;
+    double dir[3 * S4_NOALLOC_DIR_MAX];  // ndir <= FR_THREADS (checked above)
+    TRY(ac4_shell_directions(ndir, seed, dir));
+    AC4_HIP(hipMemcpyAsync(dirs, dir, sizeof(double) * 3 * ndir, hipMemcpyHostToDevice, f->s));
+    for (int p = 0; p < npairs; p++) {
+        hipLaunchKernelGGL(k_fr_phase, dim3(FR_BLOCKS), dim3(FR_THREADS), 0, f->s, f->x, f->y, f->z, f->n,
+                           (size_t)orig[p] * f->n, (size_t)lagf[p] * f->n, f->L, a * a, q0, dq, nshell, ndir, ngroup,
+                           dirs, part);
+        hipLaunchKernelGGL(k_fr_reduce, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, f->s, part, FR_BLOCKS,
+                           (int)ne, cs);
+        AC4_HIP(hipGetLastError());
+        AC4_HIP(hipMemcpyAsync(phases + (size_t)p * ne, cs, sizeof(double) * ne, hipMemcpyDeviceToHost, f->s));
+    }
+    AC4_HIP(hipStreamSynchronize(f->s));
+    s4_reduce_host(phases, npairs, ndir, nshell, (double)f->n, s4_ref, s4_dir);
     return AC4_OK;
 }

@@ -186,6 +186,20 @@ int ac4_shell_directions(int ndir, uint64_t seed, double *n3);
 int ac4_frames_s4_shells(ac4_frames *f, int npairs, const int *orig, const int *lagf, double a, double q0,
                          double dq, int nshell, int ndir, uint64_t seed, double *s4_ref, double *s4_dir,
                          double *phases);
+/* ac4_frames_s4_shells that allocates nothing, device or host: no reference
+ * counterpart.  For callers that cannot allocate (tight analysis loops in a
+ * serving process, code that must not call hipMalloc).  Its per-block
+ * partials, per-pair sums and directions sit in one file-scope device array
+ * of 26.7 MB that each GPU holds until the process ends, sized for the
+ * largest shape ac4_frames_s4_shells accepts (ndir * ceil(nshell/27) <= 256,
+ * so ndir * nshell <= 6912; larger shapes are refused with AC4_EARG, and a
+ * shape over the array with AC4_ECAP); phases is required
+ * (npairs*ndir*nshell*2 doubles, caller-owned).  Results equal
+ * ac4_frames_s4_shells bit for bit.  Calls that overlap on one GPU share the
+ * array and race. */
+int ac4_frames_s4_shells_noalloc(ac4_frames *f, int npairs, const int *orig, const int *lagf, double a,
+                                 double q0, double dq, int nshell, int ndir, uint64_t seed,
+                                 double *s4_ref, double *s4_dir, double *phases);
 
 /* Block analysis of the stored trajectory for one block edge: replaces
  * Analysis_Block/msd_pos.c:3-240 (mean_square_displacement), overlap.c:4-232
