@@ -117,6 +117,8 @@ _SIGS = {
     "ac4_autotune": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ReplicaParams), ctypes.c_int, ctypes.c_int,
                                     ctypes.c_int, _D, _D, ctypes.c_int, _I]),
     "ac4_set_skin": (ctypes.c_int, [ctypes.c_double]),
+    "ac4_group_plan": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ReplicaParams), ctypes.c_int, _I]),
+    "ac4_md_step_graph_multi": (ctypes.c_int, [ctypes.POINTER(_VP), ctypes.c_int, ctypes.c_int]),
     "ac4_format_frame": (ctypes.c_longlong, [ctypes.c_int, _D, _D, _D, _I, ctypes.c_char_p, ctypes.c_longlong]),
     "ac4_parse_frame": (ctypes.c_longlong, [ctypes.c_char_p, ctypes.c_longlong, ctypes.c_int, _D, _D, _D, _I]),
     "ac4_md_checkpoint_size": (ctypes.c_longlong, [_VP, ctypes.c_int]),
@@ -759,15 +761,32 @@ def chain_init(n, T, seed):
 def autotune(params, precision=FP32, relax=300, steps=200):
     """Times every kernel shape on this replica set and keeps the fastest for
     handles created afterwards (ac4_autotune).  Returns (tpp, nlist_mode,
-    skin) and the table of rows (tpp, nlist_mode, skin, particle-steps/s)."""
+    skin, groups) and the table of rows (tpp, nlist_mode, skin, groups,
+    particle-steps/s); groups = launch groups the set is best split into."""
     load()
     arr = (ReplicaParams * len(params))(*params)
-    best = np.zeros(3)
-    table = np.zeros((32, 4))
+    best = np.zeros(4)
+    table = np.zeros((40, 5))
     nc = ctypes.c_int(0)
-    _check(lib.ac4_autotune(len(params), arr, precision, relax, steps, _dp(best), _dp(table), 32,
+    _check(lib.ac4_autotune(len(params), arr, precision, relax, steps, _dp(best), _dp(table), 40,
                             ctypes.byref(nc)))
-    return (int(best[0]), int(best[1]), float(best[2])), table[: nc.value]
+    return (int(best[0]), int(best[1]), float(best[2]), int(best[3])), table[: nc.value]
+
+
+def group_plan(params, groups):
+    """Launch group of every replica (ac4_group_plan: balanced particle counts)."""
+    load()
+    arr = (ReplicaParams * len(params))(*params)
+    g = np.empty(len(params), dtype=np.int32)
+    _check(lib.ac4_group_plan(len(params), arr, int(groups), _ip(g)))
+    return g
+
+
+def step_multi(mds, nsteps):
+    """nsteps graph steps of several handles together (ac4_md_step_graph_multi)."""
+    load()
+    arr = (_VP * len(mds))(*[m.h for m in mds])
+    _check(lib.ac4_md_step_graph_multi(arr, len(mds), nsteps))
 
 
 def format_frame(x, y, z, typ):

@@ -167,12 +167,16 @@ __global__ void __launch_bounds__(ABLK) k_phase(const Job *__restrict__ jobs, in
             for (int s = 0; s < d; s++) {
                 int o0, o1, o2;
                 perm_q(d, s, c_vx[nq], c_vy[nq], c_vz[nq], o0, o1, o2);
-                const double qx = 2.0 * A_PI * invL * (double)(o0);
-                const double qy = 2.0 * A_PI * invL * (double)(o1);
-                const double qz = 2.0 * A_PI * invL * (double)(o2);
-                const double arg = (qx * L * p0) + (qy * L * p1) + (qz * L * p2);
+                // S4Qt.c:139-214 forms arg = q.L.p with q = 2*A_PI/L*(o0,o1,o2):
+                // arg = 2 pi f with f = o.p.  The whole turns of f are taken
+                // off exactly and sin/cos(2 pi t) evaluated for t in
+                // [-1/2, 1/2] by sincospi, whose reduction is trivial.  The
+                // reference's A_PI and its invL*L differ from pi and 1 in the
+                // 15th digit: below 1e-12 in the phase for |f| <= 63.
+                double f = (double)o0 * p0 + (double)o1 * p1 + (double)o2 * p2;
+                f -= rint(f);
                 double sa, ca;
-                sincos(arg, &sa, &ca);
+                sincospi(2.0 * f, &sa, &ca);
                 cs += ca;
                 sn += sa;
             }
@@ -202,14 +206,33 @@ __global__ void __launch_bounds__(ABLK) k_phase(const Job *__restrict__ jobs, in
     }
 }
 
-// Fixed-order sum over blocks: out[pair*nv + v] = sum_b partial[(job*nb+b)*nv + v]
-__global__ void k_reduce_jobs(const Job *__restrict__ jobs, int nb, int nv, const double *__restrict__ partial,
-                              double *out) {
-    const Job J = jobs[blockIdx.x];
-    for (int v = threadIdx.x; v < nv; v += blockDim.x) {
-        double s = 0.0;
-        for (int b = 0; b < nb; b++) s += partial[((size_t)blockIdx.x * nb + b) * nv + v];
-        out[(size_t)J.pair * nv + v] = s;
+// Fixed-order sum over blocks: out[pair*nv + v] = sum_b partial[(job*nb+b)*nv + v].
+// Grid (jobs, ceil(nv/64)), 256 threads: lane l of wave w takes value
+// v = 64*blockIdx.y + l (consecutive values, coalesced loads) and the blocks
+// b = w, w+4, w+8, ... with 8 loads in flight; the four wave sums are added
+// in wave order.  The order depends on nb and nv only.
+__global__ void __launch_bounds__(256) k_reduce_jobs(const Job *__restrict__ jobs, int nb, int nv,
+                                                     const double *__restrict__ partial, double *out) {
+    __shared__ double s_w[4][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, v = blockIdx.y * 64 + lane;
+    double acc = 0.0;
+    if (v < nv) {
+        const double *p = partial + (size_t)blockIdx.x * nb * nv + v;
+        int b = w;
+        for (; b + 28 < nb; b += 32) {
+            double t[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) t[u] = p[(size_t)(b + 4 * u) * nv];
+#pragma unroll
+            for (int u = 0; u < 8; u++) acc += t[u];
+        }
+        for (; b < nb; b += 4) acc += p[(size_t)b * nv];
+    }
+    s_w[w][lane] = acc;
+    __syncthreads();
+    if (w == 0 && v < nv) {
+        const Job J = jobs[blockIdx.x];
+        out[(size_t)J.pair * nv + v] = ((s_w[0][lane] + s_w[1][lane]) + s_w[2][lane]) + s_w[3][lane];
     }
 }
 
@@ -413,24 +436,24 @@ static int process_frame(ac4_ana *an, int pos, const double *x, const double *y,
         if (w == 0) {
             hipLaunchKernelGGL(k_msd, dim3(nb, nj), dim3(ABLK), 0, s, an->d_jobs, an->n, an->L, x, y, z, t, an->flag,
                                an->d_part);
-            hipLaunchKernelGGL(k_reduce_jobs, dim3(nj), dim3(64), 0, s, an->d_jobs, nb, 3, an->d_part, an->d_msd);
+            hipLaunchKernelGGL(k_reduce_jobs, dim3(nj, 1), dim3(256), 0, s, an->d_jobs, nb, 3, an->d_part, an->d_msd);
         } else {
             if (an->obs & AC4_OBS_OVERLAP) {
                 hipLaunchKernelGGL(k_overlap, dim3(nb, nj), dim3(ABLK), 0, s, an->d_jobs, an->n, an->L, an->param, x,
                                    y, z, t, an->flag, an->d_part);
-                hipLaunchKernelGGL(k_reduce_jobs, dim3(nj), dim3(64), 0, s, an->d_jobs, nb, 1, an->d_part, an->d_ov);
+                hipLaunchKernelGGL(k_reduce_jobs, dim3(nj, 1), dim3(256), 0, s, an->d_jobs, nb, 1, an->d_part, an->d_ov);
             }
             if (an->obs & AC4_OBS_S4) {
                 hipLaunchKernelGGL(k_phase<true>, dim3(nb, NQG, nj), dim3(ABLK), 0, s, an->d_jobs, an->n, an->L,
                                    an->invL, an->param, x, y, z, t, an->flag, an->d_part, an->d_mpart);
-                hipLaunchKernelGGL(k_reduce_jobs, dim3(nj), dim3(256), 0, s, an->d_jobs, nb, 2 * AC4_NQ, an->d_part,
+                hipLaunchKernelGGL(k_reduce_jobs, dim3(nj, (2 * AC4_NQ + 63) / 64), dim3(256), 0, s, an->d_jobs, nb, 2 * AC4_NQ, an->d_part,
                                    an->d_s4);
-                hipLaunchKernelGGL(k_reduce_jobs, dim3(nj), dim3(64), 0, s, an->d_jobs, nb, 1, an->d_mpart, an->d_s4m);
+                hipLaunchKernelGGL(k_reduce_jobs, dim3(nj, 1), dim3(256), 0, s, an->d_jobs, nb, 1, an->d_mpart, an->d_s4m);
             }
             if (an->obs & AC4_OBS_FS) {
                 hipLaunchKernelGGL(k_phase<false>, dim3(nb, NQG, nj), dim3(ABLK), 0, s, an->d_jobs, an->n, an->L,
                                    an->invL, an->param, x, y, z, t, an->flag, an->d_part, an->d_mpart);
-                hipLaunchKernelGGL(k_reduce_jobs, dim3(nj), dim3(256), 0, s, an->d_jobs, nb, 2 * AC4_NQ, an->d_part,
+                hipLaunchKernelGGL(k_reduce_jobs, dim3(nj, (2 * AC4_NQ + 63) / 64), dim3(256), 0, s, an->d_jobs, nb, 2 * AC4_NQ, an->d_part,
                                    an->d_fs);
             }
         }

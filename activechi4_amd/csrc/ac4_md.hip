@@ -1136,7 +1136,7 @@ struct PairAcc;
 template <>
 struct PairAcc<false> {  // FP64: the reference's per-pair arithmetic
     double fx = 0, fy = 0, fz = 0, pe = 0, vir = 0;
-    __device__ __forceinline__ explicit PairAcc(int) {}
+    __device__ __forceinline__ PairAcc(int, const RepC &) {}
     __device__ __forceinline__ void add(const double4 &pi, int ti, const double4 &pj, const RepC &R) {
         const double L = R.L;
         double dx = minimg_md(pi.x - pj.x);
@@ -1166,18 +1166,22 @@ template <>
 struct PairAcc<true> {
     // FP32 pair forces from fixed-point positions (qpos): exact minimum
     // image in integers, separations in units of L/2^32 rounded once to FP32.
+    // The cutoff test runs in those units (r^2 in units^2 against the
+    // cutoff divided by the unit^2), so a candidate outside costs no scaling.
     // Each FP32 pair term is widened to FP64 and summed there (exactly the
     // compensation a Kahan sum aims at, and on CDNA4 a widening add costs
     // two FP64 instructions against five FP32 ones); the force is scaled to
     // real units at the end.  The pair constants of the two possible partner
     // species are set once per particle; one hardware reciprocal of r^2
     // (v_rcp_f32, within 1 ulp) replaces the two divisions, since
-    // sigma^2/r^2 = (sigma/r)^2 and 1/r^2 appears in the force factor.
+    // sigma^2/r^2 = (sigma/r)^2 and 1/r^2 appears in the force factor.  The
+    // pair terms are formed with fused multiply-adds (each rounds once).
     double fx = 0, fy = 0, fz = 0, pe = 0, vir = 0;
-    float sg2[2], e4[2], e8[2], rc2[2], c2i[2];
-    __device__ __forceinline__ explicit PairAcc(int ti) {
+    float sg2[2], e4[2], e8[2], rcq[2], c2i[2], qs2;
+    __device__ __forceinline__ PairAcc(int ti, const RepC &R) {
         // (ti, tj) -> ab = ti + tj: sigma {1, 0.8, 0.88}, epsilon {1, 1.5, 0.5}
         // (cell_list+forces.c:19-20); cutoff (r/sigma)^2 < 6.25 (:189)
+        qs2 = R.qs2;
         for (int tj = 0; tj < 2; tj++) {
             const int ab = ti + tj;
             const double sig = ab == 0 ? 1.00 : (ab == 1 ? 0.80 : 0.88);
@@ -1185,25 +1189,26 @@ struct PairAcc<true> {
             sg2[tj] = (float)(sig * sig);
             e4[tj] = (float)(4.0 * eps);
             e8[tj] = (float)(8.0 * eps);
-            rc2[tj] = (float)(K_LCUTSQR * sig * sig);
+            rcq[tj] = (float)(K_LCUTSQR * sig * sig / (double)R.qs2);
             c2i[tj] = (float)(K_C2 / (sig * sig));  // C2 / sigma^2
         }
     }
-    __device__ __forceinline__ void add(const uint4 &pi, int, const uint4 &pj, const RepC &R) {
+    __device__ __forceinline__ void add(const uint4 &pi, int, const uint4 &pj, const RepC &) {
         const float dx = (float)(int)(pi.x - pj.x);
         const float dy = (float)(int)(pi.y - pj.y);
         const float dz = (float)(int)(pi.z - pj.z);
         const bool b = pj.w != 0u;
-        const float r2 = (dx * dx + dy * dy + dz * dz) * R.qs2;  // real units
-        if (r2 < (b ? rc2[1] : rc2[0])) {
+        const float rq = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));  // (L/2^32)^2 units
+        if (rq < (b ? rcq[1] : rcq[0])) {
+            const float r2 = rq * qs2;  // real units
             const float c2 = b ? c2i[1] : c2i[0];
             const float inv = __builtin_amdgcn_rcpf(r2);
             const float s2 = (b ? sg2[1] : sg2[0]) * inv;  // (sigma/r)^2
             const float s6 = s2 * s2 * s2;
             // cell_list+forces.c:194-203: 4 eps [s6 (s6 - 1) + C0 + C2 (r/sigma)^2],
             // 8 eps [6 s6 (s6 - 1/2) / r^2 - C2 / sigma^2]
-            const float pterm = (b ? e4[1] : e4[0]) * ((s6 * (s6 - 1.0f)) + (float)K_C0 + c2 * r2);
-            const float grad = (b ? e8[1] : e8[0]) * (6.0f * s6 * (s6 - 0.5f) * inv - c2);
+            const float pterm = (b ? e4[1] : e4[0]) * __builtin_fmaf(s6, s6 - 1.0f, __builtin_fmaf(c2, r2, (float)K_C0));
+            const float grad = (b ? e8[1] : e8[0]) * __builtin_fmaf(6.0f * s6 * inv, s6 - 0.5f, -c2);
             fx += (double)(grad * dx);
             fy += (double)(grad * dy);
             fz += (double)(grad * dz);
@@ -1217,6 +1222,15 @@ struct PairAcc<true> {
         oz = fz * R.qscale;
     }
 };
+
+// Four consecutive 16-byte-aligned list entries as one 16-byte load.  (A
+// nontemporal load here measured 79 against 56 us per k_force launch on the
+// 64-replica ensemble: the list stream is better served by the caches.)
+typedef int ac4_v4i __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ int4 ld_list4(const int *p) {
+    const ac4_v4i v = *(const ac4_v4i *)p;
+    return make_int4(v.x, v.y, v.z, v.w);
+}
 
 // Pair forces: a team of tpp lanes per particle (tpp is fixed per handle at
 // creation, so a replica's results do not depend on what else is loaded) walks the particle's
@@ -1249,7 +1263,7 @@ __global__ void __launch_bounds__(BLK) k_force(const RepC *__restrict__ reps, Ch
     if (valid) {
         const PosT pi = pos[g];
         const int ti = (int)pi.w;
-        PairAcc<FP32> acc(ti);
+        PairAcc<FP32> acc(ti, R);
         const int m = nn[g];
         if (R.tpp == 1) {
             // chunks of 8 entries (two 16-byte loads); the next chunk's
@@ -1258,7 +1272,10 @@ __global__ void __launch_bounds__(BLK) k_force(const RepC *__restrict__ reps, Ch
             const int *lp = nl + nl_at(R, cap, 0, i);
             const size_t cst = (size_t)R.n * 8;
             int4 a0 = make_int4(0, 0, 0, 0), a1 = a0;
-            if (m > 0) { a0 = *(const int4 *)lp; a1 = *(const int4 *)(lp + 4); }
+            if (m > 0) {
+                a0 = ld_list4(lp);
+                a1 = ld_list4(lp + 4);
+            }
             int k = 0;
             while (k < m) {
                 const int jj[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
@@ -1268,7 +1285,10 @@ __global__ void __launch_bounds__(BLK) k_force(const RepC *__restrict__ reps, Ch
                 for (int u = 0; u < 8; u++) q[u] = pos[u < valid ? jj[u] : g];
                 k += 8;
                 lp += cst;
-                if (k < m) { a0 = *(const int4 *)lp; a1 = *(const int4 *)(lp + 4); }
+                if (k < m) {
+                    a0 = ld_list4(lp);
+                    a1 = ld_list4(lp + 4);
+                }
 #pragma unroll
                 for (int u = 0; u < 8; u++)
                     if (u < valid) acc.add(pi, ti, q[u], R);
@@ -1454,6 +1474,9 @@ struct ac4_md {
     OvCfg ov{0, 0, 0, 0, 0.0};
     double *ox = nullptr, *oy = nullptr, *oz = nullptr, *ov_q = nullptr, *ov_q2 = nullptr;
     long long *ov_step = nullptr;
+    // block tables of every slot, in slot order, with each slot's first
+    // entry: the launch grids cover only the occupied slots (refresh_tables)
+    std::vector<int> t_brep, t_bfirst, t_bidx, t_frep, t_ffirst, t_lrep, t_lfirst, t_b0, t_f0, t_l0;
     int *ov_list = nullptr;     // replica list of ac4_md_overlap_chi4
     double *ov_out = nullptr;   // its per-lag results
     // frame text on the device (ac4_md_format_frame): 80-byte line slots in
@@ -1632,6 +1655,13 @@ extern "C" int ac4_md_create_slots(int nslots, const int *cap_n, int precision, 
         md->max_ncell = std::max(md->max_ncell, cap_cells);
         md->reps_h.push_back(R);  // idle: n = 0
     }
+    md->t_brep = brep; md->t_bfirst = bfirst; md->t_bidx = bidx;
+    md->t_frep = frep; md->t_ffirst = ffirst; md->t_lrep = lrep; md->t_lfirst = lfirst;
+    for (const RepC &R : md->reps_h) {
+        md->t_b0.push_back(R.blk_off);
+        md->t_f0.push_back(R.fblk_off);
+        md->t_l0.push_back(R.lblk_off);
+    }
     md->ntot = off;
     md->nact_tot = off_act;
     md->ncell_tot = off_cell;
@@ -1716,18 +1746,61 @@ extern "C" int ac4_md_create_slots(int nslots, const int *cap_n, int precision, 
     return AC4_OK;
 }
 
+template <class T>
+static int h2d(T *dst, const T *src, size_t n, hipStream_t s);
+
+// Block tables of the occupied slots only (all slots while none is
+// occupied), in slot order: the step's launch grids skip idle slots, whose
+// blocks would only exit.  A replica's blocks, and so its arithmetic (every
+// per-replica sum runs over its own blocks in their order), do not change.
+// The captured step graph is dropped: its grids change.
+static int refresh_tables(ac4_md *md) {
+    bool any = false;
+    for (const RepC &R : md->reps_h) any |= R.n > 0;
+    std::vector<int> br, bf, bi, fr, ff, lr, lf;
+    for (int r = 0; r < md->R; r++) {
+        RepC &R = md->reps_h[r];
+        if (any && R.n == 0) continue;
+        R.blk_off = (int)br.size();
+        br.insert(br.end(), md->t_brep.begin() + md->t_b0[r], md->t_brep.begin() + md->t_b0[r] + R.nblk);
+        bf.insert(bf.end(), md->t_bfirst.begin() + md->t_b0[r], md->t_bfirst.begin() + md->t_b0[r] + R.nblk);
+        bi.insert(bi.end(), md->t_bidx.begin() + md->t_b0[r], md->t_bidx.begin() + md->t_b0[r] + R.nblk);
+        R.fblk_off = (int)fr.size();
+        fr.insert(fr.end(), md->t_frep.begin() + md->t_f0[r], md->t_frep.begin() + md->t_f0[r] + R.fnblk);
+        ff.insert(ff.end(), md->t_ffirst.begin() + md->t_f0[r], md->t_ffirst.begin() + md->t_f0[r] + R.fnblk);
+        R.lblk_off = (int)lr.size();
+        lr.insert(lr.end(), md->t_lrep.begin() + md->t_l0[r], md->t_lrep.begin() + md->t_l0[r] + R.lnblk);
+        lf.insert(lf.end(), md->t_lfirst.begin() + md->t_l0[r], md->t_lfirst.begin() + md->t_l0[r] + R.lnblk);
+    }
+    hipStream_t s = md->stream;
+    AC4_HIP(hipStreamSynchronize(s));
+    if (md->gexec) { (void)hipGraphExecDestroy(md->gexec); md->gexec = nullptr; }
+    if (md->graph) { (void)hipGraphDestroy(md->graph); md->graph = nullptr; }
+    TRY(h2d(md->blk_rep, br.data(), br.size(), s)); TRY(h2d(md->blk_first, bf.data(), bf.size(), s));
+    TRY(h2d(md->blk_idx, bi.data(), bi.size(), s));
+    TRY(h2d(md->fblk_rep, fr.data(), fr.size(), s)); TRY(h2d(md->fblk_first, ff.data(), ff.size(), s));
+    TRY(h2d(md->lblk_rep, lr.data(), lr.size(), s)); TRY(h2d(md->lblk_first, lf.data(), lf.size(), s));
+    TRY(h2d(md->reps, md->reps_h.data(), md->reps_h.size(), s));
+    AC4_HIP(hipStreamSynchronize(s));
+    md->nblk = (int)br.size();
+    md->fnblk = (int)fr.size();
+    md->lnblk = (int)lr.size();
+    return AC4_OK;
+}
+
 extern "C" int ac4_md_assign(ac4_md *md, int slot, const ac4_replica_params *p) {
     AC4_REQUIRE(md && slot >= 0 && slot < md->R && p, AC4_EARG, "ac4_md_assign: bad arguments");
     RepC R = md->reps_h[slot];
     const char *why = configure_rep(R, *p);
     AC4_REQUIRE(!why, AC4_EARG, "slot %d: %s", slot, why ? why : "");
+    const bool was_idle = md->reps_h[slot].n == 0;
     md->reps_h[slot] = R;
     ChainS c;
     fresh_chain(c, R);
     AC4_HIP(hipMemcpyAsync(md->reps + slot, &md->reps_h[slot], sizeof(RepC), hipMemcpyHostToDevice, md->stream));
     AC4_HIP(hipMemcpyAsync(md->chain + slot, &c, sizeof(ChainS), hipMemcpyHostToDevice, md->stream));
     AC4_HIP(hipStreamSynchronize(md->stream));
-    return AC4_OK;
+    return was_idle ? refresh_tables(md) : AC4_OK;
 }
 
 extern "C" int ac4_md_release(ac4_md *md, int slot) {
@@ -1739,7 +1812,7 @@ extern "C" int ac4_md_release(ac4_md *md, int slot) {
     AC4_HIP(hipMemcpyAsync(md->reps + slot, &R, sizeof(RepC), hipMemcpyHostToDevice, md->stream));
     AC4_HIP(hipMemcpyAsync(md->chain + slot, &c, sizeof(ChainS), hipMemcpyHostToDevice, md->stream));
     AC4_HIP(hipStreamSynchronize(md->stream));
-    return AC4_OK;
+    return refresh_tables(md);
 }
 
 extern "C" int ac4_md_slot_n(const ac4_md *md, int slot) {
@@ -2503,6 +2576,23 @@ extern "C" int ac4_md_step_graph(ac4_md *md, int nsteps) {
         md->gexec = x;
     }
     for (int k = 0; k < nsteps; k++) AC4_HIP(hipGraphLaunch(md->gexec, md->stream));
+    return AC4_OK;
+}
+
+// Several handles stepped together: each on its own stream, the graph
+// replays interleaved step by step so that the handles' kernels overlap on
+// the device (one handle's force kernel beside another's list rebuild).
+// Every handle's arithmetic is that of ac4_md_step_graph alone.
+extern "C" int ac4_md_step_graph_multi(ac4_md *const *mds, int nh, int nsteps) {
+    AC4_REQUIRE(mds && nh > 0 && nsteps >= 0, AC4_EARG, "step_graph_multi: bad arguments");
+    for (int h = 0; h < nh; h++) {
+        AC4_REQUIRE(mds[h], AC4_EARG, "step_graph_multi: handle %d is NULL", h);
+        for (int q = 0; q < h; q++)
+            AC4_REQUIRE(mds[q] != mds[h], AC4_EARG, "step_graph_multi: handle %d appears twice", h);
+        TRY(ac4_md_step_graph(mds[h], 0));  // forces and the captured step of every handle first
+    }
+    for (int k = 0; k < nsteps; k++)
+        for (int h = 0; h < nh; h++) AC4_HIP(hipGraphLaunch(mds[h]->gexec, mds[h]->stream));
     return AC4_OK;
 }
 

@@ -218,10 +218,14 @@ def serving(ac4, njobs, rate_per_s, seed=4242):
                      overlap=SERVE_OVERLAP)
     caps = srv.slots()
     ids = [srv.submit(j["n"], j["T"], j["f0"], j["tau_p"], j["steps"], j["seed"], j["t"]) for j in jobs]
+    trace = []  # (time, particle-steps done, jobs waiting) after every tick
     t0 = time.perf_counter()
     while srv.pending():
-        if srv.tick(time.perf_counter() - t0) == 0:
+        now = time.perf_counter() - t0
+        if srv.tick(now) == 0:
             time.sleep(0.001)  # nothing admitted yet: wait for the next arrival
+        waiting = sum(1 for j, i in zip(jobs, ids) if j["t"] <= now and srv.job(i)["state"] == 0)
+        trace.append((time.perf_counter() - t0, srv.particle_steps(), waiting))
     wall = time.perf_counter() - t0
     lat = []
     for j, i in zip(jobs, ids):
@@ -232,6 +236,12 @@ def serving(ac4, njobs, rate_per_s, seed=4242):
     srv.close()
     work = sum(j["n"] * j["steps"] for j in jobs)
     serve_rate = work / (wall - jobs[0]["t"])
+    # saturated window: from the first tick with a job waiting for a slot to
+    # the last such tick (the server full and requests queued)
+    sat = [k for k, (_, _, w) in enumerate(trace) if w > 0]
+    sat_rate = None
+    if len(sat) >= 2 and trace[sat[-1]][0] > trace[sat[0]][0]:
+        sat_rate = (trace[sat[-1]][1] - trace[sat[0]][1]) / (trace[sat[-1]][0] - trace[sat[0]][0])
     log("serving: %d jobs in %.1f s on %d slots; plain sequential next" % (njobs, wall, len(caps)))
     t_plain = [plain_job(ac4, j) for j in jobs]
     t_seq = sum(t_plain)
@@ -247,6 +257,9 @@ def serving(ac4, njobs, rate_per_s, seed=4242):
             "slots": len(caps), "slot_classes": {str(c): int((caps == c).sum()) for c in np.unique(caps)},
             "slot_capacity_particles": int(caps.sum()), "p50_latency_s": float(np.percentile(lat, 50)),
             "p99_latency_s": float(np.percentile(lat, 99)), "value": serve_rate, "unit": "particle-steps/s",
+            "value_note": "all jobs' particle-steps / (last completion - first arrival)",
+            "saturated_value": sat_rate,
+            "saturated_window_s": (trace[sat[-1]][0] - trace[sat[0]][0]) if len(sat) >= 2 else 0.0,
             "plain_sequential_value": work / t_seq, "plain_sequential_makespan_s": t_seq,
             "plain_sequential_measured": True,
             "plain_sequential_p50_latency_s": float(np.percentile(seq_lat, 50)),
@@ -570,6 +583,39 @@ def reference_programs(ac4, n, frames, X, Y, Z, act):
     return res
 
 
+def s4_row(ac4, n=50000, steps=10000, seed=4321, eager=False):
+    """S4(q, t) of Analysis/S4Qt.c (the 162 GetQ vectors, 573 with their
+    permutations, weighted by the overlap a=0.3) in situ on the GPU, at the
+    reference Analysis' default N=50000 and its frame schedule for 1e4 steps
+    (200 origins): the MD run pushes every frame of time_index_file.dat into
+    the streaming analysis as it is reached (MSD and overlap beside S4).
+    Reports the (origin, lag) pairs and the sincos evaluations they take (573
+    per overlapping particle per pair); the k_phase kernel time comes from a
+    kernel-trace profile of this leg."""
+    default_shape(ac4)
+    p = ac4.ReplicaParams(n, 1.2, 0.6, 1.5, -1, 200, seed)
+    md = start(ac4, [p], ac4.FP32, 300)
+    fidx = [int(v) for v in ac4.schedule(steps, 200)[0]]
+    L_ana = (n / 1.2) ** (1.0 / 3.0)  # Analysis/main.c:15
+    an = ac4.Analysis(n, L_ana, fidx, obs=ac4.OBS_MSD | ac4.OBS_OVERLAP | ac4.OBS_S4)
+    md.sync()
+    t0 = time.perf_counter()
+    t = 0
+    for f, fi in enumerate(fidx):
+        if fi > t:
+            md.step(fi - t, graph=not eager)
+            t = fi
+        an.push_md(f, md, 0)
+    an.finalize()
+    wall = time.perf_counter() - t0
+    _, masks = an.pairs(ac4.OBS_S4)
+    md.close()
+    n_ov = int(np.sum(masks))
+    return {"row": "S4(q,t) in situ, reference schedule", "n": n, "frames": len(fidx), "pairs": int(len(masks)),
+            "overlapping_particle_pairs": n_ov, "sincos": 573 * n_ov, "wall_s_md_and_analysis": wall,
+            "launch": "eager" if eager else "graph"}
+
+
 def pmc_probe(ac4):
     """For hardware-counter runs (rocprofv3 --pmc): the batched workload with
     eager launches (every dispatch is counted), then three device-to-device
@@ -612,19 +658,26 @@ def main():
     ap.add_argument("--replicas", type=int, default=64)
     ap.add_argument("--plain-steps", type=int, default=300)
     ap.add_argument("--relax", type=int, default=300)
-    ap.add_argument("--serve-jobs", type=int, default=24)
-    ap.add_argument("--serve-rate", type=float, default=2.0, help="job arrivals per second")
+    ap.add_argument("--serve-jobs", type=int, default=32)
+    ap.add_argument("--serve-rate", type=float, default=16.0, help="job arrivals per second")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--groups", type=int, default=0,
+                    help="launch groups of the batched ensemble (0: the autotuner's choice, 1 without tuning)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-chi4", action="store_true", help="skip the untimed fixed-length chi4 leg")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="short eager run + 256 MiB copies, for rocprofv3 --pmc (prints no bench line)")
     ap.add_argument("--cfg5", action="store_true",
                     help="the N=10^6 box with frames in HBM and S4 on |q| shells (its own JSON line)")
+    ap.add_argument("--s4", action="store_true",
+                    help="S4(q,t) in situ at N=50000 on the reference schedule (its own JSON line; with --eager "
+                         "for kernel-trace profiles)")
     ap.add_argument("--rows", action="store_true",
                     help="the analysis rows (frame text I/O, block analysis, g_uu, cage overlap), GPU vs oracle")
     ap.add_argument("--only-batched", action="store_true",
                     help="batched ensemble only (no plain, serving or CPU legs), e.g. under a profiler")
+    ap.add_argument("--eager", action="store_true",
+                    help="timed steps as eager launches instead of graph replays (for kernel-trace profiles)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
@@ -641,47 +694,82 @@ def main():
     if a.rows:
         print(json.dumps(rows(ac4)), flush=True)
         return
+    if a.s4:
+        print(json.dumps(s4_row(ac4, eager=a.eager)), flush=True)
+        return
 
     ps = ensemble_params(ac4, a.replicas)
     ntot = sum(p.n for p in ps)
     tune = None
+    groups = a.groups
     if not a.no_tune:
         best, table = ac4.autotune(ps, ac4.FP32, relax=a.relax, steps=200)
-        tune = {"tpp": best[0], "nlist_mode": best[1], "skin": best[2],
-                "candidates": [[int(t), int(m), float(sk), float(v)] for t, m, sk, v in table]}
-        log("autotune: tpp=%d nlist_mode=%d skin=%.2f" % best)
-    md = start(ac4, ps, ac4.FP32, a.relax)
+        groups = best[3] if a.groups == 0 else a.groups
+        tune = {"tpp": best[0], "nlist_mode": best[1], "skin": best[2], "launch_groups": best[3],
+                "candidates": [[int(t), int(m), float(sk), int(g), float(v)] for t, m, sk, g, v in table]}
+        log("autotune: tpp=%d nlist_mode=%d skin=%.2f launch groups=%d" % best)
+    groups = max(1, groups)
+    # replicas per launch: the autotuner's launch groups, one handle and
+    # stream each, stepped together (ac4_md_step_graph_multi)
+    gof = ac4.group_plan(ps, groups)
+    gps = [[p for p, g in zip(ps, gof) if g == k] for k in range(groups)]
+    mds = [start(ac4, gp, ac4.FP32, a.relax) for gp in gps]
     sample_every = CHI4_OV[0]
-    md.overlap_enable(*CHI4_OV)
-    log("%d replicas, %d particles; relaxed %d steps, warmup %d steps" % (a.replicas, ntot, a.relax, a.warmup))
-    md.step(a.warmup, graph=True)  # chain steps before the first origin
-    md.sync()
+    for md in mds:
+        md.overlap_enable(*CHI4_OV)
+    log("%d replicas, %d particles in %d launch group(s) %s; relaxed %d steps, warmup %d steps"
+        % (a.replicas, ntot, groups, [sum(p.n for p in gp) for gp in gps], a.relax, a.warmup))
+    graph = not a.eager
+
+    def advance(k):
+        if graph:
+            ac4.step_multi(mds, k)
+        else:
+            for md in mds:
+                md.step(k)
+
+    def sync():
+        for md in mds:
+            md.sync()
+
+    advance(a.warmup)  # chain steps before the first origin
+    sync()
     # step counters restart at the origin of the accumulators: the first
     # sample is taken at the first multiple of sample_every
     done = a.warmup
     if done % sample_every:
-        md.step(sample_every - done % sample_every, graph=True)
-    md.sync()
+        advance(sample_every - done % sample_every)
+    sync()
     t0 = time.perf_counter()
     left = a.steps
-    md.overlap_sample()
+    for md in mds:
+        md.overlap_sample()
     while left > 0:
         k = min(sample_every, left)
-        md.step(k, graph=True)
+        advance(k)
         left -= k
         if k == sample_every:
-            md.overlap_sample()
-    md.sync()
+            for md in mds:
+                md.overlap_sample()
+    sync()
     t_batched = time.perf_counter() - t0
     value = ntot * a.steps / t_batched
     log("batched %.3f s for %d steps" % (t_batched, a.steps))
+    rebuilds = sum(md.rebuilds(r) for md, gp in zip(mds, gps) for r in range(len(gp)))
+    for md in mds:
+        md.close()
+    # per-kernel HIP-event times (eager, on the library's stream) of one
+    # handle holding the whole ensemble: one launch of each kernel over all
+    # particles, the kernels the launch groups run on their shares
+    md = start(ac4, ps, ac4.FP32, a.relax)
+    md.step(200, graph=graph)
     kt = md.kernel_times(200)
-    rebuilds = sum(md.rebuilds(r) for r in range(a.replicas))
     md.close()
 
     if a.only_batched:
         print(json.dumps({"metric": "particle-steps/s (batched only)", "value": value, "unit": "particle-steps/s",
-                          "steps": a.steps, "kernel_ms_per_step": kt}), flush=True)
+                          "steps": a.steps, "launch": "graph" if graph else "eager",
+                          "kernel_ms_per_step": kt}), flush=True)
         return
     chi4 = None
     if not a.no_chi4:
@@ -713,7 +801,8 @@ def main():
         "config": {"workload": "batched ensemble: %d replicas KA 80:20 rho=1.2 T=0.5, N logU[1000,5000] "
                                "(sum %d), f0 x tau_p grid, FP32 pair forces + FP64 state, hipGraph, "
                                "in-situ Q(t) a=0.3 every 100 steps" % (a.replicas, ntot),
-                   "replicas": a.replicas, "particles": ntot, "forces": "fp32", "positions": "fp64"},
+                   "replicas": a.replicas, "particles": ntot, "forces": "fp32", "positions": "fp64",
+                   "launch_groups": groups, "particles_per_group": [sum(p.n for p in gp) for gp in gps]},
         "plain_single_run": {"value": plain_value, "replicas_run_sequentially": a.replicas, "dtype": "fp64",
                              "launch": "eager", "steps_each": a.plain_steps,
                              "ms_per_step_all_replicas": plain_ms},
@@ -726,7 +815,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": pmc_traffic(),
                      "kernel": "k_force<fp32,kick>", "bytes_per_particle": FORCE_BYTES_PER_PARTICLE,
-                     "particles_per_launch": ntot},
+                     "particles_per_launch": ntot,
+                     "note": "one k_force launch over the whole ensemble (HIP events, eager); the timed run "
+                             "splits the same particles into %d concurrent launch groups" % groups},
     }
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(ac4)

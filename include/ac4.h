@@ -104,6 +104,14 @@ int ac4_md_step(ac4_md *md, int nsteps);
 /* Same as ac4_md_step but records the per-step launch sequence once in a
  * hipGraph and replays it (batched serving path). */
 int ac4_md_step_graph(ac4_md *md, int nsteps);
+/* nsteps steps of each of nh distinct handles, graph replays interleaved on
+ * their own streams so their kernels overlap on the device ("replicas per
+ * launch" below the whole ensemble).  No reference counterpart. */
+int ac4_md_step_graph_multi(ac4_md *const *mds, int nh, int nsteps);
+/* Replicas split into `groups` launch groups of balanced particle count
+ * (largest first, each to the least loaded group; ties to the lower group):
+ * group_of[r] in [0, groups).  No reference counterpart. */
+int ac4_group_plan(int n_rep, const ac4_replica_params *p, int groups, int *group_of);
 /* Neighbour-list rebuild test of every replica (MDcode/time_evolve.c:508):
  * 0 (default) rebuilds when 4*max|x_u - x_old|^2*L^2 > skin^2 (no two
  * particles can close the skin before it fires); 1 is the reference's
@@ -480,12 +488,14 @@ void ac4_chain_init(int n, double T, uint64_t seed, double *chain9);
 /* Autotuner (no reference counterpart): from one relaxed start of the given
  * replica set, times `steps` graph-replayed steps for every kernel shape
  * (force lanes per particle 1..16, i.e. 256..16 particles per workgroup;
- * neighbour-list builder), then the skin for the best shape, and keeps the
- * fastest with ac4_set_tuning / ac4_set_skin.  best3 = {tpp, nlist_mode,
- * skin}; table (rows of {tpp, nlist_mode, skin, particle-steps/s}) may be
- * NULL. */
+ * neighbour-list builder), then the skin for the best shape, then the
+ * replicas per launch (the set in 1..4 launch groups of balanced particle
+ * count, ac4_group_plan, stepped together with ac4_md_step_graph_multi), and
+ * keeps the fastest shape with ac4_set_tuning / ac4_set_skin.  best4 = {tpp,
+ * nlist_mode, skin, groups}; table (rows of {tpp, nlist_mode, skin, groups,
+ * particle-steps/s}) may be NULL. */
 int ac4_autotune(int n_rep, const ac4_replica_params *p, int precision, int relax_steps, int steps,
-                 double *best3, double *table, int table_cap, int *n_cand);
+                 double *best4, double *table, int table_cap, int *n_cand);
 
 #ifdef __cplusplus
 }

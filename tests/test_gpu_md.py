@@ -98,6 +98,28 @@ def test_forces_fp64_match_oracle(gpu, n, f0):
     assert pv[1] == pytest.approx(o[1], rel=1e-12)
 
 
+@pytest.mark.parametrize("n", [300, 500, 640, 874])
+def test_small_box_forces_match_brute_force(gpu, n):
+    """Boxes below N = 875 have fewer than 3 list cells per axis, where the
+    reference's two-cell grid aliases its stencil (MDcode/Parallel.c:72,
+    cell_list+forces.c:37-49) and the build visits every cell once: forces
+    against the oracle's O(N^2) minimum-image sum, FP64 to 1e-12 and FP32
+    to 1e-5, and the FP64 energy and virial."""
+    s, L = lattice_state(n, jitter=0.06, seed=n + 3)
+    assert L > 2 * 2.85  # the list radius stays below half the box
+    fx, fy, fz, o = OL.forces_brute(s, L, 1.2)
+    for prec, tol in ((0, 1e-12), (1, 1e-5)):
+        md = make_md(gpu, [(s, L)], f0=1.2, prec=prec)
+        pv = md.calculate_forces()[0]
+        err = _rel_force_err(md.forces(0), (fx, fy, fz))
+        print("N=%d %s: force error %.2e" % (n, ("FP64", "FP32")[prec], err))
+        assert err < tol
+        if prec == 0:
+            assert pv[0] == pytest.approx(o[0], rel=1e-12)
+            assert pv[1] == pytest.approx(o[1], rel=1e-12)
+        md.close()
+
+
 @pytest.mark.parametrize("skin", [0.25, 0.5])
 def test_skin_changes_lists_not_forces(gpu, skin):
     """A different Verlet skin changes list radii, cell size and rebuild
