@@ -5,10 +5,12 @@
 // and walks (origin, lag) pairs with one MPI_Reduce per pair; here frames are
 // pushed once in file order, origin frames stay resident in HBM, and each
 // pushed frame is compared with all origins whose walk it completes in one
-// launch (grid = particle blocks x q groups x pairs).  Per-pair results are
-// reduced over blocks in a fixed order and accumulated into lag slots in the
-// reference's origin order on the device, so integer observables (overlap
-// counts) reproduce the reference's output bit for bit.
+// launch (grid = particle blocks x pairs, x q groups for Fs); S4's phase sums
+// are formed once per origin over all of its lags on the FP64 matrix cores.
+// Per-pair results are reduced in a fixed order and accumulated into lag
+// slots in the reference's origin order on the device, so integer
+// observables (overlap counts) reproduce the reference's output bit for
+// bit.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <string.h>
@@ -110,7 +112,7 @@ __global__ void __launch_bounds__(ABLK) k_overlap(const Job *__restrict__ jobs, 
 }
 
 // Permutation s of a degeneracy-d vector in the order of S4Qt.c:133-214.
-__device__ __forceinline__ void perm_q(int d, int s, int a, int b, int c, int &o0, int &o1, int &o2) {
+__host__ __device__ __forceinline__ void perm_q(int d, int s, int a, int b, int c, int &o0, int &o1, int &o2) {
     if (d == 1 || s == 0) { o0 = a; o1 = b; o2 = c; return; }
     if (d == 3) {
         if (s == 1) { o0 = b; o1 = c; o2 = a; } else { o0 = c; o1 = a; o2 = b; }
@@ -125,41 +127,23 @@ __device__ __forceinline__ void perm_q(int d, int s, int a, int b, int c, int &o
     }
 }
 
-// S4 (S4Qt.c:101-216, IS_S4) and Fs (Fsqt.c:103-222): grid (blocks, NQG, jobs).
-// partial[((job*nb+blk)*AC4_NQ*2) + n] = cos sum, + AC4_NQ + n = sin sum;
-// mask count of S4 in mpartial[(job*nb+blk)] from q group 0.
-template <bool IS_S4>
-__global__ void __launch_bounds__(ABLK) k_phase(const Job *__restrict__ jobs, int n, double L, double invL,
-                                                double param, const double *__restrict__ x,
+// Fs (Fsqt.c:103-222): grid (blocks, NQG, jobs).  The phase is that of each
+// particle's displacement, so every (origin, lag) pair takes its own 573
+// sincos per particle.
+// partial[((job*nb+blk)*AC4_NQ*2) + n] = cos sum, + AC4_NQ + n = sin sum.
+__global__ void __launch_bounds__(ABLK) k_phase(const Job *__restrict__ jobs, int n, const double *__restrict__ x,
                                                 const double *__restrict__ y, const double *__restrict__ z,
-                                                const int *__restrict__ ty, int flag, double *partial,
-                                                double *mpartial) {
+                                                const int *__restrict__ ty, int flag, double *partial) {
     const Job J = jobs[blockIdx.z];
     const int i = blockIdx.x * ABLK + threadIdx.x;
     const int qg = blockIdx.y;
     double v[2 * QG];
     for (int k = 0; k < 2 * QG; k++) v[k] = 0.0;
-    double m[1] = {0.0};
-    bool use = false;
-    double p0 = 0.0, p1 = 0.0, p2 = 0.0;
-    if (i < n && (IS_S4 || ty[i] != 1 || flag == 0)) {
+    if (i < n && (ty[i] != 1 || flag == 0)) {
         const double x0 = wrapn(J.x0[i], J.slot), y0 = wrapn(J.y0[i], J.slot), z0 = wrapn(J.z0[i], J.slot);
         const double xs = wrap1(x[i]), ys = wrap1(y[i]), zs = wrap1(z[i]);
-        double dx = (xs - x0), dy = (ys - y0), dz = (zs - z0);
-        dz = minimg_ana(dz);
-        dy = minimg_ana(dy);
-        dx = minimg_ana(dx);
-        if (IS_S4) {
-            double r2 = L * L * (dx * dx + dy * dy + dz * dz);  // S4Qt.c:131-132
-            use = r2 < param;
-            p0 = x0; p1 = y0; p2 = z0;   // phases at the origin position
-            m[0] = use ? 1.0 : 0.0;
-        } else {
-            use = true;
-            p0 = dx; p1 = dy; p2 = dz;   // phases of the displacement (Fsqt.c:145-146)
-        }
-    }
-    if (use) {
+        // phases of the displacement (Fsqt.c:145-146)
+        const double p0 = minimg_ana(xs - x0), p1 = minimg_ana(ys - y0), p2 = minimg_ana(zs - z0);
         for (int k = 0; k < QG; k++) {
             const int nq = qg * QG + k;
             const int d = c_dg[nq];
@@ -167,16 +151,12 @@ __global__ void __launch_bounds__(ABLK) k_phase(const Job *__restrict__ jobs, in
             for (int s = 0; s < d; s++) {
                 int o0, o1, o2;
                 perm_q(d, s, c_vx[nq], c_vy[nq], c_vz[nq], o0, o1, o2);
-                // S4Qt.c:139-214 forms arg = q.L.p with q = 2*A_PI/L*(o0,o1,o2):
-                // arg = 2 pi f with f = o.p.  The whole turns of f are taken
-                // off exactly and sin/cos(2 pi t) evaluated for t in
-                // [-1/2, 1/2] by sincospi, whose reduction is trivial.  The
+                // Fsqt.c:147-213 forms arg = q.L.p with q = 2*A_PI/L*(o0,o1,o2):
+                // arg = 2 pi f with f = o.p turns, evaluated in turns.  The
                 // reference's A_PI and its invL*L differ from pi and 1 in the
                 // 15th digit: below 1e-12 in the phase for |f| <= 63.
-                double f = (double)o0 * p0 + (double)o1 * p1 + (double)o2 * p2;
-                f -= rint(f);
                 double sa, ca;
-                sincospi(2.0 * f, &sa, &ca);
+                ac4_sincos_turns((double)o0 * p0 + (double)o1 * p1 + (double)o2 * p2, &sa, &ca);
                 cs += ca;
                 sn += sa;
             }
@@ -200,9 +180,164 @@ __global__ void __launch_bounds__(ABLK) k_phase(const Job *__restrict__ jobs, in
         if (k < QG) out[qg * QG + k] = s;
         else out[AC4_NQ + qg * QG + (k - QG)] = s;
     }
-    if (IS_S4 && qg == 0) {
+}
+
+// ---------------------------------------------------------------- S4
+// S4Qt.c:101-216 sums, for each (origin, lag) pair and GetQ vector q, the
+// phase exp(i q.r0) of every particle that still overlaps its origin
+// position at the lag (|r(t) - r0| < a), averaged over q's permutations.
+// The phase depends on the origin frame alone, so it is evaluated once per
+// origin and shared by all of that origin's lags:
+//   k_s4_mask   at each lag frame: the overlap mask of the pair, one bit per
+//               particle (and the overlap count of the pair);
+//   k_s4_origin when the origin's last lag has been masked: the phase matrix
+//               P[particle][573 permuted vectors] times the mask matrix
+//               M[particle][lags] on FP64 matrix cores, P formed in registers
+//               and never stored;
+//   k_s4_reduce sums the chunk partials in a fixed order and averages each
+//               vector's permutations.
+// This evaluates 573 sincos per particle per origin instead of per pair.
+#define S4_NCOL 576           // the 573 permuted vectors, padded to whole 16-column tiles
+#define S4_NCT (S4_NCOL / 16) // column tiles
+#define S4_RT_MAX 4           // lag rows of one pass: up to 64 lags
+#define S4_CHUNKS 64          // particle chunks a pass is split into (at most)
+
+__constant__ int c_pv[3 * S4_NCOL];  // permuted vector (o0, o1, o2) per column; zero in the padding
+__constant__ int c_pcol[AC4_NQ];     // first column of each GetQ vector
+
+typedef double ac4_d4 __attribute__((ext_vector_type(4)));
+
+// Overlap mask of S4Qt.c:125-133 for each pair: grid (blocks, jobs).  Bit i
+// of mask[pair*nwords + i/32] is set when particle i overlaps; the count per
+// block goes to mpartial[job*nb+blk].  The origin coordinates take the
+// reference's per-lag single-if wraps (wrapn) exactly as it applies them.
+__global__ void __launch_bounds__(ABLK) k_s4_mask(const Job *__restrict__ jobs, int n, double L, double param,
+                                                  const double *__restrict__ x, const double *__restrict__ y,
+                                                  const double *__restrict__ z, int nwords, unsigned *mask,
+                                                  double *mpartial) {
+    const Job J = jobs[blockIdx.y];
+    const int i = blockIdx.x * ABLK + threadIdx.x;
+    bool use = false;
+    if (i < n) {
+        const double x0 = wrapn(J.x0[i], J.slot), y0 = wrapn(J.y0[i], J.slot), z0 = wrapn(J.z0[i], J.slot);
+        const double xs = wrap1(x[i]), ys = wrap1(y[i]), zs = wrap1(z[i]);
+        double dx = (xs - x0), dy = (ys - y0), dz = (zs - z0);
+        dz = minimg_ana(dz);
+        dy = minimg_ana(dy);
+        dx = minimg_ana(dx);
+        const double r2 = L * L * (dx * dx + dy * dy + dz * dz);  // S4Qt.c:131-132
+        use = r2 < param;
+    }
+    const unsigned long long b = __ballot(use);
+    const int lane = threadIdx.x & 63;
+    const int w = (blockIdx.x * ABLK + (threadIdx.x & ~63)) >> 5;  // first of the wave's two words
+    unsigned *row = mask + (size_t)J.pair * nwords;
+    if (lane == 0 && w < nwords) row[w] = (unsigned)b;
+    if (lane == 1 && w + 1 < nwords) row[w + 1] = (unsigned)(b >> 32);
+    double m[1] = {use ? 1.0 : 0.0};
+    block_sum_store<1>(m, mpartial + ((size_t)blockIdx.y * gridDim.x + blockIdx.x));
+}
+
+// One origin, one pass of up to RT*16 of its lags: grid (chunks, S4_NCT),
+// 256 threads.  Wave w of block (c, ct) takes the mask words
+// [(4c+w)*wpw, (4c+w+1)*wpw) -- 32 particles each, 4 per MFMA step -- and
+// the 16 permuted vectors of column tile ct.  v_mfma_f64_16x16x4f64 operand
+// layout: lane l = i + 16k holds A[i][k] (mask bit of lag i, particle k) and
+// B[k][j] with j = i (phase of vector j for particle k); the accumulator's
+// element e of lane l is row l/16 + 4*e, column l%16.  The four waves'
+// sums are added in wave order through LDS, and the block writes
+// part[((c*rows + row)*2 + {cos,sin})*S4_NCOL + column].
+template <int RT>
+__global__ void __launch_bounds__(256) k_s4_origin(const double *__restrict__ x0, const double *__restrict__ y0,
+                                                   const double *__restrict__ z0, int n, int nwords,
+                                                   const unsigned *__restrict__ mask, const int *__restrict__ lags,
+                                                   int nl, int wpw, double *__restrict__ part) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int ct = blockIdx.y;
+    const int i16 = lane & 15, k = lane >> 4;
+    const int col = ct * 16 + i16;
+    const double o0 = (double)c_pv[3 * col], o1 = (double)c_pv[3 * col + 1], o2 = (double)c_pv[3 * col + 2];
+    const unsigned *mrow[RT];
+    bool live[RT];
+    for (int r = 0; r < RT; r++) {
+        const int l = 16 * r + i16;
+        live[r] = l < nl;
+        mrow[r] = mask + (size_t)lags[live[r] ? l : 0] * nwords;
+    }
+    ac4_d4 ac[RT], as[RT];
+    for (int r = 0; r < RT; r++) {
+        ac[r] = ac4_d4{0.0, 0.0, 0.0, 0.0};
+        as[r] = ac4_d4{0.0, 0.0, 0.0, 0.0};
+    }
+    const int w0 = (blockIdx.x * 4 + wv) * wpw;
+    const int w1 = min(w0 + wpw, nwords);
+    for (int w = w0; w < w1; w++) {
+        unsigned mw[RT];
+        for (int r = 0; r < RT; r++) mw[r] = live[r] ? mrow[r][w] : 0u;
+#pragma unroll
+        for (int kk = 0; kk < 8; kk++) {
+            // A particle past n has no mask bit: its (clamped) phase is
+            // multiplied by zero.  exp(i q.r0) is periodic in r0 with the
+            // box, so the stored origin coordinate is used as it is, whatever
+            // single-if wraps the reference applies to it at a given lag.
+            const int i = min(32 * w + 4 * kk + k, n - 1);
+            double s, c;
+            ac4_sincos_turns(fma(o2, z0[i], fma(o1, y0[i], o0 * x0[i])), &s, &c);
+            const int bit = 4 * kk + k;
+            for (int r = 0; r < RT; r++) {
+                const double a = ((mw[r] >> bit) & 1u) ? 1.0 : 0.0;
+                ac[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, c, ac[r], 0, 0, 0);
+                as[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, s, as[r], 0, 0, 0);
+            }
+        }
+    }
+    __shared__ double s_acc[RT * 16 * 32];
+    for (int step = 0; step < 4; step++) {
+        if (wv == step)
+            for (int r = 0; r < RT; r++)
+                for (int e = 0; e < 4; e++) {
+                    const int row = 16 * r + k + 4 * e;
+                    double *pc = &s_acc[row * 32 + i16], *ps = &s_acc[row * 32 + 16 + i16];
+                    if (step == 0) { *pc = ac[r][e]; *ps = as[r][e]; }
+                    else { *pc += ac[r][e]; *ps += as[r][e]; }
+                }
         __syncthreads();
-        block_sum_store<1>(m, mpartial + ((size_t)blockIdx.z * gridDim.x + blockIdx.x));
+    }
+    const int rows = RT * 16;
+    for (int t = threadIdx.x; t < rows * 32; t += 256) {
+        const int row = t >> 5, cc = t & 31, sn = cc >> 4;
+        part[((size_t)(blockIdx.x * rows + row) * 2 + sn) * S4_NCOL + ct * 16 + (cc & 15)] = s_acc[t];
+    }
+}
+
+// Per lag l of the pass and value v (v < AC4_NQ: cos sum of vector v, else
+// sin sum of vector v - AC4_NQ): grid (nl, ceil(2*AC4_NQ/64)), 256 threads.
+// Wave w sums chunks w, w+4, ... (each over the vector's permutations in
+// order); the four wave sums are added in wave order and divided by the
+// number of permutations.  s4[pair*2*AC4_NQ + v].
+__global__ void __launch_bounds__(256) k_s4_reduce(const double *__restrict__ part, int nchunk, int rows,
+                                                   const int *__restrict__ lags, double *__restrict__ s4) {
+    const int l = blockIdx.x;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int v = blockIdx.y * 64 + lane;
+    const bool ok = v < 2 * AC4_NQ;
+    const int sn = ok && v >= AC4_NQ ? 1 : 0;
+    const int q = ok ? v - sn * AC4_NQ : 0;
+    const int d = c_dg[q], c0 = c_pcol[q];
+    double acc = 0.0;
+    if (ok)
+        for (int c = wv; c < nchunk; c += 4) {
+            const double *src = part + ((size_t)(c * rows + l) * 2 + sn) * S4_NCOL + c0;
+            double t = 0.0;
+            for (int s = 0; s < d; s++) t += src[s];
+            acc += t;
+        }
+    __shared__ double s_w[4][64];
+    s_w[wv][lane] = acc;
+    __syncthreads();
+    if (wv == 0 && ok) {
+        const double tot = ((s_w[0][lane] + s_w[1][lane]) + s_w[2][lane]) + s_w[3][lane];
+        s4[(size_t)lags[l] * 2 * AC4_NQ + v] = tot / (double)d;
     }
 }
 
@@ -280,6 +415,15 @@ struct ac4_ana {
     double *d_msd = nullptr, *d_ov = nullptr, *d_s4 = nullptr, *d_s4m = nullptr, *d_fs = nullptr;
     double *d_part = nullptr, *d_mpart = nullptr;
     Job *d_jobs = nullptr;
+    // S4 by origin: pair ids per origin (CSR, walk order), origins whose
+    // last lag is frame pos, the pairs' overlap masks and the chunk partials
+    std::vector<int> s4_optr, s4_opairs;
+    std::vector<std::vector<int>> s4_end;
+    std::vector<char> s4_done;
+    int *d_s4_opairs = nullptr;
+    unsigned *d_s4mask = nullptr;
+    double *d_s4part = nullptr;
+    int nwords = 0, s4_wpw = 1, s4_nchunk = 1;
     int max_jobs = 0, nb = 0;
     std::vector<int> vx, vy, vz, qsq;
     std::vector<double> qval;
@@ -327,6 +471,74 @@ static void build_slots(WalkInfo &W, int nslots) {
     W.slot_pairs.assign(W.w.np, 0);
     std::vector<int> fill(W.slot_ptr.begin(), W.slot_ptr.end() - 1);
     for (int p = 0; p < W.w.np; p++) W.slot_pairs[fill[W.w.slot[p]]++] = p;  // walk order kept
+}
+
+// S4 by origin: the permuted-vector columns, the per-origin pair lists and
+// the device buffers.  Masks and sums start at zero, so a pair whose lag
+// frame never arrives contributes nothing.
+static int s4_setup(ac4_ana *an, const std::vector<int> &dg) {
+    std::vector<int> pv(3 * S4_NCOL, 0), pcol(AC4_NQ, 0);
+    int col = 0;
+    for (int q = 0; q < AC4_NQ; q++) {
+        pcol[q] = col;
+        for (int s = 0; s < dg[q]; s++, col++)
+            perm_q(dg[q], s, an->vx[q], an->vy[q], an->vz[q], pv[3 * col], pv[3 * col + 1], pv[3 * col + 2]);
+    }
+    AC4_REQUIRE(col <= S4_NCOL, AC4_EARG, "S4: %d permuted vectors exceed %d columns", col, S4_NCOL);
+    AC4_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_pv), pv.data(), sizeof(int) * 3 * S4_NCOL));
+    AC4_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_pcol), pcol.data(), sizeof(int) * AC4_NQ));
+    const ac4_walk_t &w = an->W[1].w;
+    an->s4_optr.assign(an->nframes + 1, 0);
+    for (int p = 0; p < w.np; p++) an->s4_optr[w.cur[p] + 1]++;
+    for (int o = 0; o < an->nframes; o++) an->s4_optr[o + 1] += an->s4_optr[o];
+    an->s4_opairs.assign(std::max(w.np, 1), 0);
+    std::vector<int> fill(an->s4_optr.begin(), an->s4_optr.end() - 1);
+    for (int p = 0; p < w.np; p++) an->s4_opairs[fill[w.cur[p]]++] = p;
+    an->s4_end.assign(an->nframes, {});
+    for (int o = 0; o < an->nframes; o++)
+        if (an->W[1].last_use[o] >= 0) an->s4_end[an->W[1].last_use[o]].push_back(o);
+    an->s4_done.assign(an->nframes, 0);
+    an->nwords = (an->n + 31) / 32;
+    an->s4_wpw = std::max(1, (an->nwords + 4 * S4_CHUNKS - 1) / (4 * S4_CHUNKS));
+    an->s4_nchunk = (an->nwords + 4 * an->s4_wpw - 1) / (4 * an->s4_wpw);
+    const size_t np1 = std::max(w.np, 1);
+    AC4_HIP(hipMalloc(&an->d_s4_opairs, sizeof(int) * np1));
+    AC4_HIP(hipMemcpy(an->d_s4_opairs, an->s4_opairs.data(), sizeof(int) * np1, hipMemcpyHostToDevice));
+    AC4_HIP(hipMalloc(&an->d_s4mask, sizeof(unsigned) * np1 * an->nwords));
+    AC4_HIP(hipMalloc(&an->d_s4part, sizeof(double) * (size_t)an->s4_nchunk * 16 * S4_RT_MAX * 2 * S4_NCOL));
+    AC4_HIP(hipMemset(an->d_s4mask, 0, sizeof(unsigned) * np1 * an->nwords));
+    AC4_HIP(hipMemset(an->d_s4, 0, sizeof(double) * 2 * AC4_NQ * np1));
+    AC4_HIP(hipMemset(an->d_s4m, 0, sizeof(double) * np1));
+    return AC4_OK;
+}
+
+// The S4 sums of every pair of origin o, from its resident frame and the
+// masks its lags left; passes of up to 16*S4_RT_MAX lags.
+static int s4_origin(ac4_ana *an, int o) {
+    const Frame &F = an->store[o];
+    AC4_REQUIRE(F.x, AC4_ESTATE, "S4: origin frame %d is not resident", o);
+    hipStream_t s = an->stream;
+    const int b = an->s4_optr[o], e = an->s4_optr[o + 1];
+    for (int p0 = b; p0 < e; p0 += 16 * S4_RT_MAX) {
+        const int nl = std::min(16 * S4_RT_MAX, e - p0), rt = (nl + 15) / 16;
+        const int *lags = an->d_s4_opairs + p0;
+        const dim3 g(an->s4_nchunk, S4_NCT);
+        switch (rt) {
+            case 1: hipLaunchKernelGGL(k_s4_origin<1>, g, dim3(256), 0, s, F.x, F.y, F.z, an->n, an->nwords,
+                                       an->d_s4mask, lags, nl, an->s4_wpw, an->d_s4part); break;
+            case 2: hipLaunchKernelGGL(k_s4_origin<2>, g, dim3(256), 0, s, F.x, F.y, F.z, an->n, an->nwords,
+                                       an->d_s4mask, lags, nl, an->s4_wpw, an->d_s4part); break;
+            case 3: hipLaunchKernelGGL(k_s4_origin<3>, g, dim3(256), 0, s, F.x, F.y, F.z, an->n, an->nwords,
+                                       an->d_s4mask, lags, nl, an->s4_wpw, an->d_s4part); break;
+            default: hipLaunchKernelGGL(k_s4_origin<4>, g, dim3(256), 0, s, F.x, F.y, F.z, an->n, an->nwords,
+                                        an->d_s4mask, lags, nl, an->s4_wpw, an->d_s4part); break;
+        }
+        hipLaunchKernelGGL(k_s4_reduce, dim3(nl, (2 * AC4_NQ + 63) / 64), dim3(256), 0, s, an->d_s4part,
+                           an->s4_nchunk, rt * 16, lags, an->d_s4);
+        AC4_HIP(hipGetLastError());
+    }
+    an->s4_done[o] = 1;
+    return AC4_OK;
 }
 
 extern "C" int ac4_ana_create(int n, double L, int nframes, const int *fidx, int gap, double factor, double a,
@@ -387,6 +599,7 @@ extern "C" int ac4_ana_create(int n, double L, int nframes, const int *fidx, int
     if (obs & AC4_OBS_S4) {
         if (hipMalloc(&an->d_s4, sizeof(double) * 2 * AC4_NQ * (size_t)np1) != hipSuccess) return fail(AC4_EHIP);
         if (hipMalloc(&an->d_s4m, sizeof(double) * (size_t)np1) != hipSuccess) return fail(AC4_EHIP);
+        if ((rc = s4_setup(an, dg)) != AC4_OK) return fail(rc);
     }
     if (obs & AC4_OBS_FS) if (hipMalloc(&an->d_fs, sizeof(double) * 2 * AC4_NQ * (size_t)np1) != hipSuccess) return fail(AC4_EHIP);
     if ((rc = frame_alloc(an->cur, n)) != AC4_OK) return fail(rc);
@@ -399,7 +612,8 @@ extern "C" int ac4_ana_destroy(ac4_ana *an) {
     if (an->stream) (void)hipStreamSynchronize(an->stream);
     for (auto &f : an->store) frame_free(f);
     frame_free(an->cur);
-    void *ptrs[] = {an->d_msd, an->d_ov, an->d_s4, an->d_s4m, an->d_fs, an->d_part, an->d_mpart, an->d_jobs};
+    void *ptrs[] = {an->d_msd, an->d_ov, an->d_s4, an->d_s4m, an->d_fs, an->d_part, an->d_mpart, an->d_jobs,
+                    an->d_s4_opairs, an->d_s4mask, an->d_s4part};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &W : an->W) ac4_walk_free(&W.w);
@@ -444,21 +658,25 @@ static int process_frame(ac4_ana *an, int pos, const double *x, const double *y,
                 hipLaunchKernelGGL(k_reduce_jobs, dim3(nj, 1), dim3(256), 0, s, an->d_jobs, nb, 1, an->d_part, an->d_ov);
             }
             if (an->obs & AC4_OBS_S4) {
-                hipLaunchKernelGGL(k_phase<true>, dim3(nb, NQG, nj), dim3(ABLK), 0, s, an->d_jobs, an->n, an->L,
-                                   an->invL, an->param, x, y, z, t, an->flag, an->d_part, an->d_mpart);
-                hipLaunchKernelGGL(k_reduce_jobs, dim3(nj, (2 * AC4_NQ + 63) / 64), dim3(256), 0, s, an->d_jobs, nb, 2 * AC4_NQ, an->d_part,
-                                   an->d_s4);
+                hipLaunchKernelGGL(k_s4_mask, dim3(nb, nj), dim3(ABLK), 0, s, an->d_jobs, an->n, an->L, an->param, x,
+                                   y, z, an->nwords, an->d_s4mask, an->d_mpart);
                 hipLaunchKernelGGL(k_reduce_jobs, dim3(nj, 1), dim3(256), 0, s, an->d_jobs, nb, 1, an->d_mpart, an->d_s4m);
             }
             if (an->obs & AC4_OBS_FS) {
-                hipLaunchKernelGGL(k_phase<false>, dim3(nb, NQG, nj), dim3(ABLK), 0, s, an->d_jobs, an->n, an->L,
-                                   an->invL, an->param, x, y, z, t, an->flag, an->d_part, an->d_mpart);
+                hipLaunchKernelGGL(k_phase, dim3(nb, NQG, nj), dim3(ABLK), 0, s, an->d_jobs, an->n, x, y, z, t,
+                                   an->flag, an->d_part);
                 hipLaunchKernelGGL(k_reduce_jobs, dim3(nj, (2 * AC4_NQ + 63) / 64), dim3(256), 0, s, an->d_jobs, nb, 2 * AC4_NQ, an->d_part,
                                    an->d_fs);
             }
         }
         AC4_HIP(hipGetLastError());
         // jobs[] is a pageable host vector: the copy must finish before it dies
+        AC4_HIP(hipStreamSynchronize(s));
+    }
+    // origins whose last lag this frame was: their S4 sums, before the
+    // origin frames are released
+    if ((an->obs & AC4_OBS_S4) && !an->s4_end[pos].empty()) {
+        for (int o : an->s4_end[pos]) TRY(s4_origin(an, o));
         AC4_HIP(hipStreamSynchronize(s));
     }
     return AC4_OK;
@@ -550,6 +768,9 @@ static void split_phase(const std::vector<double> &in, int ns, std::vector<doubl
 extern "C" int ac4_ana_finalize(ac4_ana *an) {
     AC4_REQUIRE(an, AC4_EARG, "NULL handle");
     if (an->finalized) return AC4_OK;
+    if (an->obs & AC4_OBS_S4)  // origins whose last lag frame was never pushed
+        for (int o = 0; o < an->nframes; o++)
+            if (!an->s4_done[o] && an->store[o].x && an->s4_optr[o + 1] > an->s4_optr[o]) TRY(s4_origin(an, o));
     AC4_HIP(hipStreamSynchronize(an->stream));
     const double tot = an->flag == 1 ? 0.8 * an->n : (double)an->n;  // msd_pos.c:26-27
     for (int w = 0; w < 2; w++) {  // counters and delta_t (msd_pos.c:90-92)
@@ -696,6 +917,12 @@ extern "C" int ac4_ana_write(ac4_ana *an, const char *sub_dir) {
                             an->fss.data(), an->fsc2.data(), an->fss2.data(), an->cnt1.data(), an->dt1.data(),
                             an->qval.data()));
     }
+    return AC4_OK;
+}
+
+extern "C" int ac4_sincos_turns_host(const double *f, int n, double *s, double *c) {
+    AC4_REQUIRE(f && s && c && n >= 0, AC4_EARG, "sincos_turns: bad arguments");
+    for (int i = 0; i < n; i++) ac4_sincos_turns(f[i], &s[i], &c[i]);
     return AC4_OK;
 }
 

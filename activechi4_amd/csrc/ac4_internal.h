@@ -78,6 +78,41 @@ __device__ inline int ac4_wave_sum_i(int v) {
     return v;
 }
 
+// sin and cos of 2*pi*f for a phase f given in turns (|f| < 2^28).  The
+// nearest quarter turn q = rint(4f) is taken off exactly (f - q/4 is exact
+// by Sterbenz's lemma), leaving x = 2*pi*(f - q/4) in [-pi/4, pi/4]; sin x
+// and cos x are their Taylor polynomials to x^15 and x^16, whose remainders
+// are below 5e-17 and 3e-18 there, in Horner form.  The quarter turns then
+// rotate (sin x, cos x).  Every operation is written out (explicit fma), so
+// host and device evaluate it identically.
+__host__ __device__ inline void ac4_sincos_turns(double f, double *s_out, double *c_out) {
+    const double q = rint(4.0 * f);
+    const double x = fma(-0.25, q, f) * 6.283185307179586;
+    const double z = x * x;
+    double ps = -1.0 / 1307674368000.0;              // -1/15!
+    ps = fma(z, ps, 1.0 / 6227020800.0);             //  1/13!
+    ps = fma(z, ps, -1.0 / 39916800.0);              // -1/11!
+    ps = fma(z, ps, 1.0 / 362880.0);                 //  1/9!
+    ps = fma(z, ps, -1.0 / 5040.0);                  // -1/7!
+    ps = fma(z, ps, 1.0 / 120.0);                    //  1/5!
+    ps = fma(z, ps, -1.0 / 6.0);                     // -1/3!
+    const double sx = fma(x * z, ps, x);
+    double pc = 1.0 / 20922789888000.0;              //  1/16!
+    pc = fma(z, pc, -1.0 / 87178291200.0);           // -1/14!
+    pc = fma(z, pc, 1.0 / 479001600.0);              //  1/12!
+    pc = fma(z, pc, -1.0 / 3628800.0);               // -1/10!
+    pc = fma(z, pc, 1.0 / 40320.0);                  //  1/8!
+    pc = fma(z, pc, -1.0 / 720.0);                   // -1/6!
+    pc = fma(z, pc, 1.0 / 24.0);                     //  1/4!
+    pc = fma(z, pc, -0.5);                           // -1/2!
+    const double cx = fma(z, pc, 1.0);
+    // rotate by k = q mod 4 quarter turns: (s, c) -> (c, -s) -> (-s, -c) -> (-c, s)
+    const int k = (int)q & 3;
+    const double a = (k & 1) ? cx : sx, b = (k & 1) ? sx : cx;
+    *s_out = (k & 2) ? -a : a;
+    *c_out = ((k + 1) & 2) ? -b : b;
+}
+
 // Device view of a frame store (ac4_frames.hip) for the analyses built on it:
 // frame k of particle i is x[k*n + i]; step[k] is its step index.
 struct ac4_frames_view {

@@ -589,13 +589,16 @@ def s4_row(ac4, n=50000, steps=10000, seed=4321, eager=False):
     reference Analysis' default N=50000 and its frame schedule for 1e4 steps
     (200 origins): the MD run pushes every frame of time_index_file.dat into
     the streaming analysis as it is reached (MSD and overlap beside S4).
-    Reports the (origin, lag) pairs and the sincos evaluations they take (573
-    per overlapping particle per pair); the k_phase kernel time comes from a
+    Reports the (origin, lag) pairs, the sincos evaluations the device makes
+    (576 columns, the 573 permuted vectors padded to whole tiles, per
+    particle per origin) and those a per-pair evaluation would make (573 per
+    overlapping particle per pair); the S4 kernel times come from a
     kernel-trace profile of this leg."""
     default_shape(ac4)
     p = ac4.ReplicaParams(n, 1.2, 0.6, 1.5, -1, 200, seed)
     md = start(ac4, [p], ac4.FP32, 300)
-    fidx = [int(v) for v in ac4.schedule(steps, 200)[0]]
+    n_origins = 200
+    fidx = [int(v) for v in ac4.schedule(steps, n_origins)[0]]
     L_ana = (n / 1.2) ** (1.0 / 3.0)  # Analysis/main.c:15
     an = ac4.Analysis(n, L_ana, fidx, obs=ac4.OBS_MSD | ac4.OBS_OVERLAP | ac4.OBS_S4)
     md.sync()
@@ -611,8 +614,9 @@ def s4_row(ac4, n=50000, steps=10000, seed=4321, eager=False):
     _, masks = an.pairs(ac4.OBS_S4)
     md.close()
     n_ov = int(np.sum(masks))
-    return {"row": "S4(q,t) in situ, reference schedule", "n": n, "frames": len(fidx), "pairs": int(len(masks)),
-            "overlapping_particle_pairs": n_ov, "sincos": 573 * n_ov, "wall_s_md_and_analysis": wall,
+    return {"row": "S4(q,t) in situ, reference schedule", "n": n, "frames": len(fidx), "origins": n_origins,
+            "pairs": int(len(masks)), "overlapping_particle_pairs": n_ov, "sincos": 576 * n * n_origins,
+            "sincos_per_pair_form": 573 * n_ov, "wall_s_md_and_analysis": wall,
             "launch": "eager" if eager else "graph"}
 
 

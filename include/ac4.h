@@ -448,9 +448,16 @@ int ac4_ana_get_phase(ac4_ana *an, int obs, double *cs, double *ss,
                       double *cs2, double *ss2, int *counters,
                       double *delta_t, double *qval);
 /* Per-pair raw results in walk order (rule of `obs`): MSD 3 sums, overlap
- * count, S4 masked count, S4/Fs {cos[162], sin[162]}. */
+ * count, S4 masked count, S4/Fs {cos[162], sin[162]}.  S4's phase sums are
+ * formed per origin, over all of its lags at once: an origin's pairs hold
+ * them once its last lag frame has been pushed (or after finalize); until
+ * then they read zero. */
 int ac4_ana_npairs(const ac4_ana *an, int obs);
 int ac4_ana_get_pairs(ac4_ana *an, int obs, double *vals, int *ivals);
+/* Test hook (no reference counterpart): the host build of the sin/cos the
+ * S4 and Fs kernels evaluate, s[i], c[i] = sin, cos(2*pi*f[i]) for phases f
+ * in turns (|f| < 2^28); the device code is the same source. */
+int ac4_sincos_turns_host(const double *f, int n, double *s, double *c);
 /* Writes msd.dat, relax_a_%.2lf.dat, S4qt_a_%.2lf.dat, Fsqt_a_%.2lf.dat into
  * sub_dir with the reference's formats (msd_pos.c:112-120, overlap.c:108-121,
  * S4Qt.c:251-271, Fsqt.c:257-278). */

@@ -46,6 +46,28 @@ def test_pairs_match_oracle(gpu, wild, flag):
     assert np.array_equal(ov["c"][:k][ov["counters"][:k] > 0], oo["c"][oo["counters"] > 0])
 
 
+def test_s4_origins_with_more_lags_than_one_pass(gpu):
+    """S4 is evaluated per origin over all of its lags, in passes of at most
+    64 lags.  With the walk factor 1.05 on 1000 consecutive frames and
+    origins 400 frames apart the origins have 100, 100 and 76 lags: passes of
+    64, 36 and 12 lags (4, 3 and 1 row tiles of 16).  N=300 is not a multiple
+    of the 32-particle mask word.  Per-pair sums and overlap counts against
+    the oracle."""
+    n, gap, factor = 300, 400, 1.05
+    frames = list(range(1000))
+    X, Y, Z, T = trajgen.make_frames(n, frames, 99, 0.05, 1.0)
+    L = OL.load().or_ana_box_length(n)
+    an = gpu.Analysis(n, L, frames, gap=gap, factor=factor, obs=gpu.OBS_S4)
+    for f in range(len(frames)):
+        an.push(f, X[f], Y[f], Z[f], T[f])
+    an.finalize()
+    os4 = OL.phase("s4", n, L, X, Y, Z, T, frames, gap, factor=factor)
+    v, m = an.pairs(gpu.OBS_S4)
+    assert len(m) == len(os4["mask"]) > 3 * 64
+    assert np.array_equal(m, os4["mask"])
+    assert np.abs(v - os4["pairs"]).max() < 1e-9 * n
+
+
 def _numbers(path):
     return np.array([float(t) for line in open(path) for t in line.split()])
 

@@ -171,3 +171,26 @@ def test_frame_parse_stops_at_the_text_end(ac4):
             ac4.parse_frame(bad, 1)
     with pytest.raises(ac4.AC4Error, match="ends at line 1"):
         ac4.parse_frame(b"1\t2\t3\t0\n", 2)
+
+
+def test_sincos_turns_against_extended_precision(ac4):
+    """The sin/cos of the S4 and Fs kernels (host build of the same source):
+    within 2.5e-16 of sin/cos(2 pi f) formed in x87 extended precision, for
+    phases in [-1/2, 1/2] and with up to 100 whole turns added; exact at the
+    quarter turns."""
+    rng = np.random.default_rng(11)
+    f = np.concatenate([rng.uniform(-0.5, 0.5, 200000), [0.0, 0.125, -0.125, 0.375, -0.375, 1e-300]])
+    f = (f[:, None] + np.array([0.0, 1.0, -3.0, 17.0, -100.0])[None, :]).ravel()
+    s, c = np.empty_like(f), np.empty_like(f)
+    dp = ctypes.POINTER(ctypes.c_double)
+    assert ac4.lib.ac4_sincos_turns_host(f.ctypes.data_as(dp), len(f), s.ctypes.data_as(dp),
+                                         c.ctypes.data_as(dp)) == 0
+    pi = np.longdouble("3.14159265358979323846264338327950288")
+    x = 2 * pi * f.astype(np.longdouble)
+    assert float(np.abs(s - np.sin(x)).max()) < 2.5e-16
+    assert float(np.abs(c - np.cos(x)).max()) < 2.5e-16
+    q = np.array([0.0, 0.25, 0.5, 0.75, -0.25, -0.5, 3.0])
+    sq, cq = np.empty_like(q), np.empty_like(q)
+    ac4.lib.ac4_sincos_turns_host(q.ctypes.data_as(dp), len(q), sq.ctypes.data_as(dp), cq.ctypes.data_as(dp))
+    assert np.array_equal(sq, [0.0, 1.0, 0.0, -1.0, -1.0, 0.0, 0.0])
+    assert np.array_equal(cq, [1.0, 0.0, -1.0, 0.0, 0.0, -1.0, 1.0])
