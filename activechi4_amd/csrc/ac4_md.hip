@@ -1232,6 +1232,43 @@ __device__ __forceinline__ int4 ld_list4(const int *p) {
     return make_int4(v.x, v.y, v.z, v.w);
 }
 
+// One lane per particle: the list walk in groups of C entries (C/8 list
+// chunks of two 16-byte index loads each), the indices of the next group
+// loaded while this group's positions are gathered; entries at or past m are
+// replaced by the particle itself before they are used as an index.
+template <int C>
+__device__ __forceinline__ void ld_idx(const int *lp, size_t cst, int k, int m, int g, int (&j)[C]) {
+#pragma unroll
+    for (int c = 0; c < C / 8; c++) {
+        if (k + 8 * c < m) {  // this list chunk holds entries
+            const int *p = lp + (size_t)(k / 8 + c) * cst;
+            const int4 a = ld_list4(p), b = ld_list4(p + 4);
+            j[8 * c + 0] = a.x; j[8 * c + 1] = a.y; j[8 * c + 2] = a.z; j[8 * c + 3] = a.w;
+            j[8 * c + 4] = b.x; j[8 * c + 5] = b.y; j[8 * c + 6] = b.z; j[8 * c + 7] = b.w;
+        } else {
+#pragma unroll
+            for (int u = 0; u < 8; u++) j[8 * c + u] = g;
+        }
+    }
+}
+template <int C, class PosT, class Acc>
+__device__ __forceinline__ void walk_list1(const RepC &R, const int *lp, int m, int g, const PosT *__restrict__ pos,
+                                           const PosT &pi, int ti, Acc &acc) {
+    const size_t cst = (size_t)R.n * 8;
+    if (m <= 0) return;
+    int j[C];
+    PosT q[C];
+    ld_idx<C>(lp, cst, 0, m, g, j);
+    for (int k = 0; k < m; k += C) {
+#pragma unroll
+        for (int u = 0; u < C; u++) q[u] = pos[k + u < m ? j[u] : g];
+        if (k + C < m) ld_idx<C>(lp, cst, k + C, m, g, j);
+#pragma unroll
+        for (int u = 0; u < C; u++)
+            if (k + u < m) acc.add(pi, ti, q[u], R);
+    }
+}
+
 // Pair forces: a team of tpp lanes per particle (tpp is fixed per handle at
 // creation, so a replica's results do not depend on what else is loaded) walks the particle's
 // list with stride tpp, 8 independent index/position loads in flight per lane;
@@ -1266,33 +1303,7 @@ __global__ void __launch_bounds__(BLK) k_force(const RepC *__restrict__ reps, Ch
         PairAcc<FP32> acc(ti, R);
         const int m = nn[g];
         if (R.tpp == 1) {
-            // chunks of 8 entries (two 16-byte loads); the next chunk's
-            // indices are loaded while this chunk's positions are gathered.
-            // Entries past m in the last chunk are loaded, never used.
-            const int *lp = nl + nl_at(R, cap, 0, i);
-            const size_t cst = (size_t)R.n * 8;
-            int4 a0 = make_int4(0, 0, 0, 0), a1 = a0;
-            if (m > 0) {
-                a0 = ld_list4(lp);
-                a1 = ld_list4(lp + 4);
-            }
-            int k = 0;
-            while (k < m) {
-                const int jj[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
-                const int valid = min(8, m - k);
-                PosT q[8];
-#pragma unroll
-                for (int u = 0; u < 8; u++) q[u] = pos[u < valid ? jj[u] : g];
-                k += 8;
-                lp += cst;
-                if (k < m) {
-                    a0 = ld_list4(lp);
-                    a1 = ld_list4(lp + 4);
-                }
-#pragma unroll
-                for (int u = 0; u < 8; u++)
-                    if (u < valid) acc.add(pi, ti, q[u], R);
-            }
+            walk_list1<8>(R, nl + nl_at(R, cap, 0, i), m, g, pos, pi, ti, acc);
             acc.result(Fx, Fy, Fz, R);
             v_pe = acc.pe;
             v_vir = acc.vir;

@@ -50,6 +50,10 @@ HBM_PEAK_GBS = 8000.0
 # entries (4 B each) are the kernel's other stream and show in the measured
 # traffic.
 FORCE_BYTES_PER_PARTICLE = 112.0
+# compulsory HBM bytes of one whole mixed-precision step per particle (force
+# kernel: packed FP32 position + type 16, force 12; integration: read v, F, x,
+# x_u 84, write v, x, x_u and the FP32 position copy 88): 200 B.
+STEP_BYTES_PER_PARTICLE = 200.0
 PMC_FILE = os.path.join(ROOT, "profiles", "k_force_batched64_pmc.txt")
 
 
@@ -821,7 +825,10 @@ def main():
                      "kernel": "k_force<fp32,kick>", "bytes_per_particle": FORCE_BYTES_PER_PARTICLE,
                      "particles_per_launch": ntot,
                      "note": "one k_force launch over the whole ensemble (HIP events, eager); the timed run "
-                             "splits the same particles into %d concurrent launch groups" % groups},
+                             "splits the same particles into %d concurrent launch groups" % groups,
+                     "whole_step": {"bytes_per_particle_step": STEP_BYTES_PER_PARTICLE,
+                                    "achieved": value * STEP_BYTES_PER_PARTICLE / 1e9,
+                                    "frac": value * STEP_BYTES_PER_PARTICLE / 1e9 / HBM_PEAK_GBS}},
     }
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(ac4)
