@@ -173,6 +173,7 @@ static inline int xcd_grid(int nb) { return ((nb + N_XCD - 1) / N_XCD) * N_XCD; 
 // the host for an explicit shuffle), after every component has read it.
 #define SHUF_CH 4096
 #define SHUF_PAR_MIN 16384  // active sets above this take the parallel path (k_shuf_part, k_shuf_apply)
+#define SHUF_TRACE_MAX 2048 // active sets up to this are shuffled by tracing every position back
 __global__ void __launch_bounds__(256) k_shuffle(const RepC *__restrict__ reps, const ChainS *__restrict__ chain,
                                                  int8_t *kx, int8_t *ky, int8_t *kz, int force, int kcap, int skip_large) {
     extern __shared__ unsigned int s_shuf[];  // SHUF_CH partners, then kcap bytes of directions
@@ -190,6 +191,26 @@ __global__ void __launch_bounds__(256) k_shuffle(const RepC *__restrict__ reps, 
     int8_t *k = lds ? s_k : kg;
     if (lds)
         for (int i = threadIdx.x; i < na; i += 256) s_k[i] = kg[i];
+    if (lds && na <= SHUF_TRACE_MAX) {
+        // Every position at once: the swaps s = 0..na-1 in order give the
+        // value at x the one that started at the position found by tracing
+        // x back through them (swap s maps s <-> j_s), so the new array is a
+        // gather of the old one; exactly the serial loop's result, in about
+        // na^2/64 lane operations instead of na dependent LDS round trips.
+        for (int q = threadIdx.x; q < na; q += 256)
+            s_j[q] = (unsigned int)(int)((double)na * ac4_shuffle_uniform(R.key0, R.key1, sh, comp, q));
+        __syncthreads();
+        for (int x = threadIdx.x; x < na; x += 256) {
+            int y = x;
+#pragma unroll 4
+            for (int q = na - 1; q >= 0; q--) {
+                const int jq = (int)s_j[q];
+                y = y == q ? jq : (y == jq ? q : y);
+            }
+            kg[x] = s_k[y];
+        }
+        return;
+    }
     for (int c0 = 0; c0 < na; c0 += SHUF_CH) {
         const int m = min(SHUF_CH, na - c0);
         __syncthreads();
@@ -400,9 +421,8 @@ __device__ __forceinline__ int clampc(int c, int n) { return min(max(c, 0), n - 
 
 // Binning: the reference cell id (bit-exact, for update_cell_list) and the
 // list-grid cell of every particle, with the place within its cell taken by
-// one atomic per distinct cell in the wave (particles arrive in cell order,
-// so a wave holds a few runs of equal cells); the particle is written to
-// that place of its cell's row of CELL_CAP entries (the reference's
+// an atomic on the cell's count; the particle is written to that place of
+// its cell's row of CELL_CAP entries (the reference's
 // InCell[cell][MaxPartInCell], MDcode/global.h:20,85; a fuller cell sets the
 // capacity error).  The last block of a replica to finish turns the cell
 // counts into start offsets (exclusive scan).
@@ -432,19 +452,30 @@ __global__ void __launch_bounds__(BLK) k_bin(const RepC *__restrict__ reps, Chai
         c = cx + cy * R.nc + cz * R.nc * R.nc;
         cell_of[g] = c;
     }
-    const unsigned long long below = (1ull << lane) - 1ull;
-    unsigned long long todo = __ballot(valid);
+    // one returning atomic per particle, all in flight at once: the place
+    // within the cell row depends on the order the atomics land in, which
+    // k_cell_sort removes (it orders each cell by particle id)
+    // Places within the cells: the wave's particles form runs of equal cells
+    // (they are in cell order from the last rebuild, up to the few that
+    // moved); the first lane of each run takes the run's places with one
+    // atomic, all runs at once, and the run's lanes follow in lane order.  A
+    // cell that reappears in another run of the wave gets a second atomic.
+    const int cp = __shfl_up(c, 1, 64);
+    const bool head = valid && (lane == 0 || cp != c);  // c == -1 on invalid lanes
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long vmask = __ballot(valid);
+    const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+    const unsigned long long hu = heads & upto;
+    const int lead = hu ? 63 - __clzll((long long)hu) : lane;  // this lane's run head
     int my_rank = 0;
-    while (todo) {
-        const int leader = __ffsll((long long)todo) - 1;
-        const int cl = __shfl(c, leader, 64);
-        const unsigned long long grp = __ballot(c == cl);
-        int base = 0;
-        if (lane == leader) base = atomicAdd(&cnt[R.off_cell + cl], __popcll(grp));
-        base = __shfl(base, leader, 64);
-        if (c == cl) my_rank = base + __popcll(grp & below);
-        todo &= ~grp;
+    if (head) {
+        // run: from this lane to the next head or the last valid lane
+        const unsigned long long after = heads & ~upto;
+        const int next = after ? __ffsll((long long)after) - 1 : 64 - __clzll((long long)vmask);
+        my_rank = atomicAdd(&cnt[R.off_cell + c], next - lane);
     }
+    const int base = __shfl(my_rank, valid ? lead : lane, 64);
+    if (valid) my_rank = base + (lane - lead);
     if (valid) {
         if (my_rank < CELL_CAP) cell_mem[(size_t)(R.off_cell + c) * CELL_CAP + my_rank] = i;
         else atomicOr(&chain[r].err, 1);
@@ -791,13 +822,36 @@ __device__ __forceinline__ void nlist_team_one(const RepC &R, int r, int i, Chai
                 qs2 = R.qs2;
     int count = 0;
     auto inlist = [&](const uint4 &q) -> bool { return qscreen(pi, q, qs2, cfa, cfb); };
+    // entry p of this particle's list at nlb[(p >> lc) * nC + (p & cm)] (nl_at
+    // relative to entry 0; 24-bit products, n < 2^21 is checked at creation)
+    int *const nlb = nl + nl_at(R, cap, 0, i);
+    const unsigned nC = (unsigned)R.n << R.lc, cm = (1u << R.lc) - 1u;
+    const int lc = R.lc;
+    // a hit's place: the team's hits of lower lanes (the wave's lower hits,
+    // mbcnt, less those of the teams below) after the team's count so far
     auto emit = [&](bool hit, int k) {
-        const unsigned long long m = __ballot(hit) & tmask;
-        if (hit) {
-            const int pos = count + __popcll(m & below);
-            if (pos < cap) nl[nl_at(R, cap, pos, i)] = k;
+        const unsigned long long b = __ballot(hit);
+        const unsigned lo = (unsigned)b, hi = (unsigned)(b >> 32);
+        const int wr = (int)__builtin_amdgcn_mbcnt_hi(hi, __builtin_amdgcn_mbcnt_lo(lo, 0u));
+        int rank, tot;
+        if (NLT == 64) {
+            rank = wr;
+            tot = __popcll(b);
+        } else if (NLT == 32) {
+            const bool up = lane >= 32;
+            const int pl = __popc(lo), ph = __popc(hi);
+            rank = wr - (up ? pl : 0);
+            tot = up ? ph : pl;
+        } else {
+            const unsigned long long m = b & tmask;
+            rank = __popcll(m & below);
+            tot = __popcll(m);
         }
-        count += __popcll(m);
+        if (hit) {
+            const unsigned pos = (unsigned)(count + rank);
+            if (pos < (unsigned)cap) nlb[__umul24(pos >> lc, nC) + (pos & cm)] = k;
+        }
+        count += tot;
     };
     // the home cell's stencil as slot ranges (k_cell_sort), in stencil order:
     // lane t holds ranges t and t + NLT (when NLT < NRANGE), and the team
@@ -816,18 +870,19 @@ __device__ __forceinline__ void nlist_team_one(const RepC &R, int r, int i, Chai
         for (int j = 0; j < na; j++) ct[pa - na + j] = make_int2(ra.x + j * NLT, min(NLT, ra.y - ra.x - j * NLT));
         for (int j = 0; j < nb; j++)
             ct[tot_a + pb - nb + j] = make_int2(rb.x + j * NLT, min(NLT, rb.y - rb.x - j * NLT));
+        if (t < CH_PER) ct[nch + t] = make_int2(0, 0);  // empty chunks up to the next whole round
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // the row is read by the other lanes
         for (int c0 = 0; c0 < nch; c0 += CH_PER) {
             int2 ch[CH_PER];
             uint4 qv[CH_PER];
 #pragma unroll
-            for (int u = 0; u < CH_PER; u++) ch[u] = c0 + u < nch ? ct[c0 + u] : make_int2(0, 0);
+            for (int u = 0; u < CH_PER; u++) ch[u] = ct[c0 + u];
 #pragma unroll
             for (int u = 0; u < CH_PER; u++) qv[u] = posq[t < ch[u].y ? ch[u].x + t : g];
 #pragma unroll
             for (int u = 0; u < CH_PER; u++) {
                 const int k = ch[u].x + t;
-                emit(t < ch[u].y && k != g && inlist(qv[u]), k);
+                emit((t < ch[u].y) & (k != g) & inlist(qv[u]), k);
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // reads done before the row is rewritten
@@ -863,7 +918,7 @@ __global__ void __launch_bounds__(NLT * LPB) k_nlist_team(const RepC *__restrict
                                                           const int *__restrict__ cell_of,
                                                           const int2 *__restrict__ ranges, int *nn, int *nl, int cap,
                                                           void *const *__restrict__ ftab) {
-    __shared__ int2 s_ct[LPB][CH_CAP];  // each team's chunk list
+    __shared__ int2 s_ct[LPB][CH_CAP + CH_PER];  // each team's chunk list, then empty chunks
     const int nit = wl.n[1];
     for (int it = blockIdx.x; it < nit; it += gridDim.x) {
         const int b = wl.item[wl.off_l + it], r = lblk_rep[b];
@@ -1634,6 +1689,8 @@ extern "C" int ac4_md_create_slots(int nslots, const int *cap_n, int precision, 
         memset(&R, 0, sizeof R);
         const int cn = cap_n[r];
         if (cn < 2) { delete md; ac4_set_error("slot %d: capacity %d < 2", r, cn); return AC4_EARG; }
+        // list addressing in the team builders uses 24-bit products of n*8
+        if (cn > (1 << 21)) { delete md; ac4_set_error("slot %d: capacity %d > 2^21", r, cn); return AC4_EARG; }
         R.cap_n = cn;
         R.skin = g_skin;
         R.off = off;

@@ -215,8 +215,37 @@ def test_nhc_trajectory_matches_oracle(gpu):
     assert md.rebuilds(0) >= 1
 
 
-def test_shuffle_follows_reference_swap_rule(gpu):
-    s, L = lattice_state(2000, seed=4)
+@pytest.mark.parametrize("crit", [0, 1])
+def test_rebuild_criterion_matches_oracle(gpu, crit):
+    """Both list-rebuild criteria against the oracle's step loop
+    (time_evolve.c:547-556): the safe 4 d^2 > skin^2 (crit 0) and the
+    reference's 3 d^2 > skin^2 (crit 1, set with
+    ac4_md_set_rebuild_criterion).  Over 300 steps at f0=2 the replica
+    rebuilds exactly as often as the oracle with the same rule, and the
+    trajectories agree as in the plain NHC comparison."""
+    n, T, f0 = 1000, 0.6, 2.0
+    s, L = lattice_state(n, T=T, jitter=0.02, seed=23)
+    s, ch, _ = OL.nhc_steps(s, L, T, f0, 200, _chain0(n, T))
+    s = {k: v for k, v in s.items() if k not in ("fx", "fy", "fz")}
+    md = make_md(gpu, [(s, L)], f0=f0, T=T)
+    md.set_chain(0, ch)
+    md.set_rebuild_criterion(crit)
+    r0 = md.rebuilds(0)
+    md.step(300)
+    g = md.get_state(0)
+    o, _, oout = OL.nhc_steps(s, L, T, f0, 300, ch, safe=1 - crit)
+    assert md.rebuilds(0) - r0 == int(oout[5]) >= 5
+    for k in ("xu", "yu", "zu"):
+        assert np.abs(g[k] - o[k]).max() < 1e-9
+    md.close()
+
+
+@pytest.mark.parametrize("n", [2000, 20480, 20490, 40000])
+def test_shuffle_follows_reference_swap_rule(gpu, n):
+    """200, 2048 and 2049 active particles (the traced shuffle up to 2048,
+    the serial chain above) and 4000: the reference's swap chain, value for
+    value, on the same uniforms."""
+    s, L = lattice_state(n, seed=4)
     md = make_md(gpu, [(s, L)], tau=200)
     before = md.get_state(0)
     u = md.shuffle_uniforms(0)
@@ -226,7 +255,10 @@ def test_shuffle_follows_reference_swap_rule(gpu):
         ref = np.array(before[k], dtype=np.int32)
         OL.load().or_shuffle(len(ref), OL.ip(ref), OL.dp(np.ascontiguousarray(u[c])))
         assert np.array_equal(after[k], ref)
-        assert after[k].sum() == 0 and set(after[k].tolist()) <= {-1, 1}
+        # a permutation: the same number of +1 and -1 (the sum is 0 for an
+        # even active count)
+        assert after[k].sum() == np.sum(before[k]) and set(after[k].tolist()) <= {-1, 1}
+        assert n % 20 != 0 or after[k].sum() == 0
     assert not np.array_equal(after["kx"], before["kx"])
 
 
