@@ -173,7 +173,40 @@ static inline int xcd_grid(int nb) { return ((nb + N_XCD - 1) / N_XCD) * N_XCD; 
 // the host for an explicit shuffle), after every component has read it.
 #define SHUF_CH 4096
 #define SHUF_PAR_MIN 16384  // active sets above this take the parallel path (k_shuf_part, k_shuf_apply)
-#define SHUF_TRACE_MAX 2048 // active sets up to this are shuffled by tracing every position back
+#define SHUF_TRACE_MAX 1024 // active sets up to this are shuffled by tracing every position back
+
+// Lane t traces positions t, t+256, ... (C of them, independent chains side
+// by side) back through the swaps na-1 ... 0; the partners come from LDS four
+// at a time, the next four loaded before the current four are applied.
+template <int C>
+__device__ __forceinline__ void shuffle_trace(const unsigned int *s_j, const int8_t *s_k, int8_t *kg, int na) {
+    int y[C];
+#pragma unroll
+    for (int c = 0; c < C; c++) y[c] = threadIdx.x + 256 * c;
+    int q = na - 1;
+    for (; q >= 0 && ((q + 1) & 3); q--) {  // down to a multiple of four
+        const int jq = (int)s_j[q];
+#pragma unroll
+        for (int c = 0; c < C; c++) y[c] = y[c] == q ? jq : (y[c] == jq ? q : y[c]);
+    }
+    if (q >= 3) {
+        uint4 nx = *(const uint4 *)(s_j + q - 3);
+        for (; q >= 3; q -= 4) {
+            const uint4 cur = nx;
+            if (q >= 7) nx = *(const uint4 *)(s_j + q - 7);
+            const int j4[4] = {(int)cur.w, (int)cur.z, (int)cur.y, (int)cur.x};  // partners of q, q-1, q-2, q-3
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+#pragma unroll
+                for (int c = 0; c < C; c++) y[c] = y[c] == q - u ? j4[u] : (y[c] == j4[u] ? q - u : y[c]);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < C; c++) {
+        const int x = threadIdx.x + 256 * c;
+        if (x < na) kg[x] = s_k[y[c]];
+    }
+}
 __global__ void __launch_bounds__(256) k_shuffle(const RepC *__restrict__ reps, const ChainS *__restrict__ chain,
                                                  int8_t *kx, int8_t *ky, int8_t *kz, int force, int kcap, int skip_large) {
     extern __shared__ unsigned int s_shuf[];  // SHUF_CH partners, then kcap bytes of directions
@@ -200,15 +233,11 @@ __global__ void __launch_bounds__(256) k_shuffle(const RepC *__restrict__ reps, 
         for (int q = threadIdx.x; q < na; q += 256)
             s_j[q] = (unsigned int)(int)((double)na * ac4_shuffle_uniform(R.key0, R.key1, sh, comp, q));
         __syncthreads();
-        for (int x = threadIdx.x; x < na; x += 256) {
-            int y = x;
-#pragma unroll 4
-            for (int q = na - 1; q >= 0; q--) {
-                const int jq = (int)s_j[q];
-                y = y == q ? jq : (y == jq ? q : y);
-            }
-            kg[x] = s_k[y];
-        }
+        const int nch = (na + 255) / 256;  // positions per lane, traced side by side
+        if (nch == 1) shuffle_trace<1>(s_j, s_k, kg, na);
+        else if (nch == 2) shuffle_trace<2>(s_j, s_k, kg, na);
+        else if (nch == 3) shuffle_trace<3>(s_j, s_k, kg, na);
+        else shuffle_trace<4>(s_j, s_k, kg, na);
         return;
     }
     for (int c0 = 0; c0 < na; c0 += SHUF_CH) {

@@ -240,11 +240,11 @@ def test_rebuild_criterion_matches_oracle(gpu, crit):
     md.close()
 
 
-@pytest.mark.parametrize("n", [2000, 20480, 20490, 40000])
+@pytest.mark.parametrize("n", [2000, 7000, 10240, 10250, 40000])
 def test_shuffle_follows_reference_swap_rule(gpu, n):
-    """200, 2048 and 2049 active particles (the traced shuffle up to 2048,
-    the serial chain above) and 4000: the reference's swap chain, value for
-    value, on the same uniforms."""
+    """200, 700 and 1024 active particles (the traced shuffle, one, three and
+    four positions per lane), 1025 and 4000 (the serial chain): the
+    reference's swap chain, value for value, on the same uniforms."""
     s, L = lattice_state(n, seed=4)
     md = make_md(gpu, [(s, L)], tau=200)
     before = md.get_state(0)
