@@ -320,14 +320,16 @@ def cpu_baseline(ac4, n=2485, steps=2000, max_workers=16):
                       "%d steps each, %.1f s wall" % (n, w, steps, wall)}
 
 
-def cfg5(ac4, n=1_000_000, steps=20000, nframes=500, relax=500, ndir=64, seed=5150):
+def cfg5(ac4, n=1_000_000, steps=20000, nframes=500, relax=500, ndir=64, seed=5150, eager=False):
     """Large single box with S4: KA N=10^6, rho=1.2, T=0.5, c=0.1, f0=1.0,
     tau_p=1.0, FP32 forces, 2e4 steps with about 500 log-spaced frames (the
     reference schedule with the smallest origin count that gives that many),
     every frame kept in HBM.  Q(t) and chi4(t) over the reference's
     origin/lag walk (a=0.3) give the peak lag tau*; S4(q, tau*) then follows
     on |q| shells from 0.1 to 7.25 in steps of 2 pi/L with ndir seeded random
-    directions per shell, weighted by the overlap of each origin at tau*."""
+    directions per shell, weighted by the overlap of each origin at tau*.
+    eager: launch every MD step's kernels one by one (for kernel-trace
+    profiles, which are not taken over graph replay)."""
     default_shape(ac4)
     p = ac4.ReplicaParams(n, 1.2, 0.5, 1.0, -1, 200, seed)
     t_all = time.perf_counter()
@@ -345,12 +347,12 @@ def cfg5(ac4, n=1_000_000, steps=20000, nframes=500, relax=500, ndir=64, seed=51
     t = 0
     for f, fi in enumerate(fidx):
         if fi > t:
-            md.step(int(fi - t), graph=True)
+            md.step(int(fi - t), graph=not eager)
             t = int(fi)
         fr.push_md(md, 0, int(fi))
         an.push_md(f, md, 0)
     if t < steps:
-        md.step(steps - t, graph=True)
+        md.step(steps - t, graph=not eager)
     md.sync()
     t_md = time.perf_counter() - t0
     log("cfg5: %d steps with frames in %.1f s" % (steps, t_md))
@@ -380,6 +382,7 @@ def cfg5(ac4, n=1_000_000, steps=20000, nframes=500, relax=500, ndir=64, seed=51
     md.close()
     return {"metric": "cfg5 large box", "n": n, "steps": steps, "frames": int(len(fidx)), "origins": no,
             "frame_store_GB": 24e-9 * n * len(fidx), "md_particle_steps_per_s": n * steps / t_md,
+            "launch": "eager" if eager else "graph",
             "md_seconds": t_md, "chi4_peak": float(chi4[k]), "tau_star": tau_star,
             "Q_at_tau_star": float(qm[k]), "chi4_t": [[float(ov["delta_t"][i]), float(qm[i]), float(chi4[i])]
                                                       for i in range(len(cnt)) if ok[i]],
@@ -697,7 +700,7 @@ def main():
         pmc_probe(ac4)
         return
     if a.cfg5:
-        print(json.dumps(cfg5(ac4)), flush=True)
+        print(json.dumps(cfg5(ac4, eager=a.eager)), flush=True)
         return
     if a.rows:
         print(json.dumps(rows(ac4)), flush=True)
