@@ -121,6 +121,12 @@ _SIGS = {
     "ac4_md_step_graph_multi": (ctypes.c_int, [ctypes.POINTER(_VP), ctypes.c_int, ctypes.c_int]),
     "ac4_format_frame": (ctypes.c_longlong, [ctypes.c_int, _D, _D, _D, _I, ctypes.c_char_p, ctypes.c_longlong]),
     "ac4_parse_frame": (ctypes.c_longlong, [ctypes.c_char_p, ctypes.c_longlong, ctypes.c_int, _D, _D, _D, _I]),
+    "ac4_textframes_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_longlong, ctypes.POINTER(_VP)]),
+    "ac4_textframes_buffer": (ctypes.c_void_p, [_VP, ctypes.POINTER(ctypes.c_longlong)]),
+    "ac4_textframes_parse": (ctypes.c_int, [_VP, ctypes.c_longlong, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                            ctypes.POINTER(ctypes.c_longlong)]),
+    "ac4_textframes_copy": (ctypes.c_int, [_VP, ctypes.c_int, _D, _D, _D, _I]),
+    "ac4_textframes_destroy": (ctypes.c_int, [_VP]),
     "ac4_md_checkpoint_size": (ctypes.c_longlong, [_VP, ctypes.c_int]),
     "ac4_md_checkpoint": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_void_p, ctypes.c_longlong]),
     "ac4_md_restore": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_void_p, ctypes.c_longlong]),
@@ -810,6 +816,41 @@ def parse_frame(text, n):
     if k < 0:
         _check(int(k))
     return x, y, z, t, k
+
+
+class TextFrames:
+    """data_coor.out text parsed on the GPU (ac4_textframes_* of include/ac4.h)."""
+
+    def __init__(self, n, chunk_bytes=0):
+        load()
+        h = _VP()
+        _check(lib.ac4_textframes_create(n, chunk_bytes, ctypes.byref(h)))
+        self.h, self.n = h, n
+        cap = ctypes.c_longlong()
+        self.buf = lib.ac4_textframes_buffer(h, ctypes.byref(cap))
+        self.cap = cap.value
+
+    def close(self):
+        if self.h:
+            lib.ac4_textframes_destroy(self.h)
+            self.h = None
+
+    def parse(self, text, max_frames=1 << 30):
+        """Parses the whole frames at the start of `text` (bytes, at most the
+        chunk size): returns (frames, bytes used)."""
+        if len(text) > self.cap:
+            raise ValueError("text longer than the chunk (%d bytes)" % self.cap)
+        ctypes.memmove(self.buf, text, len(text))
+        nf, used = ctypes.c_int(), ctypes.c_longlong()
+        _check(lib.ac4_textframes_parse(self.h, len(text), max_frames, ctypes.byref(nf), ctypes.byref(used)))
+        return nf.value, used.value
+
+    def frame(self, k):
+        """Frame k of the last parse as host arrays (x, y, z, type)."""
+        x, y, z = np.empty(self.n), np.empty(self.n), np.empty(self.n)
+        t = np.empty(self.n, dtype=np.int32)
+        _check(lib.ac4_textframes_copy(self.h, k, _dp(x), _dp(y), _dp(z), _ip(t)))
+        return x, y, z, t
 
 
 def set_skin(skin=0.35):

@@ -228,6 +228,20 @@ extern "C" int ac4_frames_push(ac4_frames *f, const double *xu, const double *yu
     return AC4_OK;
 }
 
+extern "C" int ac4_frames_push_device(ac4_frames *f, const double *xu, const double *yu, const double *zu,
+                                      long long step) {
+    AC4_REQUIRE(f && xu && yu && zu, AC4_EARG, "frames_push_device: bad arguments");
+    AC4_REQUIRE(f->count < f->cap, AC4_ECAP, "frames_push_device: store full (%d frames)", f->cap);
+    const size_t o = (size_t)f->count * f->n, b = sizeof(double) * f->n;
+    AC4_HIP(hipMemcpyAsync(f->x + o, xu, b, hipMemcpyDeviceToDevice, f->s));
+    AC4_HIP(hipMemcpyAsync(f->y + o, yu, b, hipMemcpyDeviceToDevice, f->s));
+    AC4_HIP(hipMemcpyAsync(f->z + o, zu, b, hipMemcpyDeviceToDevice, f->s));
+    AC4_HIP(hipStreamSynchronize(f->s));
+    f->step.push_back(step);
+    f->count++;
+    return AC4_OK;
+}
+
 extern "C" int ac4_frames_push_md(ac4_frames *f, ac4_md *md, int r, long long step) {
     AC4_REQUIRE(f && md, AC4_EARG, "frames_push_md: bad arguments");
     AC4_REQUIRE(f->count < f->cap, AC4_ECAP, "frames_push_md: store full (%d frames)", f->cap);

@@ -9,7 +9,9 @@
  * is the optional 4th argument.  AC4_DIR overrides the job directory.
  * AC4_BIN=1 reads the frames from data_coor.bin (written by run with
  * AC4_BIN=1: the exact values data_coor.out parses to, so every output file
- * is the same) instead of parsing the text.
+ * is the same) instead of parsing the text.  The text is parsed on the GPU
+ * (ac4_textframes, the same values as the host reader); AC4_HOST_PARSE=1
+ * parses it on the host instead.
  * Observables whose calls Analysis/main.c has commented out: AC4_GUU=1 adds
  * guuTauAlphaAct.dat (guu(factor, tauAlpha), main.c:64; active particles
  * from ActiveFileIndex.dat, read_all_data.c:49-58) and AC4_CAGE=1 adds
@@ -62,6 +64,14 @@ int main(int argc, char **argv) {
     frame_reader rd;
     if (fr_open(&rd, sub_dir, n, env_on("AC4_BIN"))) return 1;
     for (int fr = 0; fr < nframes; fr++) {
+        if (rd.tf) { /* text parsed on the GPU: device arrays */
+            const double *dx, *dy, *dz;
+            const int *dt;
+            if (fr_next_dev(&rd, &dx, &dy, &dz, &dt)) return 1;
+            CK(ac4_ana_push_frame_device(an, fr, dx, dy, dz, dt));
+            if (fs) CK(ac4_frames_push_device(fs, dx, dy, dz, fidx[fr]));
+            continue;
+        }
         if (fr_next(&rd, x, y, z, ty)) return 1;
         CK(ac4_ana_push_frame(an, fr, x, y, z, ty));
         if (fs) CK(ac4_frames_push(fs, x, y, z, fidx[fr]));

@@ -41,6 +41,13 @@ static int one_job(const char *sub_dir, int n, double L, int bin) {
     frame_reader rd;
     if (fr_open(&rd, sub_dir, n, bin)) return 1;
     for (int fr = 0; fr < nframes; fr++) {
+        if (rd.tf) { /* text parsed on the GPU: device arrays */
+            const double *dx, *dy, *dz;
+            const int *dt;
+            if (fr_next_dev(&rd, &dx, &dy, &dz, &dt)) return 1;
+            CK(ac4_frames_push_device(fs, dx, dy, dz, fidx[fr]));
+            continue;
+        }
         if (fr_next(&rd, x, y, z, ty)) return 1;
         CK(ac4_frames_push(fs, x, y, z, fidx[fr]));
     }

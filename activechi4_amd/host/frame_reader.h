@@ -51,6 +51,12 @@ typedef struct {
     int n, bin, frame;
     char *buf;
     size_t cap, have, pos;
+    /* text parsed on the GPU (ac4_textframes): the pinned chunk, its fill,
+     * the bytes the current batch of frames took, and the batch */
+    ac4_textframes *tf;
+    char *tb;
+    long long tcap, thave, tused;
+    int tk, tnf, eof;
 } frame_reader;
 
 static int fr_open(frame_reader *r, const char *sub_dir, int n, int bin) {
@@ -71,6 +77,15 @@ static int fr_open(frame_reader *r, const char *sub_dir, int n, int bin) {
             r->f = NULL;
             return -1;
         }
+    } else if (!(getenv("AC4_HOST_PARSE") && atoi(getenv("AC4_HOST_PARSE")))) {
+        /* the text goes to the GPU in chunks; AC4_HOST_PARSE=1 keeps the host reader */
+        if (ac4_textframes_create(n, 0, &r->tf) != AC4_OK) {
+            fprintf(stderr, "text parser: %s\n", ac4_last_error());
+            fclose(r->f);
+            r->f = NULL;
+            return -1;
+        }
+        r->tb = ac4_textframes_buffer(r->tf, &r->tcap);
     } else {
         r->cap = (size_t)n * 96 + (1u << 20);
         r->buf = malloc(r->cap);
@@ -127,8 +142,46 @@ static int fr_next(frame_reader *r, double *x, double *y, double *z, int *ty) {
     return 0;
 }
 
+/* With the GPU parser (r->tf): the next frame as device arrays, valid until
+ * the call after the last frame of the current batch.  0 on success. */
+static int fr_next_dev(frame_reader *r, const double **x, const double **y, const double **z, const int **ty) {
+    while (r->tk == r->tnf) { /* the batch is used up: parse the next one */
+        if (r->tused) {
+            memmove(r->tb, r->tb + r->tused, (size_t)(r->thave - r->tused));
+            r->thave -= r->tused;
+            r->tused = 0;
+        }
+        if (!r->eof && r->thave < r->tcap) {
+            const size_t got = fread(r->tb + r->thave, 1, (size_t)(r->tcap - r->thave), r->f);
+            r->thave += (long long)got;
+            if (got == 0) r->eof = 1;
+        }
+        /* a last line without its newline still counts (as for ac4_parse_frame) */
+        if (r->eof && r->thave > 0 && r->tb[r->thave - 1] != '\n' && r->thave < r->tcap) r->tb[r->thave++] = '\n';
+        int nf = 0;
+        long long used = 0;
+        if (ac4_textframes_parse(r->tf, r->thave, 1 << 30, &nf, &used) != AC4_OK) {
+            fprintf(stderr, "frame %d: %s\n", r->frame, ac4_last_error());
+            return -1;
+        }
+        if (nf > 0) {
+            r->tnf = nf;
+            r->tk = 0;
+            r->tused = used;
+        } else if (r->eof || r->thave == r->tcap) {
+            fprintf(stderr, "frame data ends in frame %d\n", r->frame);
+            return -1;
+        }
+    }
+    if (ac4_textframes_get(r->tf, r->tk, x, y, z, ty) != AC4_OK) return -1;
+    r->tk++;
+    r->frame++;
+    return 0;
+}
+
 static void fr_close(frame_reader *r) {
     if (r->f) fclose(r->f);
+    if (r->tf) ac4_textframes_destroy(r->tf);
     free(r->buf);
     memset(r, 0, sizeof *r);
 }

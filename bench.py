@@ -396,7 +396,8 @@ def rows(ac4, n=50000, n_origins=20, gap=100, seed=777):
     """The analysis rows beside the MD step, each on the GPU (libac4) and on
     the oracle (one host core) over the same input:
       - trajectory text I/O of one N=10^6 frame (ac4_format_frame /
-        ac4_parse_frame against glibc sprintf / strtod per line);
+        ac4_parse_frame, and the device parser ac4_textframes, against glibc
+        sprintf / strtod per line);
       - block analysis (ac4_frames_block, Analysis_Block: MSD, overlap and van
         Hove per block) for the 17 block edges L/3 ... L/19; the oracle runs
         the edge L/3 and the GPU is timed on that edge too;
@@ -436,6 +437,20 @@ def rows(ac4, n=50000, n_origins=20, gap=100, seed=777):
     xp, yp, zp, tp, used = ac4.parse_frame(txt, m)
     t_parse = time.perf_counter() - t0
     assert used == len(txt) and np.array_equal(xp, xs) and np.array_equal(tp, ts) and np.array_equal(zp, zs)
+    # the same text parsed on the GPU (ac4_textframes: the chunk copied from
+    # pinned memory, line ends found, one line per thread); timed from the
+    # text in the pinned chunk to the parsed frame on the device
+    tf = ac4.TextFrames(m)
+    ctypes.memmove(tf.buf, txt, len(txt))
+    nfr, usedp = ctypes.c_int(), ctypes.c_longlong()
+    ac4.lib.ac4_textframes_parse(tf.h, len(txt), 1, ctypes.byref(nfr), ctypes.byref(usedp))  # first launches
+    t0 = time.perf_counter()
+    rc = ac4.lib.ac4_textframes_parse(tf.h, len(txt), 1, ctypes.byref(nfr), ctypes.byref(usedp))
+    t_gparse = time.perf_counter() - t0
+    assert rc == 0 and nfr.value == 1 and usedp.value == len(txt)
+    xg, yg, zg, tg = tf.frame(0)
+    tf.close()
+    assert np.array_equal(xg, xs) and np.array_equal(yg, ys) and np.array_equal(zg, zs) and np.array_equal(tg, ts)
     # the same frame as the unwrapped positions of an MD handle, formatted on
     # the device (ac4_md_format_frame: what bin/run writes per frame)
     md = ac4.MD([ac4.ReplicaParams(m, 1.2, 0.6, 1.5, -1, 0, 1)])
@@ -451,10 +466,12 @@ def rows(ac4, n=50000, n_origins=20, gap=100, seed=777):
     assert dev == txt
     out["trajectory_io"] = {"frame_bytes": len(txt), "format_s": t_fmt, "sprintf_s": t_printf,
                             "device_format_s": t_dev, "parse_s": t_parse, "strtod_s": t_strtod,
+                            "device_parse_s": t_gparse,
                             "format_GBps": len(txt) / t_fmt / 1e9, "device_format_GBps": len(txt) / t_dev / 1e9,
-                            "parse_GBps": len(txt) / t_parse / 1e9}
+                            "parse_GBps": len(txt) / t_parse / 1e9, "device_parse_GBps": len(txt) / t_gparse / 1e9}
     log("rows: frame text %.1f MB: format %.3f s on the host, %.4f s on the device (sprintf %.3f s), "
-        "parse %.3f s (strtod %.3f s)" % (len(txt) / 1e6, t_fmt, t_dev, t_printf, t_parse, t_strtod))
+        "parse %.3f s on the host, %.4f s on the device (strtod %.3f s)"
+        % (len(txt) / 1e6, t_fmt, t_dev, t_printf, t_parse, t_gparse, t_strtod))
     # trajectory
     frames = trajgen.case_frames(n_origins, gap)
     X, Y, Z, _ = trajgen.make_frames(n, frames, seed, 0.06, 0.5)

@@ -175,6 +175,8 @@ int ac4_frames_destroy(ac4_frames *f);
 int ac4_frames_push(ac4_frames *f, const double *xu, const double *yu, const double *zu, long long step);
 /* Appends replica r's current x_u (device to device; synchronises the MD stream). */
 int ac4_frames_push_md(ac4_frames *f, ac4_md *md, int r, long long step);
+/* Appends a frame from device arrays (e.g. one ac4_textframes_parse made). */
+int ac4_frames_push_device(ac4_frames *f, const double *xu, const double *yu, const double *zu, long long step);
 int ac4_frames_count(const ac4_frames *f);
 /* Index of the frame pushed with this step number, or -1. */
 int ac4_frames_find(const ac4_frames *f, long long step);
@@ -487,6 +489,24 @@ long long ac4_format_frame(int n, const double *x, const double *y, const double
 /* Reads n such lines (Analysis/read_all_data.c:61-91); every number equals
  * strtod's.  Returns the bytes consumed or a negative error. */
 long long ac4_parse_frame(const char *text, long long len, int n, double *x, double *y, double *z, int *type);
+/* The same reading of data_coor.out on the GPU (replaces the fscanf loop of
+ * LoadFramePP, Analysis/read_all_data.c:65-100, and its Analysis_Block
+ * twin): the caller reads the file into the handle's pinned chunk
+ * (ac4_textframes_buffer, chunk_bytes of it; 0 -> 64 MiB, at least 128 bytes
+ * per line of a frame), then ac4_textframes_parse converts the whole frames
+ * of n lines in its first len bytes, at most max_frames of them: *nframes
+ * frames, *used bytes consumed (the caller keeps the rest for the next
+ * call).  Values equal ac4_parse_frame's.  ac4_textframes_get gives frame k
+ * of the last call as device arrays, valid until the next call. */
+typedef struct ac4_textframes ac4_textframes;
+int ac4_textframes_create(int n, long long chunk_bytes, ac4_textframes **out);
+char *ac4_textframes_buffer(ac4_textframes *t, long long *cap);
+int ac4_textframes_parse(ac4_textframes *t, long long len, int max_frames, int *nframes, long long *used);
+int ac4_textframes_get(ac4_textframes *t, int k, const double **x, const double **y, const double **z,
+                       const int **type);
+/* Frame k of the last call copied to host arrays of n. */
+int ac4_textframes_copy(ac4_textframes *t, int k, double *x, double *y, double *z, int *type);
+int ac4_textframes_destroy(ac4_textframes *t);
 /* read_all_data.c:36-48 */
 int ac4_origin_gap(const int *fidx, int nframes);
 /* InitialiseNoseHoover (MDcode/time_evolve.c:476-493) with a seeded draw:

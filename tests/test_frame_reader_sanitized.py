@@ -27,7 +27,8 @@ def harness(tmp_path_factory, ac4):
 
 
 def run(exe, d, n, header=1, binary=0):
-    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1", UBSAN_OPTIONS="print_stacktrace=1")
+    # the host reader (the GPU text parser has its own tests)
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1", UBSAN_OPTIONS="print_stacktrace=1", AC4_HOST_PARSE="1")
     env.pop("LD_PRELOAD", None)
     p = subprocess.run([exe, str(d), str(n), str(header), str(binary)], capture_output=True, text=True, env=env,
                        timeout=120)
