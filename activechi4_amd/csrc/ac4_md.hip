@@ -450,8 +450,8 @@ __device__ __forceinline__ int clampc(int c, int n) { return min(max(c, 0), n - 
 
 // Binning: the reference cell id (bit-exact, for update_cell_list) and the
 // list-grid cell of every particle, with the place within its cell taken by
-// an atomic on the cell's count; the particle is written to that place of
-// its cell's row of CELL_CAP entries (the reference's
+// an atomic on the cell's count; the particle's (slot, id) is written to
+// that place of its cell's row of CELL_CAP entries (the reference's
 // InCell[cell][MaxPartInCell], MDcode/global.h:20,85; a fuller cell sets the
 // capacity error).  The last block of a replica to finish turns the cell
 // counts into start offsets (exclusive scan).
@@ -459,7 +459,8 @@ __device__ __forceinline__ int clampc(int c, int n) { return min(max(c, 0), n - 
 __global__ void __launch_bounds__(BLK) k_bin(const RepC *__restrict__ reps, ChainS *chain,
                                              const int *__restrict__ blk_rep, const int *__restrict__ blk_first,
                                              const double *__restrict__ x, const double *__restrict__ y,
-                                             const double *__restrict__ z, int *cnt, int *cell_of, int *cell_mem,
+                                             const double *__restrict__ z, const int *__restrict__ id, int *cnt,
+                                             int *cell_of, int2 *cell_mem,
                                              int *cell_ref, int *start, WorkList wl) {
     __shared__ int s_last, s_w[BLK / 64], s_base[3];
     const int b = blockIdx.x, r = blk_rep[b];
@@ -506,7 +507,7 @@ __global__ void __launch_bounds__(BLK) k_bin(const RepC *__restrict__ reps, Chai
     const int base = __shfl(my_rank, valid ? lead : lane, 64);
     if (valid) my_rank = base + (lane - lead);
     if (valid) {
-        if (my_rank < CELL_CAP) cell_mem[(size_t)(R.off_cell + c) * CELL_CAP + my_rank] = i;
+        if (my_rank < CELL_CAP) cell_mem[(size_t)(R.off_cell + c) * CELL_CAP + my_rank] = make_int2(i, id[g]);
         else atomicOr(&chain[r].err, 1);
     }
     __syncthreads();
@@ -622,7 +623,7 @@ __device__ __forceinline__ void gather_one(const PartArrays &src, const PartArra
 #define SORT_CAP 256
 __global__ void __launch_bounds__(BLK) k_cell_sort(const RepC *__restrict__ reps, WorkList wl,
                                                    const int *__restrict__ cnt, const int *__restrict__ start,
-                                                   const int *__restrict__ id, const int *__restrict__ cell_mem,
+                                                   const int *__restrict__ id, const int2 *__restrict__ cell_mem,
                                                    int *members, int2 *ranges, PartArrays src, PartArrays dst,
                                                    PosCopies pc) {
     __shared__ int s_m[BLK / 64][SORT_CAP];
@@ -655,10 +656,10 @@ __global__ void __launch_bounds__(BLK) k_cell_sort(const RepC *__restrict__ reps
         int *m = members + R.off + first;
         // the cell's members in binning order (a cell beyond CELL_CAP has set
         // the capacity error; only its first CELL_CAP are moved)
-        const int *row = cell_mem + (size_t)(R.off_cell + c) * CELL_CAP;
+        const int2 *row = cell_mem + (size_t)(R.off_cell + c) * CELL_CAP;
         const int n = min(cnt[R.off_cell + c], CELL_CAP);
         if (n <= SORT_CAP) {
-            for (int k = lane; k < n; k += 64) { const int v = row[k]; s_m[w][k] = v; s_k[w][k] = id[R.off + v]; }
+            for (int k = lane; k < n; k += 64) { const int2 v = row[k]; s_m[w][k] = v.x; s_k[w][k] = v.y; }
             __syncwarp();
             for (int k = lane; k < n; k += 64) {
                 const int v = s_k[w][k];
@@ -669,7 +670,7 @@ __global__ void __launch_bounds__(BLK) k_cell_sort(const RepC *__restrict__ reps
             }
             __syncwarp();
         } else {
-            for (int k = lane; k < n; k += 64) m[k] = row[k];
+            for (int k = lane; k < n; k += 64) m[k] = row[k].x;
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
             __syncwarp();
             if (lane == 0) {  // crowded cell: serial insertion sort on the key
@@ -1561,8 +1562,8 @@ struct ac4_md {
     double *fx = nullptr, *fy = nullptr, *fz = nullptr, *xo = nullptr, *yo = nullptr, *zo = nullptr;
     double4 *pos4 = nullptr;
     uint4 *posq = nullptr;
-    int *cell_mem = nullptr, *nn = nullptr, *nn_ref = nullptr, *nl = nullptr, *cnt = nullptr, *start = nullptr,
-        *members = nullptr;
+    int2 *cell_mem = nullptr;  // per cell: CELL_CAP (slot, id) of its members in binning order
+    int *nn = nullptr, *nn_ref = nullptr, *nl = nullptr, *cnt = nullptr, *start = nullptr, *members = nullptr;
     int8_t *kx = nullptr, *ky = nullptr, *kz = nullptr;
     double *ex = nullptr, *ey = nullptr, *ez = nullptr;  // reference-order export
     // in-situ overlap accumulators
@@ -1973,7 +1974,7 @@ extern "C" long long ac4_md_slot_bytes(int cap_n, int n_origin, int n_lags) {
     b += n * (6 * 8 + 32 + 16 + 2 * 4 + 192LL * 4 + 4 + 3);         // forces, xold, pos copies, counts, list, members, k
     b += 2 * n * (9 * 8 + 5 * 4);                                    // particle arrays and their scratch copy
     b += n * (3 * 8 + 4);                                            // reference-order export
-    b += cells * ((long long)NRANGE * 8 + 2 * 4 + (long long)CELL_CAP * 4 + 2 * 4);
+    b += cells * ((long long)NRANGE * 8 + 2 * 4 + (long long)CELL_CAP * 8 + 2 * 4);
     b += nblk * 3 * 4 + fnb * 2 * 4 + lnblk * 2 * 4;                // block tables
     b += (nblk + lnblk + ncb) * 4 + fnb * 4 * 8;                     // work list, force partials
     b += (long long)sizeof(RepC) + (long long)sizeof(ChainS);
@@ -2514,7 +2515,8 @@ static int launch_rebuild(ac4_md *md, bool with_zero = false) {
     if (with_zero)
         hipLaunchKernelGGL(k_cell_zero, dim3(md->R, cy), dim3(BLK), 0, s, md->reps, md->chain, md->cnt, md->wl_n);
     hipLaunchKernelGGL(k_bin, dim3(md->nblk), dim3(BLK), 0, s, md->reps, md->chain, md->blk_rep, md->blk_first,
-                       md->P.x, md->P.y, md->P.z, md->cnt, md->P.cell_of, md->cell_mem, md->P.cell_ref, md->start, wl);
+                       md->P.x, md->P.y, md->P.z, md->P.id, md->cnt, md->P.cell_of, md->cell_mem, md->P.cell_ref,
+                       md->start, wl);
     hipLaunchKernelGGL(k_cell_sort, dim3(gc), dim3(BLK), 0, s, md->reps, wl, md->cnt, md->start, md->P.id,
                        md->cell_mem, md->members, md->ranges, md->P, md->A,
                        PosCopies{md->pos4, md->posq, md->xo, md->yo, md->zo});
