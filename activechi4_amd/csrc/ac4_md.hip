@@ -57,17 +57,27 @@ struct RepC {
     float qs2, pad3; // qscale^2 in FP32
 };
 
-// Fixed-point copy of a wrapped position for the FP32 force path: each box
-// fraction s in [0,1) as the 32-bit integer floor(s * 2^32) (quantum L/2^32,
-// below 1e-8 sigma for L < 40), type in w.  The 32-bit difference of two
-// such coordinates, read as a signed integer, is the minimum-image separation
-// exactly (cell_list+forces.c:92-97 up to the tie at exactly half a box), and
-// a neighbour costs one 16-byte load instead of two.
-__host__ __device__ __forceinline__ uint4 qpos(double sx, double sy, double sz, int t) {
-    return make_uint4((unsigned int)(unsigned long long)(sx * 4294967296.0),
-                      (unsigned int)(unsigned long long)(sy * 4294967296.0),
-                      (unsigned int)(unsigned long long)(sz * 4294967296.0), (unsigned int)t);
+// Fixed-point copy of a wrapped position for the FP32 force path and the
+// dynamics lists: each box fraction s in [0,1) as the 32-bit integer
+// floor(s * 2^32) (quantum L/2^32, below 1e-8 sigma for L < 40), with the
+// species (0 or 1) in the lowest bit of x.  The 32-bit difference of two such
+// coordinates, read as a signed integer, is the minimum-image separation
+// (cell_list+forces.c:92-97 up to the tie at exactly half a box), exact in y
+// and z and within one quantum in x.  A neighbour costs one 12-byte load: 64
+// lanes gathering 12-byte records touch fewer 128-byte lines than with
+// 16-byte ones, and the gathers are what bound the force kernel.
+struct QPos {
+    unsigned int x, y, z;
+};
+__host__ __device__ __forceinline__ QPos qpos(double sx, double sy, double sz, int t) {
+    QPos q;
+    q.x = ((unsigned int)(unsigned long long)(sx * 4294967296.0) & ~1u) | (unsigned int)(t & 1);
+    q.y = (unsigned int)(unsigned long long)(sy * 4294967296.0);
+    q.z = (unsigned int)(unsigned long long)(sz * 4294967296.0);
+    return q;
 }
+__host__ __device__ __forceinline__ int ptype(const QPos &q) { return (int)(q.x & 1u); }
+__host__ __device__ __forceinline__ int ptype(const double4 &q) { return (int)q.w; }
 
 struct ChainS {
     double eta[3], peta[3], q[3];
@@ -363,7 +373,7 @@ __global__ void __launch_bounds__(BLK) k_first_half(const RepC *__restrict__ rep
                                                     const double *__restrict__ fy, const double *__restrict__ fz,
                                                     const double *__restrict__ xo, const double *__restrict__ yo,
                                                     const double *__restrict__ zo, const int *__restrict__ type,
-                                                    double4 *pos4, uint4 *posq, int q32, ChainS *chain_w,
+                                                    double4 *pos4, QPos *posq, int q32, ChainS *chain_w,
                                                     const int *__restrict__ blk_idx, int *cnt, int *wl_n) {
     __shared__ double s_scale[2];
     __shared__ double s_max[BLK / 64];
@@ -592,7 +602,7 @@ struct PartArrays {  // particle arrays that move with a particle
 // the position copies the list builders and the force read, at a new slot
 struct PosCopies {
     double4 *pos4;
-    uint4 *posq;
+    QPos *posq;
     double *xo, *yo, *zo;  // x_u at this rebuild (update_cell_list, cell_list+forces.c:141-146)
 };
 
@@ -733,9 +743,9 @@ __device__ __forceinline__ size_t nl_at(const RepC &R, int cap, int k, int i) {
 // FP32 against the list radius plus a margin of 1e-3 sigma^2, far above the
 // FP32 error of r^2 (below 1e-5 for the list radii), so no pair within its
 // list radius is lost.
-__device__ __forceinline__ bool qscreen(const uint4 &pi, const uint4 &q, float qs2, float cfa, float cfb) {
+__device__ __forceinline__ bool qscreen(const QPos &pi, const QPos &q, float qs2, float cfa, float cfb) {
     const float dx = (float)(int)(pi.x - q.x), dy = (float)(int)(pi.y - q.y), dz = (float)(int)(pi.z - q.z);
-    return (dx * dx + dy * dy + dz * dz) * qs2 < (q.w == 0u ? cfa : cfb);
+    return (dx * dx + dy * dy + dz * dz) * qs2 < (ptype(q) == 0 ? cfa : cfb);
 }
 
 #define NL_UNROLL 8
@@ -750,7 +760,7 @@ __device__ __forceinline__ bool qscreen(const uint4 &pi, const uint4 &q, float q
 template <bool EXACT>
 __global__ void __launch_bounds__(BLK) k_nlist(const RepC *__restrict__ reps, ChainS *chain,
                                                const int *__restrict__ blk_rep, const int *__restrict__ blk_first,
-                                               const double4 *__restrict__ pos4, const uint4 *__restrict__ posq,
+                                               const double4 *__restrict__ pos4, const QPos *__restrict__ posq,
                                                const int *__restrict__ cell_of, const int *__restrict__ cnt,
                                                const int *__restrict__ start, int *nn, int *nn_ref, int *nl,
                                                int stride, int cap) {
@@ -761,7 +771,7 @@ __global__ void __launch_bounds__(BLK) k_nlist(const RepC *__restrict__ reps, Ch
     if (i >= R.n) return;
     const int g = R.off + i, nc = R.nc;
     const double4 pi = pos4[g];
-    const uint4 pq = posq[g];
+    const QPos pq = posq[g];
     const int ti = (int)pi.w;
     const int c = cell_of[g];
     const int cx = c % nc, cy = (c / nc) % nc, cz = c / (nc * nc);
@@ -797,7 +807,7 @@ __global__ void __launch_bounds__(BLK) k_nlist(const RepC *__restrict__ reps, Ch
                             }
                         }
                     } else {
-                        uint4 q[NL_UNROLL];
+                        QPos q[NL_UNROLL];
 #pragma unroll
                         for (int u = 0; u < NL_UNROLL; u++) q[u] = posq[min(k + u, e2 - 1)];
 #pragma unroll
@@ -838,20 +848,20 @@ __global__ void __launch_bounds__(BLK) k_nlist(const RepC *__restrict__ reps, Ch
 #define CH_PER 4
 template <int NLT>
 __device__ __forceinline__ void nlist_team_one(const RepC &R, int r, int i, ChainS *chain,
-                                               const uint4 *__restrict__ posq, const int *__restrict__ cell_of,
+                                               const QPos *__restrict__ posq, const int *__restrict__ cell_of,
                                                const int2 *__restrict__ ranges, int *nn, int *nl, int cap,
                                                int2 *ct) {
     const int lane = threadIdx.x & 63, t = lane & (NLT - 1);
     const unsigned long long tmask = (NLT == 64 ? ~0ull : ((1ull << (NLT & 63)) - 1ull)) << (lane & ~(NLT - 1));
     const unsigned long long below = (1ull << lane) - 1ull;
     const int g = R.off + i;
-    const uint4 pi = posq[g];
-    const int ti = (int)pi.w;
+    const QPos pi = posq[g];
+    const int ti = ptype(pi);
     const int2 *rgs = ranges + (size_t)(R.off_cell + cell_of[g]) * NRANGE;
     const float cfa = (float)list_cut2(ti, R.skin) + 1e-3f, cfb = (float)list_cut2(ti + 1, R.skin) + 1e-3f,
                 qs2 = R.qs2;
     int count = 0;
-    auto inlist = [&](const uint4 &q) -> bool { return qscreen(pi, q, qs2, cfa, cfb); };
+    auto inlist = [&](const QPos &q) -> bool { return qscreen(pi, q, qs2, cfa, cfb); };
     // entry p of this particle's list at nlb[(p >> lc) * nC + (p & cm)] (nl_at
     // relative to entry 0; 24-bit products, n < 2^21 is checked at creation)
     int *const nlb = nl + nl_at(R, cap, 0, i);
@@ -904,7 +914,7 @@ __device__ __forceinline__ void nlist_team_one(const RepC &R, int r, int i, Chai
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // the row is read by the other lanes
         for (int c0 = 0; c0 < nch; c0 += CH_PER) {
             int2 ch[CH_PER];
-            uint4 qv[CH_PER];
+            QPos qv[CH_PER];
 #pragma unroll
             for (int u = 0; u < CH_PER; u++) ch[u] = ct[c0 + u];
 #pragma unroll
@@ -924,7 +934,7 @@ __device__ __forceinline__ void nlist_team_one(const RepC &R, int r, int i, Chai
             // (indices clamped into [s2, e2), validity applied after)
             for (int k0 = s2; k0 < e2; k0 += 2 * NLT) {
                 const int ka = k0 + t, kb = k0 + NLT + t;
-                const uint4 qa = posq[min(ka, e2 - 1)], qb = posq[min(kb, e2 - 1)];
+                const QPos qa = posq[min(ka, e2 - 1)], qb = posq[min(kb, e2 - 1)];
                 const bool ha = inlist(qa) && ka < e2 && ka != g, hb = inlist(qb) && kb < e2 && kb != g;
                 emit(ha, ka);
                 emit(hb, kb);
@@ -944,7 +954,7 @@ template <int NLT, bool MOVE>
 __global__ void __launch_bounds__(NLT * LPB) k_nlist_team(const RepC *__restrict__ reps, ChainS *chain,
                                                           const int *__restrict__ lblk_rep,
                                                           const int *__restrict__ lblk_first, WorkList wl,
-                                                          const uint4 *__restrict__ posq,
+                                                          const QPos *__restrict__ posq,
                                                           const int *__restrict__ cell_of,
                                                           const int2 *__restrict__ ranges, int *nn, int *nl, int cap,
                                                           void *const *__restrict__ ftab) {
@@ -985,9 +995,9 @@ __global__ void __launch_bounds__(NLT * LPB) k_nlist_team(const RepC *__restrict
 #define NLC_CAP 2304
 __global__ void __launch_bounds__(64) k_nlist_cell(const RepC *__restrict__ reps, ChainS *chain,
                                                    const int *__restrict__ cblk_rep, const int *__restrict__ cblk_cell,
-                                                   const uint4 *__restrict__ posq, const int *__restrict__ cnt,
+                                                   const QPos *__restrict__ posq, const int *__restrict__ cnt,
                                                    const int *__restrict__ start, int *nn, int *nl, int cap) {
-    __shared__ uint4 s_c[NLC_CAP];
+    __shared__ QPos s_c[NLC_CAP];
     __shared__ int s_pre[28], s_st[27];
     const int r = cblk_rep[blockIdx.x], c = cblk_cell[blockIdx.x];
     const RepC R = reps[r];
@@ -1030,8 +1040,8 @@ __global__ void __launch_bounds__(64) k_nlist_cell(const RepC *__restrict__ reps
     const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     int mine = 0;  // list length of home particle `lane` (and lane+64, ... kept below)
     for (int h = 0; h < hn; h++) {
-        const uint4 pi = s_c[hoff + h];
-        const int ti = (int)pi.w, kh = hoff + h;
+        const QPos pi = s_c[hoff + h];
+        const int ti = ptype(pi), kh = hoff + h;
         const float cfa = (float)list_cut2(ti, R.skin) + 1e-3f, cfb = (float)list_cut2(ti + 1, R.skin) + 1e-3f;
         int base = 0;
         for (int k0 = 0; k0 < total; k0 += 64) {
@@ -1278,11 +1288,11 @@ struct PairAcc<true> {
             c2i[tj] = (float)(K_C2 / (sig * sig));  // C2 / sigma^2
         }
     }
-    __device__ __forceinline__ void add(const uint4 &pi, int, const uint4 &pj, const RepC &) {
+    __device__ __forceinline__ void add(const QPos &pi, int, const QPos &pj, const RepC &) {
         const float dx = (float)(int)(pi.x - pj.x);
         const float dy = (float)(int)(pi.y - pj.y);
         const float dz = (float)(int)(pi.z - pj.z);
-        const bool b = pj.w != 0u;
+        const bool b = (pj.x & 1u) != 0u;
         const float rq = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));  // (L/2^32)^2 units
         if (rq < (b ? rcq[1] : rcq[0])) {
             const float r2 = rq * qs2;  // real units
@@ -1361,7 +1371,7 @@ __device__ __forceinline__ void walk_list1(const RepC &R, const int *lp, int m, 
 template <bool FP32, bool KICK>
 __global__ void __launch_bounds__(BLK) k_force(const RepC *__restrict__ reps, ChainS *chain,
                                                const int *__restrict__ fblk_rep, const int *__restrict__ fblk_first,
-                                               const double4 *__restrict__ pos4, const uint4 *__restrict__ posq,
+                                               const double4 *__restrict__ pos4, const QPos *__restrict__ posq,
                                                const int *__restrict__ nn,
                                                const int *__restrict__ nl, int cap, const int *__restrict__ act,
                                                const int8_t *__restrict__ kx, const int8_t *__restrict__ ky,
@@ -1380,11 +1390,11 @@ __global__ void __launch_bounds__(BLK) k_force(const RepC *__restrict__ reps, Ch
     const bool valid = i < R.n;
     const int g = R.off + (valid ? i : 0);
     // FP64: the wrapped box fractions; FP32: their fixed-point copy
-    using PosT = typename std::conditional<FP32, uint4, double4>::type;
+    using PosT = typename std::conditional<FP32, QPos, double4>::type;
     const PosT *__restrict__ pos = (const PosT *)(FP32 ? (const void *)posq : (const void *)pos4);
     if (valid) {
         const PosT pi = pos[g];
-        const int ti = (int)pi.w;
+        const int ti = ptype(pi);
         PairAcc<FP32> acc(ti, R);
         const int m = nn[g];
         if (R.tpp == 1) {
@@ -1561,7 +1571,7 @@ struct ac4_md {
     void **ftab = nullptr;  // their pointers on the device (PART_FIELDS of P, then of A)
     double *fx = nullptr, *fy = nullptr, *fz = nullptr, *xo = nullptr, *yo = nullptr, *zo = nullptr;
     double4 *pos4 = nullptr;
-    uint4 *posq = nullptr;
+    QPos *posq = nullptr;
     int2 *cell_mem = nullptr;  // per cell: CELL_CAP (slot, id) of its members in binning order
     int *nn = nullptr, *nn_ref = nullptr, *nl = nullptr, *cnt = nullptr, *start = nullptr, *members = nullptr;
     int8_t *kx = nullptr, *ky = nullptr, *kz = nullptr;
@@ -1971,7 +1981,7 @@ extern "C" long long ac4_md_slot_bytes(int cap_n, int n_origin, int n_lags) {
     const long long fnb = (n * 16 + BLK - 1) / BLK;  // force blocks at the widest team (16 lanes a particle)
     const long long ncb = (cells + BLK / 64 - 1) / (BLK / 64);
     long long b = 0;
-    b += n * (6 * 8 + 32 + 16 + 2 * 4 + 192LL * 4 + 4 + 3);         // forces, xold, pos copies, counts, list, members, k
+    b += n * (6 * 8 + 32 + 12 + 2 * 4 + 192LL * 4 + 4 + 3);         // forces, xold, pos copies, counts, list, members, k
     b += 2 * n * (9 * 8 + 5 * 4);                                    // particle arrays and their scratch copy
     b += n * (3 * 8 + 4);                                            // reference-order export
     b += cells * ((long long)NRANGE * 8 + 2 * 4 + (long long)CELL_CAP * 8 + 2 * 4);
@@ -2093,7 +2103,7 @@ static int set_state_impl(ac4_md *md, int r, const double *x, const double *y, c
     const RepC &R = md->reps_h[r];
     const int n = R.n, g = R.off;
     std::vector<double4> p4(n);
-    std::vector<uint4> q4(n);
+    std::vector<QPos> q4(n);
     std::vector<int> ids(n);
     double keblk = 0.0;  // time_evolve.c:586-589 order
     for (int i = 0; i < n; i++) {
@@ -2941,7 +2951,7 @@ static size_t ck_ov_bytes(const ac4_md *md, const RepC &R) {
 }
 static size_t ck_bytes(const ac4_md *md, const RepC &R) {
     const size_t n = R.n, nc = R.ncell;
-    return sizeof(CkHead) + sizeof(ChainS) + n * (15 * sizeof(double) + sizeof(double4) + sizeof(uint4)) +
+    return sizeof(CkHead) + sizeof(ChainS) + n * (15 * sizeof(double) + sizeof(double4) + sizeof(QPos)) +
            n * 7 * sizeof(int) + 3 * (size_t)R.nact + nc * 2 * sizeof(int) + n * sizeof(int) +
            nc * NRANGE * sizeof(int2) + (size_t)md->cap * n * sizeof(int) + ck_ov_bytes(md, R);
 }
@@ -3014,7 +3024,7 @@ extern "C" int ac4_md_checkpoint(ac4_md *md, int r, void *buf, long long cap) {
     CkHead h;
     memset(&h, 0, sizeof h);
     memcpy(h.magic, k_ck_magic, 8);
-    h.version = 2; h.n = R.n; h.nact = R.nact; h.ncell = R.ncell; h.cap = md->cap; h.lc = R.lc; h.tpp = R.tpp;
+    h.version = 3; h.n = R.n; h.nact = R.nact; h.ncell = R.ncell; h.cap = md->cap; h.lc = R.lc; h.tpp = R.tpp;
     h.prec = md->prec; h.L = R.L; h.skin = R.skin; h.T = R.T; h.f0 = R.f0; h.tau = R.tau;
     h.seed = (uint64_t)R.key0 | ((uint64_t)R.key1 << 32);
     h.chain_bytes = sizeof(ChainS);
@@ -3039,7 +3049,7 @@ extern "C" int ac4_md_restore(ac4_md *md, int r, const void *buf, long long len)
     AC4_REQUIRE(buf && len >= (long long)sizeof(CkHead), AC4_EARG, "restore: no checkpoint");
     CkHead h;
     memcpy(&h, buf, sizeof h);
-    AC4_REQUIRE(!memcmp(h.magic, k_ck_magic, 8) && h.version == 2 && h.chain_bytes == sizeof(ChainS), AC4_EARG,
+    AC4_REQUIRE(!memcmp(h.magic, k_ck_magic, 8) && h.version == 3 && h.chain_bytes == sizeof(ChainS), AC4_EARG,
                 "restore: not a checkpoint of this library version");
     AC4_REQUIRE((long long)h.total == len && h.total == ck_bytes(md, R), AC4_EARG,
                 "restore: checkpoint of %lld bytes does not fit slot %d", len, r);
@@ -3062,7 +3072,7 @@ extern "C" int ac4_md_restore(ac4_md *md, int r, const void *buf, long long len)
     p += sizeof c;
     const size_t g = R.off, n = R.n, nc = R.ncell;
     // rebase the replica-local indices onto this slot before copying
-    char *q = p + n * (15 * sizeof(double) + sizeof(double4) + sizeof(uint4)) + n * 7 * sizeof(int) +
+    char *q = p + n * (15 * sizeof(double) + sizeof(double4) + sizeof(QPos)) + n * 7 * sizeof(int) +
               3 * (size_t)R.nact + nc * 2 * sizeof(int) + n * sizeof(int);
     int2 *rg = (int2 *)q;
     for (size_t k = 0; k < nc * NRANGE; k++)
