@@ -460,8 +460,10 @@ __device__ __forceinline__ int clampc(int c, int n) { return min(max(c, 0), n - 
 
 // Binning: the reference cell id (bit-exact, for update_cell_list) and the
 // list-grid cell of every particle, with the place within its cell taken by
-// an atomic on the cell's count; the particle's (slot, id) is written to
-// that place of its cell's row of CELL_CAP entries (the reference's
+// an atomic on the cell's count; the particle's (slot, key) is written to
+// that place of its cell's row of CELL_CAP entries (key: the octant of the
+// cell the particle is in, then its id; the order k_cell_sort puts the cell
+// in) (the reference's
 // InCell[cell][MaxPartInCell], MDcode/global.h:20,85; a fuller cell sets the
 // capacity error).  The last block of a replica to finish turns the cell
 // counts into start offsets (exclusive scan).
@@ -479,7 +481,7 @@ __global__ void __launch_bounds__(BLK) k_bin(const RepC *__restrict__ reps, Chai
     const int i = blk_first[b] + threadIdx.x, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const bool valid = i < R.n;
     const int g = R.off + i;
-    int c = -1;
+    int c = -1, key = 0;
     if (valid) {
         const double sx = x[g], sy = y[g], sz = z[g];
         // reference cell id, cell_list+forces.c:73-74 (floor in double, then sum)
@@ -491,10 +493,15 @@ __global__ void __launch_bounds__(BLK) k_bin(const RepC *__restrict__ reps, Chai
                   cz = clampc((int)floor(sz * R.nc), R.nc);
         c = cx + cy * R.nc + cz * R.nc * R.nc;
         cell_of[g] = c;
+        // octant within the cell: the parity of the cell index on the grid
+        // of half-width cells (n < 2^21, so the id fits below bit 21)
+        const int ox = (int)floor(sx * (2 * R.nc)) & 1, oy = (int)floor(sy * (2 * R.nc)) & 1,
+                  oz = (int)floor(sz * (2 * R.nc)) & 1;
+        key = ((ox | (oy << 1) | (oz << 2)) << 21) | id[g];
     }
     // one returning atomic per particle, all in flight at once: the place
     // within the cell row depends on the order the atomics land in, which
-    // k_cell_sort removes (it orders each cell by particle id)
+    // k_cell_sort removes (it orders each cell by key)
     // Places within the cells: the wave's particles form runs of equal cells
     // (they are in cell order from the last rebuild, up to the few that
     // moved); the first lane of each run takes the run's places with one
@@ -517,7 +524,7 @@ __global__ void __launch_bounds__(BLK) k_bin(const RepC *__restrict__ reps, Chai
     const int base = __shfl(my_rank, valid ? lead : lane, 64);
     if (valid) my_rank = base + (lane - lead);
     if (valid) {
-        if (my_rank < CELL_CAP) cell_mem[(size_t)(R.off_cell + c) * CELL_CAP + my_rank] = make_int2(i, id[g]);
+        if (my_rank < CELL_CAP) cell_mem[(size_t)(R.off_cell + c) * CELL_CAP + my_rank] = make_int2(i, key);
         else atomicOr(&chain[r].err, 1);
     }
     __syncthreads();
@@ -624,16 +631,19 @@ __device__ __forceinline__ void gather_one(const PartArrays &src, const PartArra
     pc.zo[gn] = uz;
 }
 
-// Members of a cell in increasing original index (the binning order of
+// Members of a cell by octant of the cell (x, y, z bits, z highest), and in
+// increasing original index within an octant (the binning order of
 // cell_list+forces.c:71-80): a canonical order, so lists and force sums do not
-// depend on launch timing or on which replicas share the launch.  One wave per
-// cell: keys staged in LDS, each lane ranks its entries by counting and moves
-// each of its particles to its new slot (start of the cell + rank) in dst,
-// so consecutive ranks write consecutive slots.
+// depend on launch timing or on which replicas share the launch.  Consecutive
+// slots are then nearer in space than in plain index order, so the 64 lanes of
+// a force gather touch fewer cache lines.  One wave per cell: keys staged in
+// LDS, each lane ranks its entries by counting and moves each of its particles
+// to its new slot (start of the cell + rank) in dst, so consecutive ranks
+// write consecutive slots.
 #define SORT_CAP 256
 __global__ void __launch_bounds__(BLK) k_cell_sort(const RepC *__restrict__ reps, WorkList wl,
                                                    const int *__restrict__ cnt, const int *__restrict__ start,
-                                                   const int *__restrict__ id, const int2 *__restrict__ cell_mem,
+                                                   int2 *__restrict__ cell_mem,
                                                    int *members, int2 *ranges, PartArrays src, PartArrays dst,
                                                    PosCopies pc) {
     __shared__ int s_m[BLK / 64][SORT_CAP];
@@ -666,7 +676,7 @@ __global__ void __launch_bounds__(BLK) k_cell_sort(const RepC *__restrict__ reps
         int *m = members + R.off + first;
         // the cell's members in binning order (a cell beyond CELL_CAP has set
         // the capacity error; only its first CELL_CAP are moved)
-        const int2 *row = cell_mem + (size_t)(R.off_cell + c) * CELL_CAP;
+        int2 *row = cell_mem + (size_t)(R.off_cell + c) * CELL_CAP;
         const int n = min(cnt[R.off_cell + c], CELL_CAP);
         if (n <= SORT_CAP) {
             for (int k = lane; k < n; k += 64) { const int2 v = row[k]; s_m[w][k] = v.x; s_k[w][k] = v.y; }
@@ -680,19 +690,21 @@ __global__ void __launch_bounds__(BLK) k_cell_sort(const RepC *__restrict__ reps
             }
             __syncwarp();
         } else {
-            for (int k = lane; k < n; k += 64) m[k] = row[k].x;
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-            __syncwarp();
-            if (lane == 0) {  // crowded cell: serial insertion sort on the key
+            if (lane == 0) {  // crowded cell: serial insertion sort of the row on the key
                 for (int a = 1; a < n; a++) {
-                    int v = m[a], kv = id[R.off + v], k = a - 1;
-                    while (k >= 0 && id[R.off + m[k]] > kv) { m[k + 1] = m[k]; k--; }
-                    m[k + 1] = v;
+                    const int2 v = row[a];
+                    int k = a - 1;
+                    while (k >= 0 && row[k].y > v.y) { row[k + 1] = row[k]; k--; }
+                    row[k + 1] = v;
                 }
             }
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");  // lane 0's order, read by every lane
             __syncwarp();
-            for (int k = lane; k < n; k += 64) gather_one(src, dst, pc, R.off + first + k, R.off + m[k]);
+            for (int k = lane; k < n; k += 64) {
+                const int sl = row[k].x;
+                m[k] = sl;
+                gather_one(src, dst, pc, R.off + first + k, R.off + sl);
+            }
         }
     }
 }
@@ -1572,7 +1584,7 @@ struct ac4_md {
     double *fx = nullptr, *fy = nullptr, *fz = nullptr, *xo = nullptr, *yo = nullptr, *zo = nullptr;
     double4 *pos4 = nullptr;
     QPos *posq = nullptr;
-    int2 *cell_mem = nullptr;  // per cell: CELL_CAP (slot, id) of its members in binning order
+    int2 *cell_mem = nullptr;  // per cell: CELL_CAP (slot, key) of its members in binning order
     int *nn = nullptr, *nn_ref = nullptr, *nl = nullptr, *cnt = nullptr, *start = nullptr, *members = nullptr;
     int8_t *kx = nullptr, *ky = nullptr, *kz = nullptr;
     double *ex = nullptr, *ey = nullptr, *ez = nullptr;  // reference-order export
@@ -2527,7 +2539,7 @@ static int launch_rebuild(ac4_md *md, bool with_zero = false) {
     hipLaunchKernelGGL(k_bin, dim3(md->nblk), dim3(BLK), 0, s, md->reps, md->chain, md->blk_rep, md->blk_first,
                        md->P.x, md->P.y, md->P.z, md->P.id, md->cnt, md->P.cell_of, md->cell_mem, md->P.cell_ref,
                        md->start, wl);
-    hipLaunchKernelGGL(k_cell_sort, dim3(gc), dim3(BLK), 0, s, md->reps, wl, md->cnt, md->start, md->P.id,
+    hipLaunchKernelGGL(k_cell_sort, dim3(gc), dim3(BLK), 0, s, md->reps, wl, md->cnt, md->start,
                        md->cell_mem, md->members, md->ranges, md->P, md->A,
                        PosCopies{md->pos4, md->posq, md->xo, md->yo, md->zo});
     // 16- and 32-lane list teams of handles up to TEAM_MOVE_N particles move
