@@ -1609,8 +1609,9 @@ struct ac4_md {
     int *ov_n = nullptr;
     int *et = nullptr;
     hipStream_t stream = nullptr;
-    hipGraphExec_t gexec = nullptr;
-    hipGraph_t graph = nullptr;
+    // captured steps: [0] one step, [1] GRAPH_CHUNK steps in one graph
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    hipGraph_t graph[2] = {nullptr, nullptr};
     bool have_forces = false;
     int nlist_mode = 0;
 };
@@ -1869,11 +1870,18 @@ extern "C" int ac4_md_create_slots(int nslots, const int *cap_n, int precision, 
 template <class T>
 static int h2d(T *dst, const T *src, size_t n, hipStream_t s);
 
+static void drop_graphs(ac4_md *md) {
+    for (int k = 0; k < 2; k++) {
+        if (md->gexec[k]) { (void)hipGraphExecDestroy(md->gexec[k]); md->gexec[k] = nullptr; }
+        if (md->graph[k]) { (void)hipGraphDestroy(md->graph[k]); md->graph[k] = nullptr; }
+    }
+}
+
 // Block tables of the occupied slots only (all slots while none is
 // occupied), in slot order: the step's launch grids skip idle slots, whose
 // blocks would only exit.  A replica's blocks, and so its arithmetic (every
 // per-replica sum runs over its own blocks in their order), do not change.
-// The captured step graph is dropped: its grids change.
+// The captured step graphs are dropped: their grids change.
 static int refresh_tables(ac4_md *md) {
     bool any = false;
     for (const RepC &R : md->reps_h) any |= R.n > 0;
@@ -1894,8 +1902,7 @@ static int refresh_tables(ac4_md *md) {
     }
     hipStream_t s = md->stream;
     AC4_HIP(hipStreamSynchronize(s));
-    if (md->gexec) { (void)hipGraphExecDestroy(md->gexec); md->gexec = nullptr; }
-    if (md->graph) { (void)hipGraphDestroy(md->graph); md->graph = nullptr; }
+    drop_graphs(md);
     TRY(h2d(md->blk_rep, br.data(), br.size(), s)); TRY(h2d(md->blk_first, bf.data(), bf.size(), s));
     TRY(h2d(md->blk_idx, bi.data(), bi.size(), s));
     TRY(h2d(md->fblk_rep, fr.data(), fr.size(), s)); TRY(h2d(md->fblk_first, ff.data(), ff.size(), s));
@@ -1957,8 +1964,7 @@ extern "C" int ac4_md_create(int n_rep, const ac4_replica_params *p, int precisi
 extern "C" int ac4_md_destroy(ac4_md *md) {
     if (!md) return AC4_OK;
     if (md->stream) (void)hipStreamSynchronize(md->stream);
-    if (md->gexec) (void)hipGraphExecDestroy(md->gexec);
-    if (md->graph) (void)hipGraphDestroy(md->graph);
+    drop_graphs(md);
     void *ptrs[] = {md->reps, md->chain, md->blk_rep, md->blk_first, md->blk_idx, md->fblk_rep, md->fblk_first,
                     md->lblk_rep, md->lblk_first, md->wl_item, md->wl_n, md->ranges,
                     md->cblk_rep, md->cblk_cell, md->ox, md->oy, md->oz, md->ov_q, md->ov_q2,
@@ -2674,29 +2680,48 @@ extern "C" int ac4_md_step(ac4_md *md, int nsteps) {
     return AC4_OK;
 }
 
+// Steps are replayed from captured graphs: GRAPH_CHUNK steps per launch (one
+// launch per chunk instead of one per step: 1.53e9 -> 1.60e9 particle-steps/s
+// on the 64-replica ensemble in three launch groups), and a one-step graph for
+// the remainder.  Every step's kernels and arguments are the same (the step
+// state lives on the device), so a chunk replays exactly the steps launched
+// one by one.
+#define GRAPH_CHUNK 10
+static int capture_steps(ac4_md *md, int k) {
+    const int m = k ? GRAPH_CHUNK : 1;
+    AC4_HIP(hipStreamBeginCapture(md->stream, hipStreamCaptureModeThreadLocal));
+    int rc = AC4_OK;
+    for (int q = 0; q < m && rc == AC4_OK; q++) rc = launch_step(md);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(md->stream, &g);
+    if (rc != AC4_OK || e != hipSuccess) {  // nothing of a failed capture is kept
+        if (g) (void)hipGraphDestroy(g);
+        if (rc != AC4_OK) return rc;
+        AC4_HIP(e);
+    }
+    hipGraphExec_t x = nullptr;
+    e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        AC4_HIP(e);
+    }
+    md->graph[k] = g;
+    md->gexec[k] = x;
+    return AC4_OK;
+}
+
+static int ready_graphs(ac4_md *md) {
+    if (!md->have_forces) TRY(ac4_md_calculate_forces(md, nullptr));
+    for (int k = 0; k < 2; k++)
+        if (!md->gexec[k]) TRY(capture_steps(md, k));
+    return AC4_OK;
+}
+
 extern "C" int ac4_md_step_graph(ac4_md *md, int nsteps) {
     AC4_REQUIRE(md && nsteps >= 0, AC4_EARG, "bad arguments");
-    if (!md->have_forces) TRY(ac4_md_calculate_forces(md, nullptr));
-    if (!md->gexec) {
-        AC4_HIP(hipStreamBeginCapture(md->stream, hipStreamCaptureModeThreadLocal));
-        int rc = launch_step(md);
-        hipGraph_t g = nullptr;
-        hipError_t e = hipStreamEndCapture(md->stream, &g);
-        if (rc != AC4_OK || e != hipSuccess) {  // nothing of a failed capture is kept
-            if (g) (void)hipGraphDestroy(g);
-            if (rc != AC4_OK) return rc;
-            AC4_HIP(e);
-        }
-        hipGraphExec_t x = nullptr;
-        e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
-        if (e != hipSuccess) {
-            (void)hipGraphDestroy(g);
-            AC4_HIP(e);
-        }
-        md->graph = g;
-        md->gexec = x;
-    }
-    for (int k = 0; k < nsteps; k++) AC4_HIP(hipGraphLaunch(md->gexec, md->stream));
+    TRY(ready_graphs(md));
+    for (int k = 0; k < nsteps / GRAPH_CHUNK; k++) AC4_HIP(hipGraphLaunch(md->gexec[1], md->stream));
+    for (int k = 0; k < nsteps % GRAPH_CHUNK; k++) AC4_HIP(hipGraphLaunch(md->gexec[0], md->stream));
     return AC4_OK;
 }
 
@@ -2710,10 +2735,12 @@ extern "C" int ac4_md_step_graph_multi(ac4_md *const *mds, int nh, int nsteps) {
         AC4_REQUIRE(mds[h], AC4_EARG, "step_graph_multi: handle %d is NULL", h);
         for (int q = 0; q < h; q++)
             AC4_REQUIRE(mds[q] != mds[h], AC4_EARG, "step_graph_multi: handle %d appears twice", h);
-        TRY(ac4_md_step_graph(mds[h], 0));  // forces and the captured step of every handle first
+        TRY(ready_graphs(mds[h]));  // forces and the captured steps of every handle first
     }
-    for (int k = 0; k < nsteps; k++)
-        for (int h = 0; h < nh; h++) AC4_HIP(hipGraphLaunch(mds[h]->gexec, mds[h]->stream));
+    for (int k = 0; k < nsteps / GRAPH_CHUNK; k++)
+        for (int h = 0; h < nh; h++) AC4_HIP(hipGraphLaunch(mds[h]->gexec[1], mds[h]->stream));
+    for (int k = 0; k < nsteps % GRAPH_CHUNK; k++)
+        for (int h = 0; h < nh; h++) AC4_HIP(hipGraphLaunch(mds[h]->gexec[0], mds[h]->stream));
     return AC4_OK;
 }
 

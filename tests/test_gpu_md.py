@@ -332,19 +332,48 @@ def test_checkpoint_restore_is_exact(gpu, prec):
 
 
 def test_graph_replay_equals_eager(gpu):
+    """Graph replays (chunks of ten captured steps, then one-step graphs for
+    the rest: 57 = 5 x 10 + 7, 43 = 4 x 10 + 3) equal eager launches."""
     spec = [(1200, 41, 1), (2000, 42, 2)]
     e = make_relaxed_md(gpu, spec, tau=25, prec=1)
     g = make_relaxed_md(gpu, spec, tau=25, prec=1)
     e.step(100)
     e.sync()
-    g.step(60, graph=True)
+    g.step(57, graph=True)
     g.sync()
-    g.step(40, graph=True)
+    g.step(43, graph=True)
     g.sync()
     for r in range(2):
         se, sg = e.get_state(r), g.get_state(r)
         for k in ("xu", "px", "kz"):
             assert np.array_equal(se[k], sg[k])
+        assert e.stats(r)["steps"] == g.stats(r)["steps"] == 100
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_launch_groups_equal_one_handle(gpu, prec):
+    """Replicas split into launch groups (one handle and stream each, graph
+    replays interleaved by ac4_md_step_graph_multi) follow the same
+    trajectories as the same replicas stepped eagerly in one handle, with
+    the force team width pinned for all handles."""
+    spec = [(1300, 71, 3), (2100, 72, 4), (1000, 73, 5), (1700, 74, 6)]
+    with tuning(gpu, 1):
+        one = make_relaxed_md(gpu, spec, tau=20, prec=prec)
+        ga = make_relaxed_md(gpu, [spec[0], spec[3]], tau=20, prec=prec)
+        gb = make_relaxed_md(gpu, [spec[1], spec[2]], tau=20, prec=prec)
+    one.step(137)
+    one.sync()
+    gpu.step_multi([ga, gb], 137)
+    ga.sync()
+    gb.sync()
+    for r, (h, q) in enumerate([(ga, 0), (gb, 0), (gb, 1), (ga, 1)]):
+        a, b = one.get_state(r), h.get_state(q)
+        for k in ("xu", "yu", "zu", "px", "py", "pz", "kx"):
+            assert np.array_equal(a[k], b[k]), (r, k)
+        assert one.stats(r)["Heff"] == h.stats(q)["Heff"]
+        assert one.rebuilds(r) == h.rebuilds(q) > 0
+    for m in (one, ga, gb):
+        m.close()
 
 
 def test_stats_consistent_with_state(gpu):
