@@ -44,12 +44,12 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 # compulsory HBM bytes per particle of the fused force kernel (k_force, FP32
-# pair math, half kick fused): own position 32 (double4 incl. type) + list
-# length 4 + active index 4 + momentum read 24 + force write 24 + momentum
-# write 24 = 112 B.  Neighbour positions are gathered from cache; the list
-# entries (4 B each) are the kernel's other stream and show in the measured
-# traffic.
-FORCE_BYTES_PER_PARTICLE = 112.0
+# pair math, half kick fused): own fixed-point position with the species bit
+# 12 + list length 4 + active index 4 + momentum read 24 + force write 24 +
+# momentum write 24 = 92 B.  Neighbour positions are gathered from cache; the
+# list entries (4 B each) are the kernel's other stream and show in the
+# measured traffic.
+FORCE_BYTES_PER_PARTICLE = 92.0
 # compulsory HBM bytes of one whole mixed-precision step per particle (force
 # kernel: packed FP32 position + type 16, force 12; integration: read v, F, x,
 # x_u 84, write v, x, x_u and the FP32 position copy 88): 200 B.
