@@ -217,14 +217,13 @@ __device__ __forceinline__ void shuffle_trace(const unsigned int *s_j, const int
         if (x < na) kg[x] = s_k[y[c]];
     }
 }
-__global__ void __launch_bounds__(256) k_shuffle(const RepC *__restrict__ reps, const ChainS *__restrict__ chain,
-                                                 int8_t *kx, int8_t *ky, int8_t *kz, int force, int kcap, int skip_large) {
-    extern __shared__ unsigned int s_shuf[];  // SHUF_CH partners, then kcap bytes of directions
+// One component of one replica's shuffle by a workgroup of 256 threads (the
+// branch is uniform over the workgroup); s_shuf is the dynamic LDS: SHUF_CH
+// partners, then kcap bytes of directions.
+__device__ __forceinline__ void shuffle_comp(const RepC &R, const ChainS &c, int comp, int8_t *kx, int8_t *ky,
+                                             int8_t *kz, int force, int kcap, int skip_large, unsigned int *s_shuf) {
     unsigned int *s_j = s_shuf;
     int8_t *s_k = (int8_t *)(s_shuf + SHUF_CH);
-    const int r = blockIdx.x, comp = blockIdx.y;
-    const RepC R = reps[r];
-    const ChainS &c = chain[r];
     if (!force && (c.mode == 1 || R.tau <= 0 || (c.step % R.tau) != 0)) return;
     const uint64_t sh = (uint64_t)c.shuffles;
     const int na = R.nact;
@@ -267,6 +266,13 @@ __global__ void __launch_bounds__(256) k_shuffle(const RepC *__restrict__ reps, 
     __syncthreads();
     if (lds)
         for (int i = threadIdx.x; i < na; i += 256) kg[i] = s_k[i];
+}
+__global__ void __launch_bounds__(256) k_shuffle(const RepC *__restrict__ reps, const ChainS *__restrict__ chain,
+                                                 int8_t *kx, int8_t *ky, int8_t *kz, int force, int kcap, int skip_large) {
+    extern __shared__ unsigned int s_shuf[];
+    const int r = blockIdx.x, comp = blockIdx.y;
+    const RepC R = reps[r];
+    shuffle_comp(R, chain[r], comp, kx, ky, kz, force, kcap, skip_large, s_shuf);
 }
 
 // The same swap sequence in parallel for large active sets.  The swaps
@@ -366,6 +372,15 @@ __global__ void __launch_bounds__(2 * SHUF_M) k_shuf_apply(const RepC *__restric
         for (int i = threadIdx.x; i < na; i += 2 * SHUF_M) kg[i] = s_k[i];
 }
 
+// The step's shuffle as the first 3*R workgroups of k_first_half (nblk = 3*R;
+// 0 when it runs as k_shuffle before it): the shuffle only has to finish
+// before the force kernel reads the directions, so its workgroups run beside
+// the half kick instead of as a launch of their own in the step's chain.
+struct ShufArgs {
+    int8_t *kx, *ky, *kz;
+    int kcap, nblk;
+};
+
 __global__ void __launch_bounds__(BLK) k_first_half(const RepC *__restrict__ reps, const ChainS *chain,
                                                     const int *__restrict__ blk_rep, const int *__restrict__ blk_first,
                                                     double *x, double *y, double *z, double *xu, double *yu, double *zu,
@@ -374,11 +389,19 @@ __global__ void __launch_bounds__(BLK) k_first_half(const RepC *__restrict__ rep
                                                     const double *__restrict__ xo, const double *__restrict__ yo,
                                                     const double *__restrict__ zo, const int *__restrict__ type,
                                                     double4 *pos4, QPos *posq, int q32, ChainS *chain_w,
-                                                    const int *__restrict__ blk_idx, int *cnt, int *wl_n) {
+                                                    const int *__restrict__ blk_idx, int *cnt, int *wl_n,
+                                                    ShufArgs sa) {
     __shared__ double s_scale[2];
     __shared__ double s_max[BLK / 64];
-    if (blockIdx.x == 0 && threadIdx.x < 3) wl_n[threadIdx.x] = 0;  // rebuild work of this step
-    const int b = blockIdx.x, r = blk_rep[b];
+    if ((int)blockIdx.x < sa.nblk) {  // the step's shuffle: replica b / 3, component b % 3
+        extern __shared__ unsigned int s_shuf[];
+        const int rs = blockIdx.x / 3;
+        const RepC Rs = reps[rs];
+        shuffle_comp(Rs, chain[rs], blockIdx.x % 3, sa.kx, sa.ky, sa.kz, 0, sa.kcap, 0, s_shuf);
+        return;
+    }
+    const int b = blockIdx.x - sa.nblk, r = blk_rep[b];
+    if (b == 0 && threadIdx.x < 3) wl_n[threadIdx.x] = 0;  // rebuild work of this step
     const RepC R = reps[r];
     if (threadIdx.x == 0) {
         const ChainS &c = chain[r];
@@ -1575,6 +1598,7 @@ struct ac4_md {
     int2 *shuf_ent = nullptr;  // its per-block source maps
     int *shuf_ent_n = nullptr;
     size_t shuf_lds = 0;   // dynamic LDS of k_shuffle
+    bool shuf_fused = false;  // the shuffle runs in k_first_half's first workgroups
     int *wl_item = nullptr, *wl_n = nullptr;  // rebuild work list (WorkList)
     int2 *ranges = nullptr;                    // per cell: stencil slot ranges (NRANGE)
     int *lblk_rep = nullptr, *lblk_first = nullptr;  // list-build block table (LPB particles per block)
@@ -1809,6 +1833,8 @@ extern "C" int ac4_md_create_slots(int nslots, const int *cap_n, int precision, 
             md->shuf_lds = 65536;
         }
         md->shuf_bmax = cap_max > SHUF_PAR_MIN ? (cap_max + SHUF_M - 1) / SHUF_M : 0;
+        // in k_first_half when its LDS leaves that kernel at least 5 workgroups per CU
+        md->shuf_fused = md->shuf_bmax == 0 && md->shuf_lds <= 32768;
     }
     int rc = AC4_OK;
 #define ALLOC(ptr, n) if ((rc = dalloc(&md->ptr, (size_t)(n))) != AC4_OK) { ac4_md_destroy(md); return rc; }
@@ -2606,20 +2632,29 @@ static void launch_shuffle(ac4_md *md, int force, hipStream_t s) {
                            md->shuf_ent_n);
     }
 }
-static void launch_first_half(ac4_md *md) {
-    hipLaunchKernelGGL(k_first_half, dim3(md->nblk), dim3(BLK), 0, md->stream, md->reps, md->chain, md->blk_rep,
-                       md->blk_first, md->P.x, md->P.y, md->P.z, md->P.xu, md->P.yu, md->P.zu, md->P.px, md->P.py,
-                       md->P.pz, md->fx, md->fy, md->fz, md->xo, md->yo, md->zo, md->P.type, md->pos4, md->posq,
-                       md->prec == AC4_FP32 ? 1 : 0, md->chain, md->blk_idx, md->cnt, md->wl_n);
+// with_shuffle: the step's shuffle runs in k_first_half's first 3*R
+// workgroups (handles whose shuffle LDS is small, see shuf_fused)
+static void launch_first_half(ac4_md *md, bool with_shuffle) {
+    const ShufArgs sa{md->kx, md->ky, md->kz, md->shuf_kcap, with_shuffle ? 3 * md->R : 0};
+    hipLaunchKernelGGL(k_first_half, dim3(md->nblk + sa.nblk), dim3(BLK), with_shuffle ? md->shuf_lds : 0, md->stream,
+                       md->reps, md->chain, md->blk_rep, md->blk_first, md->P.x, md->P.y, md->P.z, md->P.xu, md->P.yu,
+                       md->P.zu, md->P.px, md->P.py, md->P.pz, md->fx, md->fy, md->fz, md->xo, md->yo, md->zo,
+                       md->P.type, md->pos4, md->posq, md->prec == AC4_FP32 ? 1 : 0, md->chain, md->blk_idx, md->cnt,
+                       md->wl_n, sa);
 }
 
 // One MD step, one chain of launches on the handle's stream (also when
 // captured in a graph).  The shuffle, which only the force kernel needs, once
 // ran on a second graph branch beside the half kick and the rebuild: the
-// fork and join cost more than the overlap gave (about 9 us per step).
+// fork and join cost more than the overlap gave (about 9 us per step).  It
+// now runs inside k_first_half where its LDS allows, else as k_shuffle first.
 static int launch_step(ac4_md *md) {
-    launch_shuffle(md, 0, md->stream);
-    launch_first_half(md);
+    if (md->shuf_fused) {
+        launch_first_half(md, true);
+    } else {
+        launch_shuffle(md, 0, md->stream);
+        launch_first_half(md, false);
+    }
     AC4_HIP(hipGetLastError());
     TRY(launch_rebuild(md));
     return launch_force<true>(md);
@@ -2756,7 +2791,9 @@ extern "C" int ac4_md_relax(ac4_md *md, int nsteps, double f_start, double f_end
 }
 
 // Eager steps with HIP events (on the handle's stream) around each launch
-// group; ms4 = average ms per step of {first_half, rebuild group, force, shuffle}.
+// group; ms4 = average ms per step of {first_half, rebuild group, force,
+// shuffle}; where the shuffle runs inside k_first_half its time is in the
+// first entry and the last is the empty interval between two events.
 extern "C" int ac4_md_kernel_times(ac4_md *md, int nsteps, double *ms4) {
     AC4_REQUIRE(md && nsteps > 0 && ms4, AC4_EARG, "kernel_times: bad arguments");
     if (!md->have_forces) TRY(ac4_md_calculate_forces(md, nullptr));
@@ -2766,9 +2803,9 @@ extern "C" int ac4_md_kernel_times(ac4_md *md, int nsteps, double *ms4) {
     double acc[4] = {0, 0, 0, 0};
     for (int k = 0; k < nsteps; k++) {
         AC4_HIP(hipEventRecord(ev[0], s));
-        launch_shuffle(md, 0, s);
+        if (!md->shuf_fused) launch_shuffle(md, 0, s);
         AC4_HIP(hipEventRecord(ev[1], s));
-        launch_first_half(md);
+        launch_first_half(md, md->shuf_fused);
         AC4_HIP(hipEventRecord(ev[2], s));
         TRY(launch_rebuild(md));
         AC4_HIP(hipEventRecord(ev[3], s));

@@ -141,7 +141,9 @@ int ac4_md_stats(ac4_md *md, int r, double *out8);
 int ac4_md_rebuilds(ac4_md *md, int r);
 /* Measurement hook (no reference counterpart): nsteps eager steps timed with
  * HIP events on the handle's stream around each launch group; ms4 = average
- * milliseconds per step of {first half, rebuild group, force, shuffle}. */
+ * milliseconds per step of {first half, rebuild group, force, shuffle}.  Where
+ * the handle runs the shuffle inside the first-half kernel (small active
+ * sets), the shuffle's time is part of the first entry. */
 int ac4_md_kernel_times(ac4_md *md, int nsteps, double *ms4);
 /* Waits for all device work of the handle; reports deferred device errors
  * (list/cell overflow). */

@@ -244,7 +244,8 @@ def test_rebuild_criterion_matches_oracle(gpu, crit):
 def test_shuffle_follows_reference_swap_rule(gpu, n):
     """200, 700 and 1024 active particles (the traced shuffle, one, three and
     four positions per lane), 1025 and 4000 (the serial chain): the
-    reference's swap chain, value for value, on the same uniforms."""
+    reference's swap chain, value for value, on the same uniforms, for a
+    shuffle forced now and for the one the next step makes."""
     s, L = lattice_state(n, seed=4)
     md = make_md(gpu, [(s, L)], tau=200)
     before = md.get_state(0)
@@ -260,6 +261,17 @@ def test_shuffle_follows_reference_swap_rule(gpu, n):
         assert after[k].sum() == np.sum(before[k]) and set(after[k].tolist()) <= {-1, 1}
         assert n % 20 != 0 or after[k].sum() == 0
     assert not np.array_equal(after["kx"], before["kx"])
+    # the shuffle a step makes at step 0 of tau: inside the half-kick kernel's
+    # first workgroups on handles whose shuffle LDS is small (n up to 10250
+    # here), as its own kernel otherwise (40000); the next uniforms
+    u2 = md.shuffle_uniforms(0)
+    md.step(1)
+    stepped = md.get_state(0)
+    for c, k in enumerate(("kx", "ky", "kz")):
+        ref = np.array(after[k], dtype=np.int32)
+        OL.load().or_shuffle(len(ref), OL.ip(ref), OL.dp(np.ascontiguousarray(u2[c])))
+        assert np.array_equal(stepped[k], ref)
+    assert not np.array_equal(stepped["kx"], after["kx"])
 
 
 @pytest.mark.parametrize("prec,tpp", [(0, 4), (1, 1), (1, 8)])
