@@ -774,13 +774,19 @@ __device__ __forceinline__ size_t nl_at(const RepC &R, int cap, int k, int i) {
 }
 
 // Dynamics-list screen shared by every builder (so all of them keep exactly
-// the same pairs): exact minimum image from the fixed-point positions, r^2 in
-// FP32 against the list radius plus a margin of 1e-3 sigma^2, far above the
-// FP32 error of r^2 (below 1e-5 for the list radii), so no pair within its
-// list radius is lost.
-__device__ __forceinline__ bool qscreen(const QPos &pi, const QPos &q, float qs2, float cfa, float cfb) {
+// the same pairs): minimum image from the fixed-point positions, r^2 in FP32
+// in fixed-point units (fused multiply-adds) against the list radius plus a
+// margin of 1e-3 sigma^2 in those units, far above the FP32 error of r^2
+// (below 1e-5 sigma^2 for the list radii), so no pair within its list radius
+// is lost.  Which pairs beyond the force cutoff the margin lets in does not
+// change any force: a pair contributes only inside 2.5 sigma_ab.
+__device__ __forceinline__ bool qscreen(const QPos &pi, const QPos &q, float ua, float ub) {
     const float dx = (float)(int)(pi.x - q.x), dy = (float)(int)(pi.y - q.y), dz = (float)(int)(pi.z - q.z);
-    return (dx * dx + dy * dy + dz * dz) * qs2 < (ptype(q) == 0 ? cfa : cfb);
+    return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)) < ((q.x & 1u) ? ub : ua);
+}
+// list radius^2 of species pair ab plus the screen's margin, in fixed-point units^2
+__device__ __forceinline__ float qscreen_cut(int ab, const RepC &R) {
+    return (float)((list_cut2(ab, R.skin) + 1e-3) / (double)R.qs2);
 }
 
 #define NL_UNROLL 8
@@ -814,7 +820,7 @@ __global__ void __launch_bounds__(BLK) k_nlist(const RepC *__restrict__ reps, Ch
     const int nx = stencil_axis(cx, nc, ax), ny = stencil_axis(cy, nc, ay), nz = stencil_axis(cz, nc, az);
     const double L = R.L;
     const double cut_a = list_cut2(ti, R.skin), cut_b = list_cut2(ti + 1, R.skin);
-    const float cfa = (float)cut_a + 1e-3f, cfb = (float)cut_b + 1e-3f;
+    const float ua = qscreen_cut(ti, R), ub = qscreen_cut(ti + 1, R);
     int count = 0, cref = 0;
     for (int iz = 0; iz < nz; iz++)
         for (int iy = 0; iy < ny; iy++)
@@ -849,7 +855,7 @@ __global__ void __launch_bounds__(BLK) k_nlist(const RepC *__restrict__ reps, Ch
                         for (int u = 0; u < NL_UNROLL; u++) {
                             const int j = k + u;
                             if (j >= e2 || j == g) continue;
-                            if (qscreen(pq, q[u], R.qs2, cfa, cfb)) {
+                            if (qscreen(pq, q[u], ua, ub)) {
                                 if (count < cap) nl[nl_at(R, cap, count, i)] = j;
                                 count++;
                             }
@@ -893,10 +899,9 @@ __device__ __forceinline__ void nlist_team_one(const RepC &R, int r, int i, Chai
     const QPos pi = posq[g];
     const int ti = ptype(pi);
     const int2 *rgs = ranges + (size_t)(R.off_cell + cell_of[g]) * NRANGE;
-    const float cfa = (float)list_cut2(ti, R.skin) + 1e-3f, cfb = (float)list_cut2(ti + 1, R.skin) + 1e-3f,
-                qs2 = R.qs2;
+    const float ua = qscreen_cut(ti, R), ub = qscreen_cut(ti + 1, R);
     int count = 0;
-    auto inlist = [&](const QPos &q) -> bool { return qscreen(pi, q, qs2, cfa, cfb); };
+    auto inlist = [&](const QPos &q) -> bool { return qscreen(pi, q, ua, ub); };
     // entry p of this particle's list at nlb[(p >> lc) * nC + (p & cm)] (nl_at
     // relative to entry 0; 24-bit products, n < 2^21 is checked at creation)
     int *const nlb = nl + nl_at(R, cap, 0, i);
@@ -1077,12 +1082,12 @@ __global__ void __launch_bounds__(64) k_nlist_cell(const RepC *__restrict__ reps
     for (int h = 0; h < hn; h++) {
         const QPos pi = s_c[hoff + h];
         const int ti = ptype(pi), kh = hoff + h;
-        const float cfa = (float)list_cut2(ti, R.skin) + 1e-3f, cfb = (float)list_cut2(ti + 1, R.skin) + 1e-3f;
+        const float ua = qscreen_cut(ti, R), ub = qscreen_cut(ti + 1, R);
         int base = 0;
         for (int k0 = 0; k0 < total; k0 += 64) {
             const int k = k0 + lane;
             bool hit = false;
-            if (k < total && k != kh) hit = qscreen(pi, s_c[k], R.qs2, cfa, cfb);
+            if (k < total && k != kh) hit = qscreen(pi, s_c[k], ua, ub);
             const unsigned long long mask = __ballot(hit);
             if (hit) {
                 const int pos = base + __popcll(mask & below);
