@@ -160,6 +160,10 @@ _SIGS = {
     "ac4_md_errors": (ctypes.c_int, [_VP, _I]),
     "ac4_md_raise_error": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int]),
     "ac4_server_handle": (_VP, [_VP]),
+    "ac4_server_slot_handle": (_VP, [_VP, ctypes.c_int, _I]),
+    "ac4_server_groups": (ctypes.c_int, [_VP]),
+    "ac4_server_create_groups": (ctypes.c_int, [ctypes.c_int, _I, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                ctypes.c_int, ctypes.POINTER(_VP)]),
     "ac4_server_job_slot": (ctypes.c_int, [_VP, ctypes.c_int]),
     "ac4_server_submit": (ctypes.c_int, [_VP, ctypes.POINTER(Job)]),
     "ac4_server_tick": (ctypes.c_int, [_VP, ctypes.c_double]),
@@ -521,11 +525,12 @@ class Server:
     """Request loop over device-resident slots (ac4_server_* of include/ac4.h)."""
 
     def __init__(self, slot_caps=None, precision=FP32, chunk=100, relax=300, budget=None, n_range=(500, 20000),
-                 overlap=None):
+                 overlap=None, groups=0):
         """Slots of the capacities slot_caps, or planned from an HBM byte
         budget for requests with N in n_range (ac4_server_create_budget).
         overlap = (origin_every, n_origin, n_lags, a): every job's Q(t)
-        accumulated on the device from its frames."""
+        accumulated on the device from its frames.  groups: launch groups of
+        the slots (ac4_server_create_groups; 0 = the server's choice)."""
         load()
         h = _VP()
         if budget is not None:
@@ -534,7 +539,8 @@ class Server:
                                                 relax, oe, no, nl, a, ctypes.byref(h)))
         else:
             caps = _i32(slot_caps)
-            _check(lib.ac4_server_create(len(caps), _ip(caps), precision, chunk, relax, ctypes.byref(h)))
+            _check(lib.ac4_server_create_groups(len(caps), _ip(caps), precision, chunk, relax, int(groups),
+                                                ctypes.byref(h)))
         self.h = h
         self.n_of = {}
         self.n_lags = None
@@ -549,13 +555,20 @@ class Server:
         lib.ac4_server_slots(self.h, _ip(caps))
         return caps
 
+    def groups(self):
+        return lib.ac4_server_groups(self.h)
+
     def raise_error(self, jid, bits=2):
         """Sets error bits on the replica of running job jid, as a kernel
         would on an overflow (ac4_md_raise_error); the next tick fails it."""
         slot = lib.ac4_server_job_slot(self.h, jid)
         if slot < 0:
             raise AC4Error("job %d is not running" % jid)
-        _check(lib.ac4_md_raise_error(lib.ac4_server_handle(self.h), slot, bits))
+        local = np.zeros(1, dtype=np.int32)
+        md = lib.ac4_server_slot_handle(self.h, slot, _ip(local))
+        if not md:
+            raise AC4Error(lib.ac4_last_error().decode(errors="replace"))
+        _check(lib.ac4_md_raise_error(md, int(local[0]), bits))
 
     def job_error(self, jid):
         rc = lib.ac4_server_job_error(self.h, jid)

@@ -7,6 +7,12 @@ extern "C" {
 
 void ac4_set_error(const char *fmt, ...);
 
+/* ac4_md_create_slots with the automatic force team width taken for a total
+ * capacity of cap_rule (> 0) instead of the handle's own: the launch groups
+ * of a server share one kernel shape.  (ac4.h, included first, declares
+ * ac4_md.) */
+int ac4_md_create_slots_rule(int nslots, const int *cap_n, int precision, double cap_rule, ac4_md **out);
+
 /* The (origin, lag) walk of Analysis/msd_pos.c:44-107 (rule 0) and
  * overlap.c:42-106 (rule 1): pairs in walk order. */
 typedef struct {

@@ -1744,7 +1744,16 @@ static void fresh_chain(ChainS &c, const RepC &R) {
     c.force_rebuild = R.n > 0 ? 1 : 0;
 }
 
+// cap_rule > 0: the automatic force team width is taken for that total
+// capacity instead of the handle's own (a server's launch groups share one
+// kernel shape, so a job's arithmetic does not depend on its group)
+extern "C" int ac4_md_create_slots_rule(int nslots, const int *cap_n, int precision, double cap_rule,
+                                        ac4_md **out);
 extern "C" int ac4_md_create_slots(int nslots, const int *cap_n, int precision, ac4_md **out) {
+    return ac4_md_create_slots_rule(nslots, cap_n, precision, 0.0, out);
+}
+extern "C" int ac4_md_create_slots_rule(int nslots, const int *cap_n, int precision, double cap_rule,
+                                        ac4_md **out) {
     AC4_REQUIRE(nslots >= 1 && cap_n && out, AC4_EARG, "ac4_md_create_slots: bad arguments");
     AC4_REQUIRE(precision == AC4_FP64 || precision == AC4_FP32, AC4_EARG, "precision must be 0 or 1");
     int ndev = 0;
@@ -1761,6 +1770,7 @@ extern "C" int ac4_md_create_slots(int nslots, const int *cap_n, int precision, 
     double cap_tot = 0.0;
     for (int r = 0; r < nslots; r++) cap_tot += cap_n[r] > 0 ? cap_n[r] : 0;
     int tpp = g_tpp, ltpp = 0;
+    if (cap_rule > 0.0) cap_tot = cap_rule;
     if (tpp == 0) {
         tpp = 1;
         while (tpp < 16 && cap_tot * tpp * 2 <= TPP_LANES) tpp *= 2;

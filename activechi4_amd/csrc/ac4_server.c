@@ -5,8 +5,14 @@
  * time; the server keeps a fixed set of device-resident slots, admits queued
  * jobs into a free slot of their size class (FIFO order; a later job may take
  * a free slot of its own class while an earlier one waits), advances every
- * slot together in chunks of `chunk` steps with one replayed hipGraph, and
+ * slot together in chunks of `chunk` steps with replayed hipGraphs, and
  * retires finished jobs.
+ *
+ * - The slots are split into launch groups of balanced capacity, one MD
+ *   handle and stream each, stepped together (ac4_md_step_graph_multi: one
+ *   group's force kernel runs beside another's list rebuild).  Every group
+ *   uses the kernel shape of the server's total capacity, so a job's
+ *   arithmetic does not depend on the group its slot is in.
  *
  * - Slots come from an HBM byte budget (ac4_server_plan): size classes a
  *   factor 2 apart over the request range [n_min, n_max], slot counts per
@@ -46,9 +52,12 @@ typedef struct {
 } job_rec;
 
 struct ac4_server {
-    ac4_md *md;
+    ac4_md **mds; /* launch groups: one handle and stream each */
+    int ng;
     int nslots, chunk, relax; /* relax: ramp steps rounded up to whole chunks */
     int *cap, *slot_job, *err;
+    int *slot_h, *slot_l;     /* slot -> its group, and its index in that group's handle */
+    int *gerr;                /* error bits of one group's slots */
     job_rec *jobs;
     int njobs, jcap;
     int ov_on, ov_nl;         /* in-situ overlap: on, lags + 1 */
@@ -108,16 +117,28 @@ int ac4_server_plan(double hbm_bytes, int n_min, int n_max, int n_origin, int n_
     return ns;
 }
 
-int ac4_server_create(int nslots, const int *slot_cap, int precision, int chunk, int relax_steps,
-                      ac4_server **out) {
-    if (nslots < 1 || !slot_cap || chunk < 1 || relax_steps < 1 || !out) {
+/* Launch groups when the caller leaves it to the server: SRV_GROUPS groups
+ * once the slots hold at least SRV_GROUP_MIN particles per group (on the
+ * 64-replica ensemble three groups of about 51k particles ran 1.57e9
+ * particle-steps/s against 1.42e9 for one handle; a fourth stream exceeds
+ * the process's four hardware queues), else one. */
+#define SRV_GROUPS 3
+#define SRV_GROUP_MIN 16384
+
+int ac4_server_create_groups(int nslots, const int *slot_cap, int precision, int chunk, int relax_steps,
+                             int groups, ac4_server **out) {
+    if (nslots < 1 || !slot_cap || chunk < 1 || relax_steps < 1 || groups < 0 || !out) {
         ac4_set_error("ac4_server_create: bad arguments");
         return AC4_EARG;
     }
     ac4_server *s = calloc(1, sizeof *s);
     if (!s) { ac4_set_error("ac4_server_create: out of host memory"); return AC4_ECAP; }
-    int rc = ac4_md_create_slots(nslots, slot_cap, precision, &s->md);
-    if (rc != AC4_OK) { free(s); return rc; }
+    double cap_tot = 0.0;
+    for (int k = 0; k < nslots; k++) cap_tot += slot_cap[k] > 0 ? slot_cap[k] : 0;
+    int ng = groups ? groups
+                    : ((nslots >= SRV_GROUPS && cap_tot >= (double)SRV_GROUP_MIN * SRV_GROUPS) ? SRV_GROUPS : 1);
+    if (ng > nslots) ng = nslots;
+    s->ng = ng;
     s->nslots = nslots;
     s->chunk = chunk;
     /* the ramp does not advance a replica's step counter: whole chunks keep
@@ -125,15 +146,51 @@ int ac4_server_create(int nslots, const int *slot_cap, int precision, int chunk,
     s->relax = ((relax_steps + chunk - 1) / chunk) * chunk;
     s->cap = malloc(sizeof(int) * nslots);
     s->slot_job = malloc(sizeof(int) * nslots);
-    s->err = malloc(sizeof(int) * nslots);
-    if (!s->cap || !s->slot_job || !s->err) {
+    s->err = calloc((size_t)nslots, sizeof(int));
+    s->slot_h = malloc(sizeof(int) * nslots);
+    s->slot_l = malloc(sizeof(int) * nslots);
+    s->gerr = malloc(sizeof(int) * nslots);
+    s->mds = calloc((size_t)ng, sizeof(ac4_md *));
+    int *ord = malloc(sizeof(int) * nslots), *caps_g = malloc(sizeof(int) * nslots);
+    double *load = calloc((size_t)ng, sizeof(double));
+    if (!s->cap || !s->slot_job || !s->err || !s->slot_h || !s->slot_l || !s->gerr || !s->mds || !ord || !caps_g ||
+        !load) {
+        free(ord); free(caps_g); free(load);
         ac4_server_destroy(s);
         ac4_set_error("ac4_server_create: out of host memory");
         return AC4_ECAP;
     }
-    for (int k = 0; k < nslots; k++) { s->cap[k] = slot_cap[k]; s->slot_job[k] = -1; }
+    for (int k = 0; k < nslots; k++) { s->cap[k] = slot_cap[k]; s->slot_job[k] = -1; ord[k] = k; }
+    /* slots to groups: largest capacity first (equal ones in slot order), each
+     * to the group with the least capacity so far (the lowest on a tie) */
+    for (int a = 1; a < nslots; a++) {
+        const int v = ord[a];
+        int b = a - 1;
+        while (b >= 0 && slot_cap[ord[b]] < slot_cap[v]) { ord[b + 1] = ord[b]; b--; }
+        ord[b + 1] = v;
+    }
+    for (int a = 0; a < nslots; a++) {
+        int g = 0;
+        for (int q = 1; q < ng; q++) if (load[q] < load[g]) g = q;
+        s->slot_h[ord[a]] = g;
+        load[g] += slot_cap[ord[a]];
+    }
+    int rc = AC4_OK;
+    for (int g = 0; g < ng && rc == AC4_OK; g++) {
+        int n = 0;
+        for (int k = 0; k < nslots; k++)
+            if (s->slot_h[k] == g) { s->slot_l[k] = n; caps_g[n++] = slot_cap[k]; }
+        rc = ac4_md_create_slots_rule(n, caps_g, precision, cap_tot, &s->mds[g]);
+    }
+    free(ord); free(caps_g); free(load);
+    if (rc != AC4_OK) { ac4_server_destroy(s); return rc; }
     *out = s;
     return AC4_OK;
+}
+
+int ac4_server_create(int nslots, const int *slot_cap, int precision, int chunk, int relax_steps,
+                      ac4_server **out) {
+    return ac4_server_create_groups(nslots, slot_cap, precision, chunk, relax_steps, 0, out);
 }
 
 int ac4_server_create_budget(double hbm_bytes, int n_min, int n_max, int precision, int chunk, int relax_steps,
@@ -159,8 +216,10 @@ int ac4_server_enable_overlap(ac4_server *s, int origin_every, int n_origin, int
         ac4_set_error("enable_overlap: call once, before the first job is submitted");
         return AC4_ESTATE;
     }
-    int rc = ac4_md_overlap_enable(s->md, s->chunk, origin_every, n_origin, n_lags, a);
-    if (rc != AC4_OK) return rc;
+    for (int g = 0; g < s->ng; g++) {
+        int rc = ac4_md_overlap_enable(s->mds[g], s->chunk, origin_every, n_origin, n_lags, a);
+        if (rc != AC4_OK) return rc;
+    }
     s->ov_on = 1;
     s->ov_nl = n_lags + 1;
     return AC4_OK;
@@ -174,9 +233,12 @@ int ac4_server_slots(const ac4_server *s, int *caps) {
 
 int ac4_server_destroy(ac4_server *s) {
     if (!s) return AC4_OK;
-    if (s->md) ac4_md_destroy(s->md);
+    if (s->mds)
+        for (int g = 0; g < s->ng; g++)
+            if (s->mds[g]) ac4_md_destroy(s->mds[g]);
     for (int j = 0; j < s->njobs; j++) { free(s->jobs[j].result); free(s->jobs[j].ov); free(s->jobs[j].ckpt); }
     free(s->jobs); free(s->cap); free(s->slot_job); free(s->err);
+    free(s->slot_h); free(s->slot_l); free(s->gerr); free(s->mds);
     free(s);
     return AC4_OK;
 }
@@ -206,13 +268,15 @@ int ac4_server_submit(ac4_server *s, const ac4_job *job) {
 static int admit(ac4_server *s, int j, int slot, double now) {
     job_rec *r = &s->jobs[j];
     const ac4_job *q = &r->job;
+    ac4_md *md = s->mds[s->slot_h[slot]];
+    const int ls = s->slot_l[slot];
     int tau = (int)floor(q->tau_p / 0.005 + 0.5); /* tau_p in time units, dt = 0.005 */
     if (tau < 1) tau = 1;
     ac4_replica_params p = {q->n, 1.2, q->T, q->f0, -1, tau, q->seed};
-    int rc = ac4_md_assign(s->md, slot, &p);
+    int rc = ac4_md_assign(md, ls, &p);
     if (rc != AC4_OK) return rc;
     if (r->ckpt) { /* a preempted job continues exactly where it stopped (its Q(t) origins and sums included) */
-        rc = ac4_md_restore(s->md, slot, r->ckpt, r->ckpt_len);
+        rc = ac4_md_restore(md, ls, r->ckpt, r->ckpt_len);
         if (rc != AC4_OK) return rc;
         free(r->ckpt);
         r->ckpt = NULL;
@@ -221,7 +285,7 @@ static int admit(ac4_server *s, int j, int slot, double now) {
         s->slot_job[slot] = j;
         return AC4_OK;
     }
-    if (s->ov_on && (rc = ac4_md_overlap_reset(s->md, slot)) != AC4_OK) return rc;
+    if (s->ov_on && (rc = ac4_md_overlap_reset(md, ls)) != AC4_OK) return rc;
     int n = q->n, nact = (int)((double)n * 0.1000);
     double *buf = malloc(sizeof(double) * 9 * (size_t)n);
     int *ib = malloc(sizeof(int) * (2 * (size_t)n + 3 * (size_t)(nact + 1)));
@@ -236,7 +300,7 @@ static int admit(ac4_server *s, int j, int slot, double now) {
     int *type = ib, *act = type + n, *kx = act + n, *ky = kx + nact + 1, *kz = ky + nact + 1;
     rc = ac4_init_lattice(n, q->T, nact, q->seed, x, y, z, xu, yu, zu, px, py, pz, type, act, kx, ky, kz);
     if (rc == AC4_OK)
-        rc = ac4_md_load_slot(s->md, slot, x, y, z, xu, yu, zu, px, py, pz, type, act, kx, ky, kz);
+        rc = ac4_md_load_slot(md, ls, x, y, z, xu, yu, zu, px, py, pz, type, act, kx, ky, kz);
     free(buf);
     free(ib);
     if (rc != AC4_OK) return rc;
@@ -244,8 +308,8 @@ static int admit(ac4_server *s, int j, int slot, double now) {
      * after the relaxation ramp */
     double c[9];
     ac4_chain_init(n, q->T, q->seed, c);
-    if ((rc = ac4_md_set_chain(s->md, slot, c)) != AC4_OK) return rc;
-    if ((rc = ac4_md_relax_slot(s->md, slot, s->relax, 1.5, 1.0)) != AC4_OK) return rc;
+    if ((rc = ac4_md_set_chain(md, ls, c)) != AC4_OK) return rc;
+    if ((rc = ac4_md_relax_slot(md, ls, s->relax, 1.5, 1.0)) != AC4_OK) return rc;
     r->state = 1;
     r->slot = slot;
     r->t_start = now;
@@ -256,9 +320,11 @@ static int admit(ac4_server *s, int j, int slot, double now) {
 /* A finished job: final x_u in reference order and its Q(t) sums. */
 static int retire(ac4_server *s, int k, job_rec *r) {
     const int n = r->job.n;
+    ac4_md *md = s->mds[s->slot_h[k]];
+    const int ls = s->slot_l[k];
     r->result = malloc(sizeof(double) * 3 * (size_t)n);
     if (!r->result) { ac4_set_error("retire: out of host memory"); return AC4_ECAP; }
-    int rc = ac4_md_get_state(s->md, k, NULL, NULL, NULL, r->result, r->result + n, r->result + 2 * n, NULL, NULL,
+    int rc = ac4_md_get_state(md, ls, NULL, NULL, NULL, r->result, r->result + n, r->result + 2 * n, NULL, NULL,
                               NULL, NULL, NULL, NULL);
     if (rc != AC4_OK) return rc;
     if (s->ov_on) {
@@ -266,7 +332,7 @@ static int retire(ac4_server *s, int k, job_rec *r) {
         r->ov = malloc(sizeof(double) * 3 * (size_t)nl);
         int *cnt = malloc(sizeof(int) * (size_t)nl);
         if (!r->ov || !cnt) { free(cnt); ac4_set_error("retire: out of host memory"); return AC4_ECAP; }
-        rc = ac4_md_overlap_get(s->md, k, r->ov, r->ov + nl, cnt);
+        rc = ac4_md_overlap_get(md, ls, r->ov, r->ov + nl, cnt);
         for (int l = 0; l < nl; l++) r->ov[2 * nl + l] = (double)cnt[l];
         free(cnt);
         if (rc != AC4_OK) return rc;
@@ -301,11 +367,25 @@ int ac4_server_tick(ac4_server *s, double now) {
     int running = 0;
     for (int k = 0; k < s->nslots; k++) running += s->slot_job[k] >= 0;
     if (!running) return 0;
-    int rc = ac4_md_step_graph(s->md, s->chunk);
+    /* the groups that hold a running job, stepped together */
+    ac4_md *act[64];
+    int gact[64], na = 0;
+    for (int g = 0; g < s->ng; g++) {
+        int busy = 0;
+        for (int k = 0; k < s->nslots; k++) busy |= s->slot_h[k] == g && s->slot_job[k] >= 0;
+        if (busy && na < 64) { gact[na] = g; act[na++] = s->mds[g]; }
+    }
+    int rc = ac4_md_step_graph_multi(act, na, s->chunk);
     if (rc != AC4_OK) return rc;
     /* every running job's frame of this chunk, counted on the device */
-    if (s->ov_on && (rc = ac4_md_overlap_sample(s->md)) != AC4_OK) return rc;
-    if ((rc = ac4_md_errors(s->md, s->err)) != AC4_OK) return rc;
+    for (int a = 0; a < na; a++)
+        if (s->ov_on && (rc = ac4_md_overlap_sample(act[a])) != AC4_OK) return rc;
+    for (int k = 0; k < s->nslots; k++) s->err[k] = 0;
+    for (int a = 0; a < na; a++) {
+        if ((rc = ac4_md_errors(act[a], s->gerr)) != AC4_OK) return rc;
+        for (int k = 0; k < s->nslots; k++)
+            if (s->slot_h[k] == gact[a]) s->err[k] = s->gerr[s->slot_l[k]];
+    }
     for (int k = 0; k < s->nslots; k++) {
         int j = s->slot_job[k];
         if (j < 0) continue;
@@ -315,7 +395,7 @@ int ac4_server_tick(ac4_server *s, double now) {
             r->err = s->err[k];
             r->t_done = now;
             s->slot_job[k] = -1;
-            if ((rc = ac4_md_release(s->md, k)) != AC4_OK) return rc;
+            if ((rc = ac4_md_release(s->mds[s->slot_h[k]], s->slot_l[k])) != AC4_OK) return rc;
             continue;
         }
         r->steps_done += s->chunk;
@@ -326,7 +406,7 @@ int ac4_server_tick(ac4_server *s, double now) {
             r->state = 2;
             r->t_done = now;
             s->slot_job[k] = -1;
-            if ((rc = ac4_md_release(s->md, k)) != AC4_OK) return rc;
+            if ((rc = ac4_md_release(s->mds[s->slot_h[k]], s->slot_l[k])) != AC4_OK) return rc;
         }
     }
     return running;
@@ -336,15 +416,17 @@ int ac4_server_preempt(ac4_server *s, int id) {
     if (!s || id < 0 || id >= s->njobs) { ac4_set_error("preempt: bad id"); return AC4_EARG; }
     job_rec *r = &s->jobs[id];
     if (r->state != 1) { ac4_set_error("preempt: job %d is not running", id); return AC4_ESTATE; }
-    long long len = ac4_md_checkpoint_size(s->md, r->slot);
+    ac4_md *md = s->mds[s->slot_h[r->slot]];
+    const int ls = s->slot_l[r->slot];
+    long long len = ac4_md_checkpoint_size(md, ls);
     if (len < 0) return (int)len;
     r->ckpt = malloc((size_t)len);
     if (!r->ckpt) { ac4_set_error("preempt: cannot allocate %lld bytes", len); return AC4_ECAP; }
-    int rc = ac4_md_checkpoint(s->md, r->slot, r->ckpt, len);
+    int rc = ac4_md_checkpoint(md, ls, r->ckpt, len);
     if (rc != AC4_OK) { free(r->ckpt); r->ckpt = NULL; return rc; }
     r->ckpt_len = len;
     s->slot_job[r->slot] = -1;
-    rc = ac4_md_release(s->md, r->slot);
+    rc = ac4_md_release(md, ls);
     r->slot = -1;
     r->state = 0; /* queued again; its arrival time keeps its place in the FIFO */
     r->preemptions++;
@@ -404,7 +486,15 @@ int ac4_server_job_overlap(const ac4_server *s, int id, double *sq, double *sq2,
 
 long long ac4_server_particle_steps(const ac4_server *s) { return s ? s->particle_steps : 0; }
 
-ac4_md *ac4_server_handle(ac4_server *s) { return s ? s->md : NULL; }
+ac4_md *ac4_server_handle(ac4_server *s) { return s ? s->mds[0] : NULL; }
+
+ac4_md *ac4_server_slot_handle(ac4_server *s, int slot, int *local) {
+    if (!s || slot < 0 || slot >= s->nslots) { ac4_set_error("slot_handle: bad slot"); return NULL; }
+    if (local) *local = s->slot_l[slot];
+    return s->mds[s->slot_h[slot]];
+}
+
+int ac4_server_groups(const ac4_server *s) { return s ? s->ng : AC4_EARG; }
 
 int ac4_server_job_slot(const ac4_server *s, int id) {
     if (!s || id < 0 || id >= s->njobs) { ac4_set_error("job_slot: bad id"); return AC4_EARG; }

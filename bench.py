@@ -221,6 +221,7 @@ def serving(ac4, njobs, rate_per_s, seed=4242):
     srv = ac4.Server(precision=ac4.FP32, chunk=100, relax=300, budget=SERVE_BUDGET, n_range=(500, 20000),
                      overlap=SERVE_OVERLAP)
     caps = srv.slots()
+    groups = srv.groups()
     ids = [srv.submit(j["n"], j["T"], j["f0"], j["tau_p"], j["steps"], j["seed"], j["t"]) for j in jobs]
     trace = []  # (time, particle-steps done, jobs waiting) after every tick
     t0 = time.perf_counter()
@@ -258,7 +259,8 @@ def serving(ac4, njobs, rate_per_s, seed=4242):
         seq_lat.append(free - j["t"])
     qq, _, qc = q1[0]
     return {"jobs": njobs, "arrival_rate_per_s": rate_per_s, "hbm_budget_bytes": SERVE_BUDGET,
-            "slots": len(caps), "slot_classes": {str(c): int((caps == c).sum()) for c in np.unique(caps)},
+            "slots": len(caps), "launch_groups": groups,
+            "slot_classes": {str(c): int((caps == c).sum()) for c in np.unique(caps)},
             "slot_capacity_particles": int(caps.sum()), "p50_latency_s": float(np.percentile(lat, 50)),
             "p99_latency_s": float(np.percentile(lat, 99)), "value": serve_rate, "unit": "particle-steps/s",
             "value_note": "all jobs' particle-steps / (last completion - first arrival)",

@@ -351,9 +351,17 @@ typedef struct {
 typedef struct ac4_server ac4_server;
 /* Slots of the given capacities; chunk = steps between frames;
  * relax_steps = velocity-rescale ramp before each job's production steps
- * (rounded up to whole chunks). */
+ * (rounded up to whole chunks).  The slots are split into launch groups of
+ * balanced capacity, one MD handle and stream each, stepped together (three
+ * groups when the slots hold at least 3 x 16384 particles, else one); every
+ * group uses the kernel shape of the server's total capacity, so a job's
+ * results do not depend on its group. */
 int ac4_server_create(int nslots, const int *slot_cap, int precision, int chunk,
                       int relax_steps, ac4_server **out);
+/* ac4_server_create with `groups` launch groups (at most one per slot; 0 = as
+ * ac4_server_create chooses). */
+int ac4_server_create_groups(int nslots, const int *slot_cap, int precision, int chunk,
+                             int relax_steps, int groups, ac4_server **out);
 /* Slot capacities from an HBM byte budget for requests with N in
  * [n_min, n_max]: classes a factor 2 apart, slot counts per class in
  * proportion to the share of a log-uniform N each class takes (at least one),
@@ -397,9 +405,12 @@ int ac4_server_result(const ac4_server *s, int id, double *xu, double *yu, doubl
 /* Per-lag overlap sums of a finished job (as ac4_md_overlap_get). */
 int ac4_server_job_overlap(const ac4_server *s, int id, double *sq, double *sq2, int *cnt);
 long long ac4_server_particle_steps(const ac4_server *s);
-/* The server's MD handle (kernel timings, error injection in tests) and the
- * slot a running job occupies (-1 when it is not running). */
+/* The first launch group's MD handle, the handle of the group holding a slot
+ * and the slot's index in it (error injection in tests), the number of launch
+ * groups, and the slot a running job occupies (-1 when it is not running). */
 ac4_md *ac4_server_handle(ac4_server *s);
+ac4_md *ac4_server_slot_handle(ac4_server *s, int slot, int *local);
+int ac4_server_groups(const ac4_server *s);
 int ac4_server_job_slot(const ac4_server *s, int id);
 
 /* ------------------------------------------------------------ Analysis --

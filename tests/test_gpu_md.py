@@ -399,13 +399,13 @@ def test_stats_consistent_with_state(gpu):
     assert st["steps"] == 30
 
 
-def _run_server(ac4, caps, jobs, chunk=100, overlap=None, budget=None, fail=None):
+def _run_server(ac4, caps, jobs, chunk=100, overlap=None, budget=None, fail=None, groups=0):
     """Runs the jobs to completion, one tick per time unit; fail = (job
     index, tick) raises an error flag on that job's replica after the tick."""
     if budget is not None:
         srv = ac4.Server(precision=ac4.FP32, chunk=chunk, relax=200, budget=budget, n_range=caps, overlap=overlap)
     else:
-        srv = ac4.Server(caps, precision=ac4.FP32, chunk=chunk, relax=200, overlap=overlap)
+        srv = ac4.Server(caps, precision=ac4.FP32, chunk=chunk, relax=200, overlap=overlap, groups=groups)
     ids = [srv.submit(*j) for j in jobs]
     t = 0
     while srv.pending() and t < 10000:
@@ -416,17 +416,22 @@ def _run_server(ac4, caps, jobs, chunk=100, overlap=None, budget=None, fail=None
     return srv, ids
 
 
-def test_server_job_alone_equals_in_mix(gpu):
+@pytest.mark.parametrize("groups", [1, 3])
+def test_server_job_alone_equals_in_mix(gpu, groups):
+    """Jobs in a mix (slots in one launch group, or in three groups stepped
+    together) give bitwise the results they give alone in a one-group
+    server of the same slots."""
     caps = [2048, 2048, 4096, 8192]
     # (n, T, f0, tau_p, steps, seed, t_arrive in ticks)
     jobs = [(1500, 0.5, 1.0, 1.0, 600, 11, 0), (3000, 0.6, 0.5, 0.1, 900, 12, 0), (1800, 0.45, 2.0, 10.0, 500, 13, 1),
             (7000, 0.5, 1.5, 1.0, 400, 14, 2), (1200, 0.8, 0.0, 1.0, 700, 15, 3), (2000, 0.5, 1.0, 0.5, 300, 16, 5)]
-    mix, ids = _run_server(gpu, caps, jobs)
+    mix, ids = _run_server(gpu, caps, jobs, groups=groups)
+    assert mix.groups() == groups
     st = [mix.job(i) for i in ids]
     assert all(s["state"] == 2 for s in st)
     assert st[5]["t_start"] > 0  # the third 2048-class job had to wait for a free slot
     for k in (0, 2, 3):
-        alone, aid = _run_server(gpu, caps, [jobs[k][:6] + (0,)])
+        alone, aid = _run_server(gpu, caps, [jobs[k][:6] + (0,)], groups=1)
         a, m = alone.result(aid[0]), mix.result(ids[k])
         for u in range(3):
             assert np.array_equal(a[u], m[u]), (k, u)
@@ -434,16 +439,19 @@ def test_server_job_alone_equals_in_mix(gpu):
     assert mix.particle_steps() > 0
 
 
-def test_server_failed_job_is_isolated(gpu):
+@pytest.mark.parametrize("groups", [1, 3])
+def test_server_failed_job_is_isolated(gpu, groups):
     """A job whose replica raises an error flag (set from the host, as a
     kernel sets it on a list overflow or a runaway coordinate) fails alone:
     its slot is cleared and serves a later job, and every other job finishes
-    bitwise as when it runs alone."""
+    bitwise as when it runs alone (in a one-group server; the mix in one or
+    three launch groups)."""
     caps = [2048, 2048, 4096]
     ov = (200, 4, 8, 0.3)
     jobs = [(1500, 0.5, 1.0, 1.0, 800, 31, 0), (1800, 0.6, 0.5, 0.1, 800, 32, 0), (3000, 0.5, 2.0, 10.0, 600, 33, 0),
             (1900, 0.7, 1.5, 1.0, 500, 34, 6)]
-    mix, ids = _run_server(gpu, caps, jobs, overlap=ov, fail=(1, 3))
+    mix, ids = _run_server(gpu, caps, jobs, overlap=ov, fail=(1, 3), groups=groups)
+    assert mix.groups() == groups
     st = [mix.job(i) for i in ids]
     assert st[1]["state"] == 3 and mix.job_error(ids[1]) == 2
     assert st[1]["steps_done"] == 400  # four chunks ran, the fifth saw the flag
@@ -452,7 +460,7 @@ def test_server_failed_job_is_isolated(gpu):
     assert [st[k]["state"] for k in (0, 2, 3)] == [2, 2, 2]
     assert st[3]["t_start"] >= 6
     for k in (0, 2, 3):
-        alone, aid = _run_server(gpu, caps, [jobs[k][:6] + (0,)], overlap=ov)
+        alone, aid = _run_server(gpu, caps, [jobs[k][:6] + (0,)], overlap=ov, groups=1)
         a, m = alone.result(aid[0]), mix.result(ids[k])
         for u in range(3):
             assert np.array_equal(a[u], m[u]), (k, u)
@@ -509,7 +517,8 @@ def test_server_preempted_job_resumes_exactly(gpu):
     ref = alone.result(aid[0])
     ref_q = alone.job_overlap(aid[0])
     alone.close()
-    srv = gpu.Server(caps, precision=gpu.FP32, chunk=100, relax=200, overlap=ov)
+    srv = gpu.Server(caps, precision=gpu.FP32, chunk=100, relax=200, overlap=ov, groups=2)  # one slot a group
+    assert srv.groups() == 2
     ia = srv.submit(*a_job)
     srv.tick(0.0)
     srv.tick(1.0)
