@@ -8,16 +8,18 @@
 // advances all of them.
 //
 // Per step (every launch covers all replicas, kernels early-exit per replica):
-//   k_shuffle      ShuffleActDirec when step % tau_per == 0 (sequential swap
-//                  chain of active.c:58-75 per component, Philox uniforms)
+//   k_shuffle      ShuffleActDirec when step % tau_per == 0 (swap chain of
+//                  active.c:58-75 per component, Philox uniforms); in the
+//                  first workgroups of k_first_half where its LDS is small
 //   k_first_half   U4 U3 U4 (chain start, computed redundantly per block),
 //                  U2 kick, U1 drift + wrap + max displacement (atomicMax)
-//   k_cell_zero .. k_nlist   counting-sort cell build + neighbour lists,
+//   k_bin .. k_nlist_team   counting-sort cell build + neighbour lists,
 //                  executed only for replicas whose displacement criterion
 //                  fired (flag read on device, so the sequence is graphable)
 //   k_force        pair forces + active force + second U2 kick + block
-//                  partial sums; the last block of each replica reduces the
-//                  partials in block order and runs U4 U3 U4 (chain end).
+//                  partial sums
+//   k_force_end    per replica: the partials summed in block order, U4 U3 U4
+//                  (chain end).
 // The momentum factor of the last U3 is kept pending (p_scale) and applied at
 // the next first half, so a step needs one sum of p^2 instead of four.
 #include <hip/hip_runtime.h>
@@ -86,7 +88,7 @@ struct ChainS {
     unsigned long long maxbits;
     long long step, shuffles;
     int rebuilds, force_rebuild, err, crit;
-    unsigned int done_blocks, bin_done;  // last-block counters of k_force and k_bin
+    unsigned int done_blocks, bin_done;  // no longer counted (kept for the checkpoint layout)
     int mode;                 // 0: Nose-Hoover chain, 1: velocity-rescale ramp, 2: NVE
     int ramp_n, ramp_k;
     double ramp0, ramp1;      // target T = T*(ramp0 + (ramp1-ramp0)*k/(n-1))
@@ -148,7 +150,7 @@ __device__ __forceinline__ bool need_rebuild(const RepC &R, const ChainS &c) {
 }
 
 // Rebuild work of one step: the blocks of the replicas due for a rebuild,
-// appended by the last k_bin block of each such replica, so the kernels after
+// appended by k_bin_end for each such replica, so the kernels after
 // k_bin run a fixed grid over this list instead of one workgroup per block of
 // every replica (most of which would have nothing to do).  n[0]: particle
 // blocks of BLK (global block ids); n[1]: list blocks of LPB particles
@@ -484,24 +486,21 @@ __device__ __forceinline__ int clampc(int c, int n) { return min(max(c, 0), n - 
 // Binning: the reference cell id (bit-exact, for update_cell_list) and the
 // list-grid cell of every particle, with the place within its cell taken by
 // an atomic on the cell's count; the particle's (slot, key) is written to
-// that place of its cell's row of CELL_CAP entries (key: the octant of the
-// cell the particle is in, then its id; the order k_cell_sort puts the cell
-// in) (the reference's
+// that place of its cell's row of CELL_CAP entries (the reference's
 // InCell[cell][MaxPartInCell], MDcode/global.h:20,85; a fuller cell sets the
-// capacity error).  The last block of a replica to finish turns the cell
-// counts into start offsets (exclusive scan).
+// capacity error).  key: the octant of the cell the particle is in, then its
+// id, the order k_cell_sort puts the cell in.  k_bin_end then turns the cell
+// counts into start offsets.
 #define CELL_CAP 600
 __global__ void __launch_bounds__(BLK) k_bin(const RepC *__restrict__ reps, ChainS *chain,
                                              const int *__restrict__ blk_rep, const int *__restrict__ blk_first,
                                              const double *__restrict__ x, const double *__restrict__ y,
                                              const double *__restrict__ z, const int *__restrict__ id, int *cnt,
-                                             int *cell_of, int2 *cell_mem,
-                                             int *cell_ref, int *start, WorkList wl) {
-    __shared__ int s_last, s_w[BLK / 64], s_base[3];
+                                             int *cell_of, int2 *cell_mem, int *cell_ref) {
     const int b = blockIdx.x, r = blk_rep[b];
     const RepC R = reps[r];
     if (!need_rebuild(R, chain[r])) return;  // uniform over the replica's blocks
-    const int i = blk_first[b] + threadIdx.x, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int i = blk_first[b] + threadIdx.x, lane = threadIdx.x & 63;
     const bool valid = i < R.n;
     const int g = R.off + i;
     int c = -1, key = 0;
@@ -550,20 +549,22 @@ __global__ void __launch_bounds__(BLK) k_bin(const RepC *__restrict__ reps, Chai
         if (my_rank < CELL_CAP) cell_mem[(size_t)(R.off_cell + c) * CELL_CAP + my_rank] = make_int2(i, key);
         else atomicOr(&chain[r].err, 1);
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        const unsigned int prev = atomicAdd(&chain[r].bin_done, 1u);
-        s_last = prev == (unsigned int)(R.nblk - 1);
-    }
-    __syncthreads();
-    if (!s_last) return;
-    __threadfence();
-    // exclusive scan of the replica's cell counts (counts written by atomics
-    // of other blocks: read through volatile)
-    const volatile int *vc = cnt + R.off_cell;
+}
+
+// After k_bin, one workgroup per replica due for a rebuild: the cell counts
+// turned into start offsets (exclusive scan) and the replica's share of the
+// step's rebuild work appended to the work list.  A kernel of its own rather
+// than the last k_bin block of each replica, which needs an agent-scope
+// fence per block (on gfx950 a write-back and invalidation of the XCD's L2).
+__global__ void __launch_bounds__(BLK) k_bin_end(const RepC *__restrict__ reps, const ChainS *__restrict__ chain,
+                                                 const int *__restrict__ cnt, int *start, WorkList wl) {
+    __shared__ int s_w[BLK / 64], s_base[3];
+    const int r = blockIdx.x, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const RepC R = reps[r];
+    if (!need_rebuild(R, chain[r])) return;  // uniform over the workgroup
+    const int *vc = cnt + R.off_cell;
     const int per = (R.ncell + BLK - 1) / BLK;
-    const int c0 = min(threadIdx.x * per, R.ncell), c1 = min(c0 + per, R.ncell);
+    const int c0 = min((int)threadIdx.x * per, R.ncell), c1 = min(c0 + per, R.ncell);
     int own = 0;
     for (int q = c0; q < c1; q++) own += vc[q];
     int v = own;
@@ -585,7 +586,6 @@ __global__ void __launch_bounds__(BLK) k_bin(const RepC *__restrict__ reps, Chai
         s_base[0] = atomicAdd(&wl.n[0], nb);
         s_base[1] = atomicAdd(&wl.n[1], nl);
         s_base[2] = atomicAdd(&wl.n[2], ncb);
-        chain[r].bin_done = 0u;
     }
     __syncthreads();
     for (int q = threadIdx.x; q < nb; q += BLK) wl.item[s_base[0] + q] = R.blk_off + q;
@@ -1407,9 +1407,11 @@ __device__ __forceinline__ void walk_list1(const RepC &R, const int *lp, int m, 
 // Pair forces: a team of tpp lanes per particle (tpp is fixed per handle at
 // creation, so a replica's results do not depend on what else is loaded) walks the particle's
 // list with stride tpp, 8 independent index/position loads in flight per lane;
-// the team's partial forces are combined by a fixed butterfly.
+// the team's partial forces are combined by a fixed butterfly.  Each block
+// writes its partial sums of p^2, PE, virial and p_x; k_force_end reduces
+// them per replica.
 template <bool FP32, bool KICK>
-__global__ void __launch_bounds__(BLK) k_force(const RepC *__restrict__ reps, ChainS *chain,
+__global__ void __launch_bounds__(BLK) k_force(const RepC *__restrict__ reps,
                                                const int *__restrict__ fblk_rep, const int *__restrict__ fblk_first,
                                                const double4 *__restrict__ pos4, const QPos *__restrict__ posq,
                                                const int *__restrict__ nn,
@@ -1418,7 +1420,6 @@ __global__ void __launch_bounds__(BLK) k_force(const RepC *__restrict__ reps, Ch
                                                const int8_t *__restrict__ kz, double *fx, double *fy, double *fz,
                                                double *px, double *py, double *pz, double *partial, int nb) {
     __shared__ double s_red[4][BLK / 64];
-    __shared__ int s_last;
     const int b = xcd_block(nb);
     if (b < 0) return;
     const int r = fblk_rep[b];
@@ -1505,34 +1506,37 @@ __global__ void __launch_bounds__(BLK) k_force(const RepC *__restrict__ reps, Ch
             for (int w = 0; w < BLK / 64; w++) s += s_red[q][w];
             partial[4 * (size_t)b + q] = s;
         }
-        __threadfence();
-        unsigned int prev = atomicAdd(&chain[r].done_blocks, 1u);
-        s_last = (prev == (unsigned int)(R.fnblk - 1));
     }
-    __syncthreads();
-    if (!s_last || threadIdx.x >= 64) return;
-    // last block of replica r: the first wave sums the partials in a fixed
-    // order (lane t takes blocks t, t+64, ...; then a fixed butterfly)
-    __threadfence();
+}
+
+// End of the force pass, one wave per replica after k_force: the block
+// partials summed in a fixed order (lane t takes blocks t, t+64, ...; then a
+// fixed butterfly), then the end-of-step chain update.  A kernel of its own
+// rather than the last force block of each replica: that pattern needs an
+// agent-scope release fence in every block, which on gfx950 writes back and
+// invalidates the L2 of the block's XCD (buffer_wbl2 / buffer_inv), and the
+// invalidations cost the neighbour gathers of every other block their L2
+// hits (k_force 55.6 -> 42.3 us on the 64-replica ensemble without them).
+template <bool KICK>
+__global__ void __launch_bounds__(64) k_force_end(const RepC *__restrict__ reps, ChainS *chain,
+                                                  const double *__restrict__ partial) {
+    const int r = blockIdx.x;
+    const RepC R = reps[r];
     double a4[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int k = threadIdx.x; k < R.fnblk; k += 64) {  // volatile: written by other blocks
-        const volatile double *pp = partial + 4 * (size_t)(R.fblk_off + k);
+    for (int k = threadIdx.x; k < R.fnblk; k += 64) {
+        const double *pp = partial + 4 * (size_t)(R.fblk_off + k);
         a4[0] += pp[0]; a4[1] += pp[1]; a4[2] += pp[2]; a4[3] += pp[3];
     }
     for (int q = 0; q < 4; q++)
         for (int o = 1; o < 64; o <<= 1) a4[q] += __shfl_xor(a4[q], o, 64);
     if (threadIdx.x != 0) return;
     ChainS &c = chain[r];
-    if (R.n == 0) {  // idle slot
-        c.done_blocks = 0u;
-        return;
-    }
+    if (R.n == 0) return;  // idle slot
     double S = a4[0], pe = a4[1], vir = a4[2], spx = a4[3];
     pe = 0.5 * pe;                  // every pair seen from both sides
     vir = (0.5 * vir) / (3.0e0);   // cell_list+forces.c:220
     c.pe = pe;
     c.vir = vir;
-    c.done_blocks = 0u;
     if (!KICK) return;
     const bool rebuilt = need_rebuild(R, c);
     if (c.mode == 2) {
@@ -2584,8 +2588,8 @@ static int launch_rebuild(ac4_md *md, bool with_zero = false) {
     if (with_zero)
         hipLaunchKernelGGL(k_cell_zero, dim3(md->R, cy), dim3(BLK), 0, s, md->reps, md->chain, md->cnt, md->wl_n);
     hipLaunchKernelGGL(k_bin, dim3(md->nblk), dim3(BLK), 0, s, md->reps, md->chain, md->blk_rep, md->blk_first,
-                       md->P.x, md->P.y, md->P.z, md->P.id, md->cnt, md->P.cell_of, md->cell_mem, md->P.cell_ref,
-                       md->start, wl);
+                       md->P.x, md->P.y, md->P.z, md->P.id, md->cnt, md->P.cell_of, md->cell_mem, md->P.cell_ref);
+    hipLaunchKernelGGL(k_bin_end, dim3(md->R), dim3(BLK), 0, s, md->reps, md->chain, md->cnt, md->start, wl);
     hipLaunchKernelGGL(k_cell_sort, dim3(gc), dim3(BLK), 0, s, md->reps, wl, md->cnt, md->start,
                        md->cell_mem, md->members, md->ranges, md->P, md->A,
                        PosCopies{md->pos4, md->posq, md->xo, md->yo, md->zo});
@@ -2623,15 +2627,16 @@ template <bool KICK>
 static int launch_force(ac4_md *md) {
     hipStream_t s = md->stream;
     if (md->prec == AC4_FP32)
-        hipLaunchKernelGGL((k_force<true, KICK>), dim3(xcd_grid(md->fnblk)), dim3(BLK), 0, s, md->reps, md->chain,
+        hipLaunchKernelGGL((k_force<true, KICK>), dim3(xcd_grid(md->fnblk)), dim3(BLK), 0, s, md->reps,
                            md->fblk_rep, md->fblk_first, md->pos4, md->posq, md->nn, md->nl, md->cap, md->P.act, md->kx,
                            md->ky, md->kz, md->fx, md->fy, md->fz, md->P.px, md->P.py, md->P.pz, md->partial,
                            md->fnblk);
     else
-        hipLaunchKernelGGL((k_force<false, KICK>), dim3(xcd_grid(md->fnblk)), dim3(BLK), 0, s, md->reps, md->chain,
+        hipLaunchKernelGGL((k_force<false, KICK>), dim3(xcd_grid(md->fnblk)), dim3(BLK), 0, s, md->reps,
                            md->fblk_rep, md->fblk_first, md->pos4, md->posq, md->nn, md->nl, md->cap, md->P.act, md->kx,
                            md->ky, md->kz, md->fx, md->fy, md->fz, md->P.px, md->P.py, md->P.pz, md->partial,
                            md->fnblk);
+    hipLaunchKernelGGL(k_force_end<KICK>, dim3(md->R), dim3(64), 0, s, md->reps, md->chain, md->partial);
     AC4_HIP(hipGetLastError());
     return AC4_OK;
 }
