@@ -556,14 +556,18 @@ __global__ void __launch_bounds__(BLK) k_bin(const RepC *__restrict__ reps, Chai
 // step's rebuild work appended to the work list.  A kernel of its own rather
 // than the last k_bin block of each replica, which needs an agent-scope
 // fence per block (on gfx950 a write-back and invalidation of the XCD's L2).
-__global__ void __launch_bounds__(BLK) k_bin_end(const RepC *__restrict__ reps, const ChainS *__restrict__ chain,
-                                                 const int *__restrict__ cnt, int *start, WorkList wl) {
-    __shared__ int s_w[BLK / 64], s_base[3];
-    const int r = blockIdx.x, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+// BIN_END_T threads per replica where a slot holds more than BIN_END_T cells
+// (fixed per handle at creation), else BLK.
+#define BIN_END_T 1024
+__global__ void __launch_bounds__(BIN_END_T) k_bin_end(const RepC *__restrict__ reps,
+                                                       const ChainS *__restrict__ chain, const int *__restrict__ cnt,
+                                                       int *start, WorkList wl) {
+    __shared__ int s_w[BIN_END_T / 64], s_base[3];
+    const int r = blockIdx.x, lane = threadIdx.x & 63, w = threadIdx.x >> 6, nt = blockDim.x;
     const RepC R = reps[r];
     if (!need_rebuild(R, chain[r])) return;  // uniform over the workgroup
     const int *vc = cnt + R.off_cell;
-    const int per = (R.ncell + BLK - 1) / BLK;
+    const int per = (R.ncell + nt - 1) / nt;
     const int c0 = min((int)threadIdx.x * per, R.ncell), c1 = min(c0 + per, R.ncell);
     int own = 0;
     for (int q = c0; q < c1; q++) own += vc[q];
@@ -588,9 +592,9 @@ __global__ void __launch_bounds__(BLK) k_bin_end(const RepC *__restrict__ reps, 
         s_base[2] = atomicAdd(&wl.n[2], ncb);
     }
     __syncthreads();
-    for (int q = threadIdx.x; q < nb; q += BLK) wl.item[s_base[0] + q] = R.blk_off + q;
-    for (int q = threadIdx.x; q < nl; q += BLK) wl.item[wl.off_l + s_base[1] + q] = R.lblk_off + q;
-    for (int q = threadIdx.x; q < ncb; q += BLK) wl.item[wl.off_c + s_base[2] + q] = (r << 16) | q;
+    for (int q = threadIdx.x; q < nb; q += nt) wl.item[s_base[0] + q] = R.blk_off + q;
+    for (int q = threadIdx.x; q < nl; q += nt) wl.item[wl.off_l + s_base[1] + q] = R.lblk_off + q;
+    for (int q = threadIdx.x; q < ncb; q += nt) wl.item[wl.off_c + s_base[2] + q] = (r << 16) | q;
 }
 
 // Distinct neighbour-cell coordinates along one axis (cells are at least one
@@ -1509,27 +1513,37 @@ __global__ void __launch_bounds__(BLK) k_force(const RepC *__restrict__ reps,
     }
 }
 
-// End of the force pass, one wave per replica after k_force: the block
-// partials summed in a fixed order (lane t takes blocks t, t+64, ...; then a
-// fixed butterfly), then the end-of-step chain update.  A kernel of its own
+// End of the force pass, one workgroup per replica after k_force: the block
+// partials summed in a fixed order (thread t of nt takes blocks t, t+nt, ...;
+// a fixed butterfly per wave; the waves in order), then the end-of-step chain
+// update.  nt = 64, or 256 where a replica has more than 64 force blocks
+// (fixed per handle at creation).  A kernel of its own
 // rather than the last force block of each replica: that pattern needs an
 // agent-scope release fence in every block, which on gfx950 writes back and
 // invalidates the L2 of the block's XCD (buffer_wbl2 / buffer_inv), and the
 // invalidations cost the neighbour gathers of every other block their L2
 // hits (k_force 55.6 -> 42.3 us on the 64-replica ensemble without them).
 template <bool KICK>
-__global__ void __launch_bounds__(64) k_force_end(const RepC *__restrict__ reps, ChainS *chain,
-                                                  const double *__restrict__ partial) {
-    const int r = blockIdx.x;
+__global__ void __launch_bounds__(256) k_force_end(const RepC *__restrict__ reps, ChainS *chain,
+                                                   const double *__restrict__ partial) {
+    __shared__ double s_w[4][4];
+    const int r = blockIdx.x, lane = threadIdx.x & 63, w = threadIdx.x >> 6, nt = blockDim.x;
     const RepC R = reps[r];
     double a4[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int k = threadIdx.x; k < R.fnblk; k += 64) {
+    for (int k = threadIdx.x; k < R.fnblk; k += nt) {  // thread t: blocks t, t+nt, ...
         const double *pp = partial + 4 * (size_t)(R.fblk_off + k);
         a4[0] += pp[0]; a4[1] += pp[1]; a4[2] += pp[2]; a4[3] += pp[3];
     }
     for (int q = 0; q < 4; q++)
         for (int o = 1; o < 64; o <<= 1) a4[q] += __shfl_xor(a4[q], o, 64);
+    if (lane == 0)
+        for (int q = 0; q < 4; q++) s_w[w][q] = a4[q];
+    __syncthreads();
     if (threadIdx.x != 0) return;
+    for (int q = 0; q < 4; q++) {
+        a4[q] = s_w[0][q];
+        for (int k = 1; k < nt / 64; k++) a4[q] += s_w[k][q];
+    }
     ChainS &c = chain[r];
     if (R.n == 0) return;  // idle slot
     double S = a4[0], pe = a4[1], vir = a4[2], spx = a4[3];
@@ -1596,6 +1610,7 @@ __global__ void __launch_bounds__(64) k_force_end(const RepC *__restrict__ reps,
 // ------------------------------------------------------------- host side
 struct ac4_md {
     int R = 0, prec = 0, ntot = 0, nblk = 0, ncell_tot = 0, nact_tot = 0, cap = 192, stride = 0, max_ncell = 0;
+    int max_fnblk = 0;  // force blocks of the largest slot
     std::vector<RepC> reps_h;
     RepC *reps = nullptr;
     ChainS *chain = nullptr;
@@ -1817,6 +1832,7 @@ extern "C" int ac4_md_create_slots_rule(int nslots, const int *cap_n, int precis
         blk_off += R.nblk;
         fblk_off += R.fnblk;
         md->max_ncell = std::max(md->max_ncell, cap_cells);
+        md->max_fnblk = std::max(md->max_fnblk, R.fnblk);
         md->reps_h.push_back(R);  // idle: n = 0
     }
     md->t_brep = brep; md->t_bfirst = bfirst; md->t_bidx = bidx;
@@ -2589,7 +2605,8 @@ static int launch_rebuild(ac4_md *md, bool with_zero = false) {
         hipLaunchKernelGGL(k_cell_zero, dim3(md->R, cy), dim3(BLK), 0, s, md->reps, md->chain, md->cnt, md->wl_n);
     hipLaunchKernelGGL(k_bin, dim3(md->nblk), dim3(BLK), 0, s, md->reps, md->chain, md->blk_rep, md->blk_first,
                        md->P.x, md->P.y, md->P.z, md->P.id, md->cnt, md->P.cell_of, md->cell_mem, md->P.cell_ref);
-    hipLaunchKernelGGL(k_bin_end, dim3(md->R), dim3(BLK), 0, s, md->reps, md->chain, md->cnt, md->start, wl);
+    hipLaunchKernelGGL(k_bin_end, dim3(md->R), dim3(md->max_ncell > BIN_END_T ? BIN_END_T : BLK), 0, s, md->reps, md->chain,
+                       md->cnt, md->start, wl);
     hipLaunchKernelGGL(k_cell_sort, dim3(gc), dim3(BLK), 0, s, md->reps, wl, md->cnt, md->start,
                        md->cell_mem, md->members, md->ranges, md->P, md->A,
                        PosCopies{md->pos4, md->posq, md->xo, md->yo, md->zo});
@@ -2636,7 +2653,8 @@ static int launch_force(ac4_md *md) {
                            md->fblk_rep, md->fblk_first, md->pos4, md->posq, md->nn, md->nl, md->cap, md->P.act, md->kx,
                            md->ky, md->kz, md->fx, md->fy, md->fz, md->P.px, md->P.py, md->P.pz, md->partial,
                            md->fnblk);
-    hipLaunchKernelGGL(k_force_end<KICK>, dim3(md->R), dim3(64), 0, s, md->reps, md->chain, md->partial);
+    hipLaunchKernelGGL(k_force_end<KICK>, dim3(md->R), dim3(md->max_fnblk > 64 ? 256 : 64), 0, s, md->reps, md->chain,
+                       md->partial);
     AC4_HIP(hipGetLastError());
     return AC4_OK;
 }
