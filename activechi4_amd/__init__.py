@@ -429,10 +429,11 @@ class MD:
         _check(lib.ac4_md_overlap_reset(self.h, r))
 
     def kernel_times(self, nsteps):
-        """Average ms per step of {first_half, rebuild, force, shuffle} (HIP events)."""
-        o = np.empty(4)
+        """Average ms per step of {first_half, rebuild, force, shuffle,
+        force_end} (HIP events; force is k_force alone)."""
+        o = np.empty(5)
         _check(lib.ac4_md_kernel_times(self.h, nsteps, _dp(o)))
-        return dict(zip(["first_half", "rebuild", "force", "shuffle"], o))
+        return dict(zip(["first_half", "rebuild", "force", "shuffle", "force_end"], o))
 
 
 class Analysis:

@@ -2641,7 +2641,7 @@ static int launch_rebuild(ac4_md *md, bool with_zero = false) {
 }
 
 template <bool KICK>
-static int launch_force(ac4_md *md) {
+static int launch_force_pass(ac4_md *md) {
     hipStream_t s = md->stream;
     if (md->prec == AC4_FP32)
         hipLaunchKernelGGL((k_force<true, KICK>), dim3(xcd_grid(md->fnblk)), dim3(BLK), 0, s, md->reps,
@@ -2653,10 +2653,20 @@ static int launch_force(ac4_md *md) {
                            md->fblk_rep, md->fblk_first, md->pos4, md->posq, md->nn, md->nl, md->cap, md->P.act, md->kx,
                            md->ky, md->kz, md->fx, md->fy, md->fz, md->P.px, md->P.py, md->P.pz, md->partial,
                            md->fnblk);
-    hipLaunchKernelGGL(k_force_end<KICK>, dim3(md->R), dim3(md->max_fnblk > 64 ? 256 : 64), 0, s, md->reps, md->chain,
-                       md->partial);
     AC4_HIP(hipGetLastError());
     return AC4_OK;
+}
+template <bool KICK>
+static int launch_force_end(ac4_md *md) {
+    hipLaunchKernelGGL(k_force_end<KICK>, dim3(md->R), dim3(md->max_fnblk > 64 ? 256 : 64), 0, md->stream, md->reps,
+                       md->chain, md->partial);
+    AC4_HIP(hipGetLastError());
+    return AC4_OK;
+}
+template <bool KICK>
+static int launch_force(ac4_md *md) {
+    TRY(launch_force_pass<KICK>(md));
+    return launch_force_end<KICK>(md);
 }
 
 static void launch_shuffle(ac4_md *md, int force, hipStream_t s) {
@@ -2829,16 +2839,17 @@ extern "C" int ac4_md_relax(ac4_md *md, int nsteps, double f_start, double f_end
 }
 
 // Eager steps with HIP events (on the handle's stream) around each launch
-// group; ms4 = average ms per step of {first_half, rebuild group, force,
-// shuffle}; where the shuffle runs inside k_first_half its time is in the
-// first entry and the last is the empty interval between two events.
-extern "C" int ac4_md_kernel_times(ac4_md *md, int nsteps, double *ms4) {
-    AC4_REQUIRE(md && nsteps > 0 && ms4, AC4_EARG, "kernel_times: bad arguments");
+// group; ms = average ms per step of {first_half, rebuild group, force,
+// shuffle, force end}: the force entry is k_force alone, the last k_force_end;
+// where the shuffle runs inside k_first_half its time is in the first entry
+// and the fourth is the empty interval between two events.
+extern "C" int ac4_md_kernel_times(ac4_md *md, int nsteps, double *ms) {
+    AC4_REQUIRE(md && nsteps > 0 && ms, AC4_EARG, "kernel_times: bad arguments");
     if (!md->have_forces) TRY(ac4_md_calculate_forces(md, nullptr));
     hipStream_t s = md->stream;
-    hipEvent_t ev[5];
+    hipEvent_t ev[6];
     for (auto &e : ev) AC4_HIP(hipEventCreate(&e));
-    double acc[4] = {0, 0, 0, 0};
+    double acc[5] = {0, 0, 0, 0, 0};
     for (int k = 0; k < nsteps; k++) {
         AC4_HIP(hipEventRecord(ev[0], s));
         if (!md->shuf_fused) launch_shuffle(md, 0, s);
@@ -2847,15 +2858,17 @@ extern "C" int ac4_md_kernel_times(ac4_md *md, int nsteps, double *ms4) {
         AC4_HIP(hipEventRecord(ev[2], s));
         TRY(launch_rebuild(md));
         AC4_HIP(hipEventRecord(ev[3], s));
-        TRY(launch_force<true>(md));
+        TRY(launch_force_pass<true>(md));
         AC4_HIP(hipEventRecord(ev[4], s));
-        AC4_HIP(hipEventSynchronize(ev[4]));
-        float t[4];
-        for (int q = 0; q < 4; q++) AC4_HIP(hipEventElapsedTime(&t[q], ev[q], ev[q + 1]));
-        acc[3] += t[0]; acc[0] += t[1]; acc[1] += t[2]; acc[2] += t[3];
+        TRY(launch_force_end<true>(md));
+        AC4_HIP(hipEventRecord(ev[5], s));
+        AC4_HIP(hipEventSynchronize(ev[5]));
+        float t[5];
+        for (int q = 0; q < 5; q++) AC4_HIP(hipEventElapsedTime(&t[q], ev[q], ev[q + 1]));
+        acc[3] += t[0]; acc[0] += t[1]; acc[1] += t[2]; acc[2] += t[3]; acc[4] += t[4];
     }
     for (auto &e : ev) (void)hipEventDestroy(e);
-    for (int q = 0; q < 4; q++) ms4[q] = acc[q] / nsteps;
+    for (int q = 0; q < 5; q++) ms[q] = acc[q] / nsteps;
     return check_err(md);
 }
 

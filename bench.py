@@ -706,6 +706,9 @@ def main():
                     help="the analysis rows (frame text I/O, block analysis, g_uu, cage overlap), GPU vs oracle")
     ap.add_argument("--only-batched", action="store_true",
                     help="batched ensemble only (no plain, serving or CPU legs), e.g. under a profiler")
+    ap.add_argument("--tpp", type=int, default=0,
+                    help="force lanes per particle for --no-tune and --pmc-probe runs (0: automatic), e.g. to "
+                         "profile the shape the autotuner chose")
     ap.add_argument("--eager", action="store_true",
                     help="timed steps as eager launches instead of graph replays (for kernel-trace profiles)")
     a = ap.parse_args()
@@ -715,6 +718,8 @@ def main():
         sys.exit(2)
     import activechi4_amd as ac4
     ac4.require_gpu()
+    if a.tpp:
+        ac4.set_tuning(a.tpp, 3)
     if a.pmc_probe:
         pmc_probe(ac4)
         return
@@ -846,8 +851,10 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": pmc_traffic(),
                      "kernel": "k_force<fp32,kick>", "bytes_per_particle": FORCE_BYTES_PER_PARTICLE,
                      "particles_per_launch": ntot,
-                     "note": "one k_force launch over the whole ensemble (HIP events, eager); the timed run "
-                             "splits the same particles into %d concurrent launch groups" % groups,
+                     "note": "one k_force launch over the whole ensemble at the timed run's kernel shape "
+                             "(HIP events, eager; the per-replica sums and chain update, k_force_end, are "
+                             "timed apart); the timed run splits the same particles into %d concurrent "
+                             "launch groups" % groups,
                      "whole_step": {"bytes_per_particle_step": STEP_BYTES_PER_PARTICLE,
                                     "achieved": value * STEP_BYTES_PER_PARTICLE / 1e9,
                                     "frac": value * STEP_BYTES_PER_PARTICLE / 1e9 / HBM_PEAK_GBS}},

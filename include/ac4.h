@@ -140,11 +140,12 @@ int ac4_md_stats(ac4_md *md, int r, double *out8);
 /* Neighbour-list rebuilds performed by replica r so far. */
 int ac4_md_rebuilds(ac4_md *md, int r);
 /* Measurement hook (no reference counterpart): nsteps eager steps timed with
- * HIP events on the handle's stream around each launch group; ms4 = average
- * milliseconds per step of {first half, rebuild group, force, shuffle}.  Where
- * the handle runs the shuffle inside the first-half kernel (small active
- * sets), the shuffle's time is part of the first entry. */
-int ac4_md_kernel_times(ac4_md *md, int nsteps, double *ms4);
+ * HIP events on the handle's stream around each launch group; ms (5 doubles)
+ * = average milliseconds per step of {first half, rebuild group, force pass,
+ * shuffle, force end (per-replica sums and chain update)}.  Where the handle
+ * runs the shuffle inside the first-half kernel (small active sets), the
+ * shuffle's time is part of the first entry. */
+int ac4_md_kernel_times(ac4_md *md, int nsteps, double *ms);
 /* Waits for all device work of the handle; reports deferred device errors
  * (list/cell overflow). */
 int ac4_md_sync(ac4_md *md);
