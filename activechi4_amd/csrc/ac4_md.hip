@@ -181,7 +181,7 @@ static inline int xcd_grid(int nb) { return ((nb + N_XCD - 1) / N_XCD) * N_XCD; 
 // on the array, so the 256 lanes draw them SHUF_CH at a time into LDS and one
 // lane applies the swaps in order.  The component lives in dynamic LDS when
 // it fits (kcap bytes, sized from the device's LDS per workgroup), else in
-// global memory.  The shuffle counter is advanced by the force epilogue (or
+// global memory.  The shuffle counter is advanced by k_force_end (or
 // the host for an explicit shuffle), after every component has read it.
 #define SHUF_CH 4096
 #define SHUF_PAR_MIN 16384  // active sets above this take the parallel path (k_shuf_part, k_shuf_apply)
