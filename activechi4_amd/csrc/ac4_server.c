@@ -124,11 +124,12 @@ int ac4_server_plan(double hbm_bytes, int n_min, int n_max, int n_origin, int n_
  * the process's four hardware queues), else one. */
 #define SRV_GROUPS 3
 #define SRV_GROUP_MIN 16384
+#define SRV_GROUPS_MAX 64
 
 int ac4_server_create_groups(int nslots, const int *slot_cap, int precision, int chunk, int relax_steps,
                              int groups, ac4_server **out) {
-    if (nslots < 1 || !slot_cap || chunk < 1 || relax_steps < 1 || groups < 0 || !out) {
-        ac4_set_error("ac4_server_create: bad arguments");
+    if (nslots < 1 || !slot_cap || chunk < 1 || relax_steps < 1 || groups < 0 || groups > SRV_GROUPS_MAX || !out) {
+        ac4_set_error("ac4_server_create: bad arguments (groups in 0..%d)", SRV_GROUPS_MAX);
         return AC4_EARG;
     }
     ac4_server *s = calloc(1, sizeof *s);
@@ -368,12 +369,12 @@ int ac4_server_tick(ac4_server *s, double now) {
     for (int k = 0; k < s->nslots; k++) running += s->slot_job[k] >= 0;
     if (!running) return 0;
     /* the groups that hold a running job, stepped together */
-    ac4_md *act[64];
-    int gact[64], na = 0;
+    ac4_md *act[SRV_GROUPS_MAX];
+    int gact[SRV_GROUPS_MAX], na = 0;
     for (int g = 0; g < s->ng; g++) {
         int busy = 0;
         for (int k = 0; k < s->nslots; k++) busy |= s->slot_h[k] == g && s->slot_job[k] >= 0;
-        if (busy && na < 64) { gact[na] = g; act[na++] = s->mds[g]; }
+        if (busy) { gact[na] = g; act[na++] = s->mds[g]; }
     }
     int rc = ac4_md_step_graph_multi(act, na, s->chunk);
     if (rc != AC4_OK) return rc;

@@ -359,8 +359,8 @@ typedef struct ac4_server ac4_server;
  * results do not depend on its group. */
 int ac4_server_create(int nslots, const int *slot_cap, int precision, int chunk,
                       int relax_steps, ac4_server **out);
-/* ac4_server_create with `groups` launch groups (at most one per slot; 0 = as
- * ac4_server_create chooses). */
+/* ac4_server_create with `groups` launch groups (0 = as ac4_server_create
+ * chooses; at most 64, and at most one per slot). */
 int ac4_server_create_groups(int nslots, const int *slot_cap, int precision, int chunk,
                              int relax_steps, int groups, ac4_server **out);
 /* Slot capacities from an HBM byte budget for requests with N in

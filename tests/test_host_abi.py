@@ -42,6 +42,14 @@ def test_no_silent_cpu_path(ac4):
         ac4.Analysis(100, 4.4, [0, 1, 2, 3, 4, 5, 20, 21, 22, 23, 24, 25])
 
 
+def test_server_group_count_is_checked(ac4):
+    """Launch-group counts outside 0..64 are refused before anything touches
+    the device (a tick steps at most 64 groups)."""
+    for g in (-1, 65):
+        with pytest.raises(ac4.AC4Error, match="groups in 0..64"):
+            ac4.Server([2048, 2048, 4096], groups=g)
+
+
 @pytest.mark.parametrize("ctr,key,want", [
     ([0, 0, 0, 0], [0, 0], [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]),
     ([0xffffffff] * 4, [0xffffffff] * 2, [0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd]),
