@@ -1472,7 +1472,17 @@ __global__ void __launch_bounds__(BLK) k_force(const RepC *__restrict__ reps,
 #pragma unroll
             for (int u = 0; u < 8; u++) acc.add(pi, ti, q[u], R);
         }
-        for (; k < m; k += tpp, lp += cstep) acc.add(pi, ti, pos[*lp], R);
+        if (k < m) {  // the last, partial group: its loads issued together, the adds in list order
+            int jt[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) jt[u] = k + u * tpp < m ? lp[u * cstep] : g;
+            PosT q[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) q[u] = pos[jt[u]];
+#pragma unroll
+            for (int u = 0; u < 8; u++)
+                if (k + u * tpp < m) acc.add(pi, ti, q[u], R);
+        }
         acc.result(Fx, Fy, Fz, R);
         v_pe = acc.pe;
         v_vir = acc.vir;
