@@ -540,6 +540,40 @@ def test_server_preempted_job_resumes_exactly(gpu):
     srv.close()
 
 
+def test_server_preempted_job_resumes_in_another_group(gpu):
+    """A job preempted in one launch group and readmitted into a slot of the
+    other (its old slot taken meanwhile) finishes bitwise as when it runs
+    uninterrupted in a one-group server of the same slots."""
+    caps = [2048, 2048]
+    ov = (200, 4, 6, 0.3)
+    a_job = (1500, 0.5, 1.0, 1.0, 700, 21, 0)
+    alone, aid = _run_server(gpu, caps, [a_job], overlap=ov, groups=1)
+    ref, ref_q = alone.result(aid[0]), alone.job_overlap(aid[0])
+    alone.close()
+    srv = gpu.Server(caps, precision=gpu.FP32, chunk=100, relax=200, overlap=ov, groups=2)
+    ia = srv.submit(*a_job)
+    for t in (0.0, 1.0, 2.0):
+        srv.tick(t)
+    first = gpu.lib.ac4_server_job_slot(srv.h, ia)
+    srv.preempt(ia)
+    ib = srv.submit(1800, 0.6, 0.5, 0.1, 500, 22, 3.0)  # takes the freed slot
+    srv.tick(3.0)
+    assert gpu.lib.ac4_server_job_slot(srv.h, ib) == first
+    second = gpu.lib.ac4_server_job_slot(srv.h, ia)
+    assert second >= 0 and second != first  # readmitted into the other group's slot
+    t = 4.0
+    while srv.pending() and t < 200:
+        srv.tick(t)
+        t += 1.0
+    assert srv.job(ia)["state"] == 2 and srv.job(ib)["state"] == 2
+    got = srv.result(ia)
+    for u in range(3):
+        assert np.array_equal(got[u], ref[u]), u
+    for u, (a, b) in enumerate(zip(srv.job_overlap(ia), ref_q)):
+        assert np.array_equal(a, b), u
+    srv.close()
+
+
 def test_insitu_overlap_accumulators(gpu):
     specs = [(1000, 71, 1), (1600, 72, 2)]
     md = make_relaxed_md(gpu, specs, tau=20, prec=1)
