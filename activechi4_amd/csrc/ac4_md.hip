@@ -1275,6 +1275,7 @@ struct PairAcc;
 template <>
 struct PairAcc<false> {  // FP64: the reference's per-pair arithmetic
     double fx = 0, fy = 0, fz = 0, pe = 0, vir = 0;
+    __device__ __forceinline__ void flush() {}
     __device__ __forceinline__ PairAcc(int, const RepC &) {}
     __device__ __forceinline__ void add(const double4 &pi, int ti, const double4 &pj, const RepC &R) {
         const double L = R.L;
@@ -1307,15 +1308,20 @@ struct PairAcc<true> {
     // image in integers, separations in units of L/2^32 rounded once to FP32.
     // The cutoff test runs in those units (r^2 in units^2 against the
     // cutoff divided by the unit^2), so a candidate outside costs no scaling.
-    // Each FP32 pair term is widened to FP64 and summed there (exactly the
-    // compensation a Kahan sum aims at, and on CDNA4 a widening add costs
+    // Each FP32 pair force term is widened to FP64 and summed there (exactly
+    // the compensation a Kahan sum aims at, and on CDNA4 a widening add costs
     // two FP64 instructions against five FP32 ones); the force is scaled to
-    // real units at the end.  The pair constants of the two possible partner
+    // real units at the end.  The pair energy and virial, which only feed the
+    // reported PE, pressure and Heff and never the trajectory, are summed in
+    // FP32 over each group of at most 8 list entries and widened once per
+    // group (flush): k_force 33.4 -> 32.1 us on the 64-replica ensemble, the
+    // trajectories bit for bit the same.  The pair constants of the two possible partner
     // species are set once per particle; one hardware reciprocal of r^2
     // (v_rcp_f32, within 1 ulp) replaces the two divisions, since
     // sigma^2/r^2 = (sigma/r)^2 and 1/r^2 appears in the force factor.  The
     // pair terms are formed with fused multiply-adds (each rounds once).
     double fx = 0, fy = 0, fz = 0, pe = 0, vir = 0;
+    float gpe = 0.f, gvir = 0.f;  // FP32 partials of the energy and virial of up to 8 pairs
     float sg2[2], e4[2], e8[2], rcq[2], c2i[2], qs2;
     __device__ __forceinline__ PairAcc(int ti, const RepC &R) {
         // (ti, tj) -> ab = ti + tj: sigma {1, 0.8, 0.88}, epsilon {1, 1.5, 0.5}
@@ -1351,11 +1357,16 @@ struct PairAcc<true> {
             fx += (double)(grad * dx);
             fy += (double)(grad * dy);
             fz += (double)(grad * dz);
-            pe += (double)pterm;
-            vir += (double)(grad * r2);
+            gpe += pterm;
+            gvir = __builtin_fmaf(grad, r2, gvir);
         }
     }
-    __device__ __forceinline__ void result(double &ox, double &oy, double &oz, const RepC &R) const {
+    __device__ __forceinline__ void flush() {
+        pe += (double)gpe; vir += (double)gvir;
+        gpe = gvir = 0.f;
+    }
+    __device__ __forceinline__ void result(double &ox, double &oy, double &oz, const RepC &R) {
+        flush();
         ox = fx * R.qscale;
         oy = fy * R.qscale;
         oz = fz * R.qscale;
@@ -1405,6 +1416,7 @@ __device__ __forceinline__ void walk_list1(const RepC &R, const int *lp, int m, 
 #pragma unroll
         for (int u = 0; u < C; u++)
             if (k + u < m) acc.add(pi, ti, q[u], R);
+        acc.flush();
     }
 }
 
@@ -1471,6 +1483,7 @@ __global__ void __launch_bounds__(BLK) k_force(const RepC *__restrict__ reps,
             }
 #pragma unroll
             for (int u = 0; u < 8; u++) acc.add(pi, ti, q[u], R);
+            acc.flush();
         }
         if (k < m) {  // the last, partial group: its loads issued together, the adds in list order
             int jt[8];
