@@ -101,7 +101,7 @@ int ac4_md_calculate_forces(ac4_md *md, double *out2);
  * time_evolve.c:508); criterion = 1 selects the reference's 3*maxD^2.
  * Requires forces (ac4_md_calculate_forces or a previous step). */
 int ac4_md_step(ac4_md *md, int nsteps);
-/* Same as ac4_md_step but replays captured hipGraphs: ten steps per graph
+/* Same as ac4_md_step but replays captured hipGraphs: 25 steps per graph
  * launch, then one-step graphs for the rest of nsteps (batched serving path);
  * the result equals ac4_md_step's bit for bit. */
 int ac4_md_step_graph(ac4_md *md, int nsteps);

@@ -344,8 +344,8 @@ def test_checkpoint_restore_is_exact(gpu, prec):
 
 
 def test_graph_replay_equals_eager(gpu):
-    """Graph replays (chunks of ten captured steps, then one-step graphs for
-    the rest: 57 = 5 x 10 + 7, 43 = 4 x 10 + 3) equal eager launches."""
+    """Graph replays (chunks of 25 captured steps, then one-step graphs for
+    the rest: 57 = 2 x 25 + 7, 43 = 25 + 18) equal eager launches."""
     spec = [(1200, 41, 1), (2000, 42, 2)]
     e = make_relaxed_md(gpu, spec, tau=25, prec=1)
     g = make_relaxed_md(gpu, spec, tau=25, prec=1)
