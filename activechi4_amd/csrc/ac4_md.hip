@@ -56,7 +56,8 @@ struct RepC {
     uint32_t key0, key1;
     double qscale;  // L / 2^32: real length of one unit of a fixed-point coordinate
     double skin;    // Verlet skin: list radii 2.5 sigma_ab + skin, rebuild criterion
-    float qs2, pad3; // qscale^2 in FP32
+    float qs2;      // qscale^2 in FP32
+    int xcull;      // list teams skip the far x-half of the cells beside along x (configure_rep)
 };
 
 // Fixed-point copy of a wrapped position for the FP32 force path and the
@@ -488,9 +489,9 @@ __device__ __forceinline__ int clampc(int c, int n) { return min(max(c, 0), n - 
 // an atomic on the cell's count; the particle's (slot, key) is written to
 // that place of its cell's row of CELL_CAP entries (the reference's
 // InCell[cell][MaxPartInCell], MDcode/global.h:20,85; a fuller cell sets the
-// capacity error).  key: the octant of the cell the particle is in, then its
-// id, the order k_cell_sort puts the cell in.  k_bin_end then turns the cell
-// counts into start offsets.
+// capacity error).  key: the octant of the cell the particle is in (x bit
+// highest), then its id, the order k_cell_sort puts the cell in.  k_bin_end
+// then turns the cell counts into start offsets.
 #define CELL_CAP 600
 __global__ void __launch_bounds__(BLK) k_bin(const RepC *__restrict__ reps, ChainS *chain,
                                              const int *__restrict__ blk_rep, const int *__restrict__ blk_first,
@@ -516,10 +517,14 @@ __global__ void __launch_bounds__(BLK) k_bin(const RepC *__restrict__ reps, Chai
         c = cx + cy * R.nc + cz * R.nc * R.nc;
         cell_of[g] = c;
         // octant within the cell: the parity of the cell index on the grid
-        // of half-width cells (n < 2^21, so the id fits below bit 21)
-        const int ox = (int)floor(sx * (2 * R.nc)) & 1, oy = (int)floor(sy * (2 * R.nc)) & 1,
-                  oz = (int)floor(sz * (2 * R.nc)) & 1;
-        key = ((ox | (oy << 1) | (oz << 2)) << 21) | id[g];
+        // of half-width cells, clamped as the cell index is (s * 2nc is
+        // exactly twice s * nc, so the half agrees with the cell); the x bit
+        // highest, so each x-half of a cell is one slot range (k_cell_sort).
+        // n < 2^21, so the id fits below bit 21.
+        const int ox = clampc((int)floor(sx * (2 * R.nc)), 2 * R.nc) & 1,
+                  oy = clampc((int)floor(sy * (2 * R.nc)), 2 * R.nc) & 1,
+                  oz = clampc((int)floor(sz * (2 * R.nc)), 2 * R.nc) & 1;
+        key = (((ox << 2) | (oy << 1) | oz) << 21) | id[g];
     }
     // one returning atomic per particle, all in flight at once: the place
     // within the cell row depends on the order the atomics land in, which
@@ -658,7 +663,7 @@ __device__ __forceinline__ void gather_one(const PartArrays &src, const PartArra
     pc.zo[gn] = uz;
 }
 
-// Members of a cell by octant of the cell (x, y, z bits, z highest), and in
+// Members of a cell by octant of the cell (z, y, x bits, x highest), and in
 // increasing original index within an octant (the binning order of
 // cell_list+forces.c:71-80): a canonical order, so lists and force sums do not
 // depend on launch timing or on which replicas share the launch.  Consecutive
@@ -666,13 +671,14 @@ __device__ __forceinline__ void gather_one(const PartArrays &src, const PartArra
 // a force gather touch fewer cache lines.  One wave per cell: keys staged in
 // LDS, each lane ranks its entries by counting and moves each of its particles
 // to its new slot (start of the cell + rank) in dst, so consecutive ranks
-// write consecutive slots.
+// write consecutive slots.  xmid: the first slot of the cell's upper x-half.
 #define SORT_CAP 256
+#define XHALF (4 << 21)  // keys from here on: the upper x-half of the cell
 __global__ void __launch_bounds__(BLK) k_cell_sort(const RepC *__restrict__ reps, WorkList wl,
                                                    const int *__restrict__ cnt, const int *__restrict__ start,
                                                    int2 *__restrict__ cell_mem,
-                                                   int *members, int2 *ranges, PartArrays src, PartArrays dst,
-                                                   PosCopies pc) {
+                                                   int *members, int2 *ranges, int *xmid, PartArrays src,
+                                                   PartArrays dst, PosCopies pc) {
     __shared__ int s_m[BLK / 64][SORT_CAP];
     __shared__ int s_k[BLK / 64][SORT_CAP];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -708,6 +714,9 @@ __global__ void __launch_bounds__(BLK) k_cell_sort(const RepC *__restrict__ reps
         if (n <= SORT_CAP) {
             for (int k = lane; k < n; k += 64) { const int2 v = row[k]; s_m[w][k] = v.x; s_k[w][k] = v.y; }
             __syncwarp();
+            int lo = 0;
+            for (int k0 = 0; k0 < n; k0 += 64) lo += __popcll(__ballot(k0 + lane < n && s_k[w][k0 + lane] < XHALF));
+            if (lane == 0) xmid[R.off_cell + c] = R.off + first + lo;
             for (int k = lane; k < n; k += 64) {
                 const int v = s_k[w][k];
                 int rank = 0;
@@ -727,6 +736,9 @@ __global__ void __launch_bounds__(BLK) k_cell_sort(const RepC *__restrict__ reps
             }
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");  // lane 0's order, read by every lane
             __syncwarp();
+            int lo = 0;
+            for (int k0 = 0; k0 < n; k0 += 64) lo += __popcll(__ballot(k0 + lane < n && row[k0 + lane].y < XHALF));
+            if (lane == 0) xmid[R.off_cell + c] = R.off + first + lo;
             for (int k = lane; k < n; k += 64) {
                 const int sl = row[k].x;
                 m[k] = sl;
@@ -889,20 +901,29 @@ __global__ void __launch_bounds__(BLK) k_nlist(const RepC *__restrict__ reps, Ch
 // boundaries: about half the dependent rounds of a range-by-range walk, the
 // same candidates in the same order.  A stencil of more than CH_CAP chunks
 // (crowded cells, narrow teams) is walked range by range.
+//
+// Where the replica allows it (RepC::xcull), the range holding the cell
+// beside the home cell along x on the far side loses that cell's far x-half:
+// its start moves to the cell's xmid for a particle in the upper x-half of
+// its own cell (the cell below), its end for one in the lower half (the cell
+// above).  Those candidates are at least one cell width away, beyond every
+// list radius, so the lists are the same pairs in the same order; each of the
+// 9 rows loses half a cell, 1/6 of the candidates.
 #define CH_CAP 64
 #define CH_PER 4
 template <int NLT>
 __device__ __forceinline__ void nlist_team_one(const RepC &R, int r, int i, ChainS *chain,
                                                const QPos *__restrict__ posq, const int *__restrict__ cell_of,
-                                               const int2 *__restrict__ ranges, int *nn, int *nl, int cap,
-                                               int2 *ct) {
+                                               const int2 *__restrict__ ranges,
+                                               const int *__restrict__ xmid, int *nn, int *nl, int cap, int2 *ct) {
     const int lane = threadIdx.x & 63, t = lane & (NLT - 1);
     const unsigned long long tmask = (NLT == 64 ? ~0ull : ((1ull << (NLT & 63)) - 1ull)) << (lane & ~(NLT - 1));
     const unsigned long long below = (1ull << lane) - 1ull;
     const int g = R.off + i;
     const QPos pi = posq[g];
     const int ti = ptype(pi);
-    const int2 *rgs = ranges + (size_t)(R.off_cell + cell_of[g]) * NRANGE;
+    const int cg = cell_of[g];
+    const int2 *rgs = ranges + (size_t)(R.off_cell + cg) * NRANGE;
     const float ua = qscreen_cut(ti, R), ub = qscreen_cut(ti + 1, R);
     int count = 0;
     auto inlist = [&](const QPos &q) -> bool { return qscreen(pi, q, ua, ub); };
@@ -943,6 +964,24 @@ __device__ __forceinline__ void nlist_team_one(const RepC &R, int r, int i, Chai
     int2 ra = make_int2(0, 0), rb = make_int2(0, 0);
     if (t < NRANGE) ra = rgs[t];
     if (NLT < NRANGE && t + NLT < NRANGE) rb = rgs[t + NLT];
+    if (R.xcull) {
+        const int nc = R.nc, cx = cg % nc, cy = (cg / nc) % nc, cz = cg / (nc * nc);
+        const bool up = g >= xmid[R.off_cell + cg];
+        // the cell whose far half goes, and the part of its row's runs
+        // holding it (xruns: part 1 where the home cell is at either end)
+        const int xc = up ? (cx == 0 ? nc - 1 : cx - 1) : (cx == nc - 1 ? 0 : cx + 1);
+        const int qp = (!up && (cx == 0 || cx == nc - 1)) ? 1 : 0;
+        auto cull = [&](int q, int2 &rg) {
+            if (q >= NRANGE || (q & 1) != qp) return;
+            const int row = q >> 1, iz = row / 3, iy = row - 3 * iz;
+            const int yy = (cy + iy + nc - 1) % nc, zz = (cz + iz + nc - 1) % nc;
+            const int m = xmid[R.off_cell + xc + yy * nc + zz * nc * nc];
+            if (up) rg.x = m;
+            else rg.y = m;
+        };
+        cull(t, ra);
+        if (NLT < NRANGE) cull(t + NLT, rb);
+    }
     const int na = (ra.y - ra.x + NLT - 1) / NLT, nb = (rb.y - rb.x + NLT - 1) / NLT;
     int pa = na, pb = nb;  // inclusive scans over the team
     for (int o = 1; o < NLT; o <<= 1) {
@@ -1000,7 +1039,8 @@ __global__ void __launch_bounds__(NLT * LPB) k_nlist_team(const RepC *__restrict
                                                           const int *__restrict__ lblk_first, WorkList wl,
                                                           const QPos *__restrict__ posq,
                                                           const int *__restrict__ cell_of,
-                                                          const int2 *__restrict__ ranges, int *nn, int *nl, int cap,
+                                                          const int2 *__restrict__ ranges,
+                                                          const int *__restrict__ xmid, int *nn, int *nl, int cap,
                                                           void *const *__restrict__ ftab) {
     __shared__ int2 s_ct[LPB][CH_CAP + CH_PER];  // each team's chunk list, then empty chunks
     const int nit = wl.n[1];
@@ -1022,7 +1062,7 @@ __global__ void __launch_bounds__(NLT * LPB) k_nlist_team(const RepC *__restrict
             cv = cf < 9 ? ((const long long *)ftab[PART_FIELDS + cf])[cg] : ((const int *)ftab[PART_FIELDS + cf])[cg];
         const int i = lblk_first[b] + (int)threadIdx.x / NLT;
         if (i < R.n)  // whole team
-            nlist_team_one<NLT>(R, r, i, chain, posq, cell_of, ranges, nn, nl, cap, s_ct[threadIdx.x / NLT]);
+            nlist_team_one<NLT>(R, r, i, chain, posq, cell_of, ranges, xmid, nn, nl, cap, s_ct[threadIdx.x / NLT]);
         if (cpy) {
             if (cf < 9) ((long long *)ftab[cf])[cg] = cv;
             else ((int *)ftab[cf])[cg] = (int)cv;
@@ -1648,6 +1688,7 @@ struct ac4_md {
     bool shuf_fused = false;  // the shuffle runs in k_first_half's first workgroups
     int *wl_item = nullptr, *wl_n = nullptr;  // rebuild work list (WorkList)
     int2 *ranges = nullptr;                    // per cell: stencil slot ranges (NRANGE)
+    int *xmid = nullptr;                       // per cell: first slot of its upper x-half
     int *lblk_rep = nullptr, *lblk_first = nullptr;  // list-build block table (LPB particles per block)
     double *partial = nullptr;
     PartArrays P{}, A{};  // primary and permutation scratch
@@ -1763,6 +1804,11 @@ static const char *configure_rep(RepC &R, const ac4_replica_params &q) {
     if (R.nc_ref < 1) R.nc_ref = 1;
     R.inv_cell_ref = R.L / (Lpp / (double)R.nc_ref);
     R.nc = (int)(R.L / cell_cut);  // list grid: cells at least one list radius (and 2.85) wide
+    // A particle in the upper (lower) x-half of its cell is at least one cell
+    // width from the lower (upper) x-half of the cell below (above) it along
+    // x, for any grid of three or more cells: beyond every list radius and the
+    // screen's margin once the width exceeds the largest radius by 0.01 sigma.
+    R.xcull = R.nc >= 3 && R.L / R.nc >= 2.5 + R.skin + 0.01;
     R.ncell = R.nc * R.nc * R.nc;
     if (R.ncell > R.cap_cells) return "density below the slot's cell capacity (0.5)";
     R.qscale = R.L / 4294967296.0;
@@ -1902,7 +1948,7 @@ extern "C" int ac4_md_create_slots_rule(int nslots, const int *cap_n, int precis
     }
     ALLOC(reps, nslots) ALLOC(chain, nslots) ALLOC(blk_rep, md->nblk) ALLOC(blk_first, md->nblk) ALLOC(blk_idx, md->nblk)
     ALLOC(fblk_rep, md->fnblk) ALLOC(fblk_first, md->fnblk) ALLOC(lblk_rep, md->lnblk) ALLOC(lblk_first, md->lnblk)
-    ALLOC(wl_item, (size_t)md->nblk + md->lnblk + md->ncb) ALLOC(wl_n, 4) ALLOC(ranges, (size_t)off_cell * NRANGE) ALLOC(cblk_rep, off_cell) ALLOC(cblk_cell, off_cell) ALLOC(partial, 4 * (size_t)md->fnblk)
+    ALLOC(wl_item, (size_t)md->nblk + md->lnblk + md->ncb) ALLOC(wl_n, 4) ALLOC(ranges, (size_t)off_cell * NRANGE) ALLOC(xmid, off_cell) ALLOC(cblk_rep, off_cell) ALLOC(cblk_cell, off_cell) ALLOC(partial, 4 * (size_t)md->fnblk)
     ALLOC(fx, off) ALLOC(fy, off) ALLOC(fz, off) ALLOC(xo, off) ALLOC(yo, off) ALLOC(zo, off) ALLOC(pos4, off) ALLOC(posq, off)
     ALLOC(cell_mem, (size_t)off_cell * CELL_CAP) ALLOC(nn, off) ALLOC(nn_ref, off) ALLOC(nl, (size_t)md->cap * off) ALLOC(cnt, off_cell) ALLOC(start, off_cell)
     ALLOC(members, off) ALLOC(kx, off_act) ALLOC(ky, off_act) ALLOC(kz, off_act)
@@ -2050,7 +2096,7 @@ extern "C" int ac4_md_destroy(ac4_md *md) {
     if (md->stream) (void)hipStreamSynchronize(md->stream);
     drop_graphs(md);
     void *ptrs[] = {md->reps, md->chain, md->blk_rep, md->blk_first, md->blk_idx, md->fblk_rep, md->fblk_first,
-                    md->lblk_rep, md->lblk_first, md->wl_item, md->wl_n, md->ranges,
+                    md->lblk_rep, md->lblk_first, md->wl_item, md->wl_n, md->ranges, md->xmid,
                     md->cblk_rep, md->cblk_cell, md->ox, md->oy, md->oz, md->ov_q, md->ov_q2,
                     md->ov_step, md->ov_cnt, md->ov_n, md->ov_list, md->ov_out,
                     md->partial, md->fx, md->fy, md->fz, md->xo, md->yo, md->zo, md->pos4, md->posq, md->cell_mem, md->shuf_ent, md->shuf_ent_n, md->nn,
@@ -2086,7 +2132,7 @@ extern "C" long long ac4_md_slot_bytes(int cap_n, int n_origin, int n_lags) {
     b += n * (6 * 8 + 32 + 12 + 2 * 4 + 192LL * 4 + 4 + 3);         // forces, xold, pos copies, counts, list, members, k
     b += 2 * n * (9 * 8 + 5 * 4);                                    // particle arrays and their scratch copy
     b += n * (3 * 8 + 4);                                            // reference-order export
-    b += cells * ((long long)NRANGE * 8 + 2 * 4 + (long long)CELL_CAP * 8 + 2 * 4);
+    b += cells * ((long long)NRANGE * 8 + 3 * 4 + (long long)CELL_CAP * 8 + 2 * 4);
     b += nblk * 3 * 4 + fnb * 2 * 4 + lnblk * 2 * 4;                // block tables
     b += (nblk + lnblk + ncb) * 4 + fnb * 4 * 8;                     // work list, force partials
     b += (long long)sizeof(RepC) + (long long)sizeof(ChainS);
@@ -2631,7 +2677,7 @@ static int launch_rebuild(ac4_md *md, bool with_zero = false) {
     hipLaunchKernelGGL(k_bin_end, dim3(md->R), dim3(md->max_ncell > BIN_END_T ? BIN_END_T : BLK), 0, s, md->reps, md->chain,
                        md->cnt, md->start, wl);
     hipLaunchKernelGGL(k_cell_sort, dim3(gc), dim3(BLK), 0, s, md->reps, wl, md->cnt, md->start,
-                       md->cell_mem, md->members, md->ranges, md->P, md->A,
+                       md->cell_mem, md->members, md->ranges, md->xmid, md->P, md->A,
                        PosCopies{md->pos4, md->posq, md->xo, md->yo, md->zo});
     // 16- and 32-lane list teams of handles up to TEAM_MOVE_N particles move
     // the permuted arrays back themselves (measured: 64 replicas, 1.5e5
@@ -2650,7 +2696,7 @@ static int launch_rebuild(ac4_md *md, bool with_zero = false) {
                             : (w == 32 ? (team_moves ? k_nlist_team<32, true> : k_nlist_team<32, false>)
                                        : k_nlist_team<64, false>);
         hipLaunchKernelGGL(kern, dim3(gl), dim3(w * LPB), 0, s, md->reps, md->chain, md->lblk_rep, md->lblk_first, wl,
-                           md->posq, md->A.cell_of, md->ranges, md->nn, md->nl, md->cap, md->ftab);
+                           md->posq, md->A.cell_of, md->ranges, md->xmid, md->nn, md->nl, md->cap, md->ftab);
     }
     else if (md->nlist_mode == 0)
         hipLaunchKernelGGL(k_nlist_cell, dim3(md->ncell_tot), dim3(64), 0, s, md->reps, md->chain, md->cblk_rep,

@@ -709,6 +709,9 @@ def main():
     ap.add_argument("--tpp", type=int, default=0,
                     help="force lanes per particle for --no-tune and --pmc-probe runs (0: automatic), e.g. to "
                          "profile the shape the autotuner chose")
+    ap.add_argument("--skin", type=float, default=0.0,
+                    help="Verlet skin for --no-tune and --pmc-probe runs (0: the reference's 0.35), e.g. to "
+                         "profile the shape the autotuner chose")
     ap.add_argument("--eager", action="store_true",
                     help="timed steps as eager launches instead of graph replays (for kernel-trace profiles)")
     a = ap.parse_args()
@@ -720,6 +723,8 @@ def main():
     ac4.require_gpu()
     if a.tpp:
         ac4.set_tuning(a.tpp, 3)
+    if a.skin:
+        ac4.set_skin(a.skin)
     if a.pmc_probe:
         pmc_probe(ac4)
         return
