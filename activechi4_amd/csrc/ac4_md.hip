@@ -490,7 +490,8 @@ __device__ __forceinline__ int clampc(int c, int n) { return min(max(c, 0), n - 
 // that place of its cell's row of CELL_CAP entries (the reference's
 // InCell[cell][MaxPartInCell], MDcode/global.h:20,85; a fuller cell sets the
 // capacity error).  key: the octant of the cell the particle is in (x bit
-// highest), then its id, the order k_cell_sort puts the cell in.  k_bin_end
+// highest where the list teams cull, else z), then its id, the order
+// k_cell_sort puts the cell in.  k_bin_end
 // then turns the cell counts into start offsets.
 #define CELL_CAP 600
 __global__ void __launch_bounds__(BLK) k_bin(const RepC *__restrict__ reps, ChainS *chain,
@@ -518,13 +519,15 @@ __global__ void __launch_bounds__(BLK) k_bin(const RepC *__restrict__ reps, Chai
         cell_of[g] = c;
         // octant within the cell: the parity of the cell index on the grid
         // of half-width cells, clamped as the cell index is (s * 2nc is
-        // exactly twice s * nc, so the half agrees with the cell); the x bit
-        // highest, so each x-half of a cell is one slot range (k_cell_sort).
+        // exactly twice s * nc, so the half agrees with the cell).  Where the
+        // list teams cull (RepC::xcull) the x bit is highest, so each x-half
+        // of a cell is one slot range (k_cell_sort); elsewhere the z bit (one
+        // N=10^6 box: 2.48e9 against 2.45e9 particle-steps/s with x highest).
         // n < 2^21, so the id fits below bit 21.
         const int ox = clampc((int)floor(sx * (2 * R.nc)), 2 * R.nc) & 1,
                   oy = clampc((int)floor(sy * (2 * R.nc)), 2 * R.nc) & 1,
                   oz = clampc((int)floor(sz * (2 * R.nc)), 2 * R.nc) & 1;
-        key = (((ox << 2) | (oy << 1) | oz) << 21) | id[g];
+        key = ((R.xcull ? ((ox << 2) | (oy << 1) | oz) : (ox | (oy << 1) | (oz << 2))) << 21) | id[g];
     }
     // one returning atomic per particle, all in flight at once: the place
     // within the cell row depends on the order the atomics land in, which
@@ -663,15 +666,16 @@ __device__ __forceinline__ void gather_one(const PartArrays &src, const PartArra
     pc.zo[gn] = uz;
 }
 
-// Members of a cell by octant of the cell (z, y, x bits, x highest), and in
-// increasing original index within an octant (the binning order of
-// cell_list+forces.c:71-80): a canonical order, so lists and force sums do not
+// Members of a cell by octant of the cell (x bit highest where the list teams
+// cull, else z; k_bin), and in increasing original index within an octant
+// (the binning order of cell_list+forces.c:71-80): a canonical order, so lists and force sums do not
 // depend on launch timing or on which replicas share the launch.  Consecutive
 // slots are then nearer in space than in plain index order, so the 64 lanes of
 // a force gather touch fewer cache lines.  One wave per cell: keys staged in
 // LDS, each lane ranks its entries by counting and moves each of its particles
 // to its new slot (start of the cell + rank) in dst, so consecutive ranks
-// write consecutive slots.  xmid: the first slot of the cell's upper x-half.
+// write consecutive slots.  xmid: the first slot of the cell's upper x-half
+// (used only where the x bit is highest).
 #define SORT_CAP 256
 #define XHALF (4 << 21)  // keys from here on: the upper x-half of the cell
 __global__ void __launch_bounds__(BLK) k_cell_sort(const RepC *__restrict__ reps, WorkList wl,
@@ -911,7 +915,7 @@ __global__ void __launch_bounds__(BLK) k_nlist(const RepC *__restrict__ reps, Ch
 // 9 rows loses half a cell, 1/6 of the candidates.
 #define CH_CAP 64
 #define CH_PER 4
-template <int NLT>
+template <int NLT, bool CULL>
 __device__ __forceinline__ void nlist_team_one(const RepC &R, int r, int i, ChainS *chain,
                                                const QPos *__restrict__ posq, const int *__restrict__ cell_of,
                                                const int2 *__restrict__ ranges,
@@ -964,7 +968,7 @@ __device__ __forceinline__ void nlist_team_one(const RepC &R, int r, int i, Chai
     int2 ra = make_int2(0, 0), rb = make_int2(0, 0);
     if (t < NRANGE) ra = rgs[t];
     if (NLT < NRANGE && t + NLT < NRANGE) rb = rgs[t + NLT];
-    if (R.xcull) {
+    if (CULL && R.xcull) {
         const int nc = R.nc, cx = cg % nc, cy = (cg / nc) % nc, cz = cg / (nc * nc);
         const bool up = g >= xmid[R.off_cell + cg];
         // the cell whose far half goes, and the part of its row's runs
@@ -1033,7 +1037,11 @@ __device__ __forceinline__ void nlist_team_one(const RepC &R, int r, int i, Chai
 // Fixed grid over the step's list blocks (LPB particles each, one team per
 // particle); plain block order spreads the few rebuilding replicas over all
 // XCDs.
-template <int NLT, bool MOVE>
+// CULL: the x-half culling compiled in (handles with a replica that allows
+// it); without it the kernel keeps the registers of the plain walk (N=10^6:
+// 2.45e9 against 2.55e9 particle-steps/s with the culling code present but
+// unused).
+template <int NLT, bool MOVE, bool CULL>
 __global__ void __launch_bounds__(NLT * LPB) k_nlist_team(const RepC *__restrict__ reps, ChainS *chain,
                                                           const int *__restrict__ lblk_rep,
                                                           const int *__restrict__ lblk_first, WorkList wl,
@@ -1062,7 +1070,7 @@ __global__ void __launch_bounds__(NLT * LPB) k_nlist_team(const RepC *__restrict
             cv = cf < 9 ? ((const long long *)ftab[PART_FIELDS + cf])[cg] : ((const int *)ftab[PART_FIELDS + cf])[cg];
         const int i = lblk_first[b] + (int)threadIdx.x / NLT;
         if (i < R.n)  // whole team
-            nlist_team_one<NLT>(R, r, i, chain, posq, cell_of, ranges, xmid, nn, nl, cap, s_ct[threadIdx.x / NLT]);
+            nlist_team_one<NLT, CULL>(R, r, i, chain, posq, cell_of, ranges, xmid, nn, nl, cap, s_ct[threadIdx.x / NLT]);
         if (cpy) {
             if (cf < 9) ((long long *)ftab[cf])[cg] = cv;
             else ((int *)ftab[cf])[cg] = (int)cv;
@@ -2692,9 +2700,14 @@ static int launch_rebuild(ac4_md *md, bool with_zero = false) {
                            md->nl, md->stride, md->cap);
     else if (md->nlist_mode >= 2) {
         const int w = md->nlist_mode == 2 ? 16 : (md->nlist_mode == 3 ? 32 : 64);
-        auto kern = w == 16 ? (team_moves ? k_nlist_team<16, true> : k_nlist_team<16, false>)
-                            : (w == 32 ? (team_moves ? k_nlist_team<32, true> : k_nlist_team<32, false>)
-                                       : k_nlist_team<64, false>);
+        bool cull = false;  // any replica whose list teams may cull (the kernel checks each replica)
+        for (int r = 0; r < md->R; r++) cull |= md->reps_h[r].n > 0 && md->reps_h[r].xcull;
+        auto kern = cull ? (w == 16 ? (team_moves ? k_nlist_team<16, true, true> : k_nlist_team<16, false, true>)
+                                    : (w == 32 ? (team_moves ? k_nlist_team<32, true, true> : k_nlist_team<32, false, true>)
+                                               : k_nlist_team<64, false, true>))
+                         : (w == 16 ? (team_moves ? k_nlist_team<16, true, false> : k_nlist_team<16, false, false>)
+                                    : (w == 32 ? (team_moves ? k_nlist_team<32, true, false> : k_nlist_team<32, false, false>)
+                                               : k_nlist_team<64, false, false>));
         hipLaunchKernelGGL(kern, dim3(gl), dim3(w * LPB), 0, s, md->reps, md->chain, md->lblk_rep, md->lblk_first, wl,
                            md->posq, md->A.cell_of, md->ranges, md->xmid, md->nn, md->nl, md->cap, md->ftab);
     }
